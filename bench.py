@@ -1,0 +1,191 @@
+"""Throughput of the grid-kriging hot path on one MI355X.
+
+Workload: ordinary kriging of a 100x100x50 SugarboxGrid (500,000 cells, 5%
+seeded hard data, exponential model ranges (20,20,4), sill 1, nugget 0.05,
+radiuses (20,20,4), max_neighbours 24) -- synthetic data from
+geo_bsd.synthetic.ok_c2.  One step = one complete pass of the hot path
+(mask padding, cell hash, neighbour search, covariance assembly, Cholesky,
+solve, combine, variance) over the grid, with inputs resident in HBM.
+
+The metric is kriged cells/s of the FP32-Cholesky + FP64-refinement solver
+("mixed"); the plain FP64 solver is timed the same way and reported beside
+it, together with both solvers' largest estimate error against the FP64 CPU
+oracle.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hpgl-reborn_amd"))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64
+PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (packed)
+
+
+def flops_ok(n):
+    """Algorithmic flops of one OK cell with n neighbours: covariance evaluations
+    (25 flop each) + Cholesky + two solves + tail."""
+    import numpy as np
+    n = n.astype(np.float64)
+    e = n * (n + 1) / 2 + n + 1
+    return 25 * e + n ** 3 / 3 + n ** 2 / 2 + n / 6 + 4 * n ** 2 + 10 * n
+
+
+def run_steps(D, wl, model, v, m, precision, steps, torch):
+    out = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=precision, variance=True)
+    torch.cuda.synchronize()
+    return out
+
+
+def cpu_baseline(wl):
+    """The reference's own OpenMP CPU path (oracle/_ref, compiled from its
+    sources) on the same workload, or the oracle when that build is absent."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, 16))
+    cov = O.make_cov(wl.cov["type"], wl.cov["ranges"], wl.cov["angles"], wl.cov["sill"], wl.cov["nugget"])
+    kriged = int((wl.mask == 0).sum())
+    times = []
+    if O.ref_available():
+        kind = "reference"
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.ref_ordinary_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, threads=threads)
+            times.append(time.perf_counter() - t0)
+    else:
+        kind = "port"
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.krige(0, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, want_neighbours=False)
+            times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    return {"value": kriged / t, "unit": "kriged cells/s", "cores": threads, "kind": kind,
+            "sample": f"full 100x100x50 OK grid ({kriged} kriged cells), median of 3 runs, {threads} OpenMP threads"}
+
+
+def oracle_est64(wl):
+    from oracle import oracle as O
+    cov = O.make_cov(wl.cov["type"], wl.cov["ranges"], wl.cov["angles"], wl.cov["sill"], wl.cov["nugget"])
+    return O.krige(0, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, want_neighbours=False)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the kriging path runs on a "
+              "single GPU; see DESIGN.md for how it would shard).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from geo_bsd import CovarianceModel, device as D, synthetic as S
+
+    wl = S.ok_c2()
+    model = CovarianceModel(**wl.cov)
+    dev = torch.device("cuda:0")
+    v = torch.from_numpy(wl.values).to(dev)
+    m = torch.from_numpy(wl.mask).to(dev)
+    kriged = int((wl.mask == 0).sum())
+    stream = torch.cuda.current_stream(dev)
+
+    # Neighbour counts for the algorithmic flop count (untimed).
+    nb = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, neighbours=True, variance=False)
+    torch.cuda.synchronize()
+    counts = nb["count"].cpu().numpy()
+    counts = counts[counts > 0]
+    flops_per_pass = float(flops_ok(counts).sum())
+
+    results = {}
+    for prec in ("fp64", "mixed"):
+        for _ in range(args.warmup):
+            D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.steps):
+            out = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = ev0.elapsed_time(ev1) / args.steps
+        # Per-kernel durations: HIP events recorded by the library around the
+        # search and the solve kernels on the launch stream.
+        D.set_profiling(True)
+        search, solve = [], []
+        for _ in range(max(5, min(args.steps, 20))):
+            D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
+            s_ms, v_ms, _ = D.last_timing()
+            search.append(s_ms)
+            solve.append(v_ms)
+        D.set_profiling(False)
+        stats = D.last_call_stats()
+        est = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, est64=True,
+                      status=True)
+        torch.cuda.synchronize()
+        results[prec] = dict(ms=ms, wall_ms=wall * 1e3 / args.steps, search_ms=statistics.mean(search),
+                             solve_ms=statistics.mean(solve), est64=est["est64"].cpu().numpy(),
+                             status=est["status"].cpu().numpy(), fallbacks=stats["fp64_fallbacks"])
+
+    err = {}
+    if not args.no_oracle:
+        o = oracle_est64(wl)
+        kr = o["status"] == 0
+        for prec in results:
+            err[prec] = float(np.abs(results[prec]["est64"][kr] - o["est64"][kr]).max())
+
+    mixed = results["mixed"]
+    fp64 = results["fp64"]
+    solve_s = mixed["solve_ms"] / 1e3
+    achieved = flops_per_pass / solve_s / 1e12
+    line = {
+        "metric": "kriged cells/s on 1 MI355X: FP32+refine vs FP64, at <=1e-6 estimate error",
+        "value": kriged / (mixed["ms"] / 1e3),
+        "unit": "kriged cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (seeded moving-average field, 5% hard data)",
+        "config": {"workload": "ordinary kriging 100x100x50 (500k cells, 475k kriged), exponential (20,20,4), "
+                               "nugget 0.05, radiuses (20,20,4), max_neighbours 24", "cells": 500000,
+                   "kriged_cells": kriged, "solver": "FP32 Cholesky + 2 FP64 refinement steps, FP64 fallback"},
+        "fp64": {"value": kriged / (fp64["ms"] / 1e3), "ms_per_step": fp64["ms"], "solve_ms": fp64["solve_ms"],
+                 "search_ms": fp64["search_ms"], "max_abs_err_vs_oracle": err.get("fp64")},
+        "mixed": {"ms_per_step": mixed["ms"], "solve_ms": mixed["solve_ms"], "search_ms": mixed["search_ms"],
+                  "max_abs_err_vs_oracle": err.get("mixed"), "fp64_fallbacks": mixed["fallbacks"]},
+        "speedup_mixed_vs_fp64": fp64["ms"] / mixed["ms"],
+        "roofline": {"bound": "valu", "kernel": "k_solve<24,32,mixed>", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "flops_per_launch": flops_per_pass,
+                     "fp64_form": {"achieved": flops_per_pass / (fp64["solve_ms"] / 1e3) / 1e12,
+                                   "peak": PEAK_FP64_TFLOPS,
+                                   "frac": flops_per_pass / (fp64["solve_ms"] / 1e3) / 1e12 / PEAK_FP64_TFLOPS}},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,721 @@
+// api.hip -- C ABI of libhpgl.so: the reference's entry points (include/hpgl.h)
+// over host buffers and the device-pointer extensions (include/hpglx.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hpglx.h"
+#include "introsort.h"
+#include "krige_kernels.h"
+#include "krige_launch.h"
+#include "plan.h"
+
+namespace hpglx {
+
+// Model evaluation on the device for scattered coordinates (cov_model.h:63-77,
+// 146-183); the transform comes from the host.
+struct DevModel {
+    int type;
+    double sill, nugget, r0;
+    double t[9];
+};
+
+__device__ inline double dev_cov(const DevModel& m, double v0, double v1, double v2) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double d = v0 * m.t[3 * i + 0];
+        d += v1 * m.t[3 * i + 1];
+        d += v2 * m.t[3 * i + 2];
+        acc += d * d;
+    }
+    const double h = sqrt(acc);
+    if (h < 0.0001) return m.sill;
+    const double x = h / m.r0;
+    if (m.type == 2) return (m.sill - m.nugget) * exp(-3 * (x * x));
+    if (m.type == 1) return (m.sill - m.nugget) * exp(-3 * h / m.r0);
+    if (h > m.r0) return 0.0;
+    return (m.sill - m.nugget) * (1 - 1.5 * x + 0.5 * (x * x * x));
+}
+
+// SK weights at float coordinates (simple_kriging.cpp:18-38, the path of
+// hpgl_simple_kriging_weights): one block, FP64, any n.  The matrix lives in
+// a device buffer of n*n doubles; right-looking Cholesky, one column per step.
+__global__ __launch_bounds__(256) void k_sk_weights(DevModel m, const float* center, const float* xs,
+                                                    const float* ys, const float* zs, int n, double* A, double* x,
+                                                    int* status) {
+    __shared__ int bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) bad = 0;
+    for (long long i = tid; i < (long long)n * n; i += nt) {
+        const int r = (int)(i / n), c = (int)(i % n);
+        A[i] = dev_cov(m, (double)xs[r] - (double)xs[c], (double)ys[r] - (double)ys[c], (double)zs[r] - (double)zs[c]);
+    }
+    for (int r = tid; r < n; r += nt)
+        x[r] = dev_cov(m, (double)xs[r] - (double)center[0], (double)ys[r] - (double)center[1],
+                       (double)zs[r] - (double)center[2]);
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (tid == 0) {
+            const double d = A[(long long)j * n + j];
+            if (!(d > 0.0))
+                bad = 1;
+            else
+                A[(long long)j * n + j] = sqrt(d);
+        }
+        __syncthreads();
+        if (bad) break;
+        const double d = A[(long long)j * n + j];
+        for (int i = j + 1 + tid; i < n; i += nt) A[(long long)i * n + j] /= d;
+        __syncthreads();
+        const int t = n - j - 1;
+        for (long long idx = tid; idx < (long long)t * t; idx += nt) {
+            const int i = j + 1 + (int)(idx / t), k = j + 1 + (int)(idx % t);
+            if (k <= i) A[(long long)i * n + k] -= A[(long long)i * n + j] * A[(long long)k * n + j];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (bad) {
+            *status = 1;
+            return;
+        }
+        for (int i = 0; i < n; ++i) {
+            double s = x[i];
+            for (int k = 0; k < i; ++k) s -= A[(long long)i * n + k] * x[k];
+            x[i] = s / A[(long long)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double s = x[i];
+            for (int k = i + 1; k < n; ++k) s -= A[(long long)k * n + i] * x[k];
+            x[i] = s / A[(long long)i * n + i];
+        }
+        *status = 0;
+    }
+}
+
+namespace {
+
+struct Library {
+    std::mutex mu;
+    std::string last_error;
+    std::map<std::string, std::unique_ptr<DevPlan>> plans;
+    Workspace ws;
+    int (*out_handler)(char*, void*) = nullptr;
+    void* out_param = nullptr;
+    int (*progress_handler)(char*, int, void*) = nullptr;
+    void* progress_param = nullptr;
+    int thread_num = 1;
+    int default_precision = PREC_FP64;
+    bool profiling = false;
+    CallTiming timing;
+    hipEvent_t done = nullptr;   // completion of the last call's work
+    hipStream_t io_stream = nullptr;
+};
+
+Library& lib() {
+    static Library* L = new Library();
+    return *L;
+}
+
+void write_msg(Library& L, const std::string& s) {
+    if (L.out_handler) {
+        std::vector<char> buf(s.begin(), s.end());
+        buf.push_back(0);
+        L.out_handler(buf.data(), L.out_param);
+    } else {
+        std::fputs(("[LOG2]" + s).c_str(), stdout);
+        std::fflush(stdout);
+    }
+}
+
+void progress(Library& L, const char* stage, int pct) {
+    if (L.progress_handler) {
+        std::vector<char> buf(stage, stage + std::strlen(stage) + 1);
+        L.progress_handler(buf.data(), pct, L.progress_param);
+    }
+}
+
+std::string plan_key(int kind, const CovParams& c, const int r[3], int kmax) {
+    char b[512];
+    std::snprintf(b, sizeof b, "%d|%d|%.17g,%.17g,%.17g|%.17g,%.17g,%.17g|%.17g|%.17g|%d,%d,%d|%d", kind, c.type,
+                  c.ranges[0], c.ranges[1], c.ranges[2], c.angles[0], c.angles[1], c.angles[2], c.sill, c.nugget, r[0],
+                  r[1], r[2], kmax);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return std::string(b) + "|dev" + std::to_string(dev);
+}
+
+CovParams to_cov(const hpgl_cov_params_t& p) {
+    CovParams c;
+    c.type = p.m_covariance_type;
+    for (int i = 0; i < 3; ++i) {
+        c.ranges[i] = p.m_ranges[i];
+        c.angles[i] = p.m_angles[i];
+    }
+    c.sill = p.m_sill;
+    c.nugget = p.m_nugget;
+    return c;
+}
+
+DevPlan* get_plan(Library& L, int kind, const CovParams& c, const int r[3], int kmax, std::string& err) {
+    const std::string key = plan_key(kind, c, r, kmax);
+    auto it = L.plans.find(key);
+    if (it != L.plans.end()) return it->second.get();
+    std::unique_ptr<DevPlan> p(new DevPlan());
+    if (!build_plan(p->h, (Kind)kind, c, r, kmax, err)) return nullptr;
+    if (!p->upload(err)) return nullptr;
+    DevPlan* raw = p.get();
+    L.plans[key] = std::move(p);
+    return raw;
+}
+
+bool ensure_events(Library& L, std::string& err) {
+    if (!L.done) {
+        if (hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
+            err = "hipEventCreate failed";
+            return false;
+        }
+    }
+    if (!L.timing.valid) {
+        for (int i = 0; i < 4; ++i)
+            if (hipEventCreate(&L.timing.ev[i]) != hipSuccess) {
+                err = "hipEventCreate failed";
+                return false;
+            }
+        L.timing.valid = true;
+    }
+    return true;
+}
+
+int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err) {
+    if (!q) {
+        err = "null request";
+        return -1;
+    }
+    if (!ensure_events(L, err)) return -1;
+    const CovParams c = to_cov(q->cov);
+    DevPlan* plan = get_plan(L, q->kind, c, q->radiuses, q->max_neighbours, err);
+    if (!plan) return -1;
+    GridCall call;
+    call.kind = q->kind;
+    call.precision = q->precision;
+    call.nx = q->dims[0];
+    call.ny = q->dims[1];
+    call.nz = q->dims[2];
+    call.values = q->values;
+    call.mask = q->mask;
+    call.ind = q->ind;
+    call.zfield = q->zfield;
+    call.thresholds = q->thresholds;
+    call.nind = q->nind;
+    call.marg_host = q->marginal;
+    call.auto_mean = q->auto_mean;
+    call.mean = q->mean;
+    call.out_val = q->out_val;
+    call.out_mask = q->out_mask;
+    call.out_class = q->out_class;
+    call.out_prob = q->out_prob;
+    call.out_var = q->out_var;
+    call.out_est64 = q->out_est64;
+    call.out_status = q->out_status;
+    call.order_relations = q->order_relations;
+    call.out_count = q->out_count;
+    call.out_nboff = q->out_nboff;
+    call.pivot_tol = q->pivot_tol > 0 ? q->pivot_tol : 1e-3;
+    call.refine_steps = q->refine_steps >= 0 ? q->refine_steps : 2;
+    if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
+        err = "values and mask are required";
+        return -1;
+    }
+    if (q->kind == KIND_MIK) {
+        if (!q->marginal || (!q->ind && !(q->zfield && q->thresholds))) {
+            err = "median IK needs indicator data (or a field and thresholds) and marginal probabilities";
+            return -1;
+        }
+        if (q->zfield) call.values = q->zfield;
+    }
+    // The previous call's work must be finished before its workspace is reused.
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) {
+        err = "hipStreamWaitEvent failed";
+        return -1;
+    }
+    CallTiming* t = L.profiling ? &L.timing : nullptr;
+    int rc = launch_grid(*plan, L.ws, call, st, t, err);
+    if (rc == 0 && hipEventRecord(L.done, st) != hipSuccess) {
+        err = "hipEventRecord failed";
+        return -1;
+    }
+    return rc;
+}
+
+int fail(Library& L, const std::string& msg) {
+    L.last_error = msg;
+    return -1;
+}
+
+int shape_volume(const hpgl_shape_t* s, std::string& err) {
+    long long v = 1;
+    for (int i = 0; i < 3; ++i) v *= s->m_data[i];
+    if (v > 2147483647LL || v <= 0) {
+        err = "get_shape_volume: grid volume exceeds INT_MAX";
+        return -1;
+    }
+    return (int)v;
+}
+
+// Host-buffer driver shared by the reference entry points.
+struct HostIO {
+    int n = 0;
+    float* d_val = nullptr;
+    unsigned char* d_mask = nullptr;
+    unsigned char* d_ind = nullptr;
+    float* d_out = nullptr;
+    unsigned char* d_out_mask = nullptr;
+    unsigned char* d_out_class = nullptr;
+};
+
+bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned char* mask, const unsigned char* ind,
+              std::string& err) {
+    io.n = n;
+    io.d_mask = (unsigned char*)L.ws.get("io_mask", n, err);
+    io.d_out_mask = (unsigned char*)L.ws.get("io_out_mask", n, err);
+    if (val) {
+        io.d_val = (float*)L.ws.get("io_val", (size_t)n * 4, err);
+        io.d_out = (float*)L.ws.get("io_out", (size_t)n * 4, err);
+    }
+    if (ind) {
+        io.d_ind = (unsigned char*)L.ws.get("io_ind", n, err);
+        io.d_out_class = (unsigned char*)L.ws.get("io_out_class", n, err);
+    }
+    if (!err.empty()) return false;
+    if (!L.io_stream && hipStreamCreateWithFlags(&L.io_stream, hipStreamNonBlocking) != hipSuccess) {
+        err = "hipStreamCreate failed";
+        return false;
+    }
+    if (!ensure_events(L, err)) return false;
+    if (hipStreamWaitEvent(L.io_stream, L.done, 0) != hipSuccess) {
+        err = "hipStreamWaitEvent failed";
+        return false;
+    }
+    if (hipMemcpyAsync(io.d_mask, mask, n, hipMemcpyHostToDevice, L.io_stream) != hipSuccess ||
+        (val && hipMemcpyAsync(io.d_val, val, (size_t)n * 4, hipMemcpyHostToDevice, L.io_stream) != hipSuccess) ||
+        (ind && hipMemcpyAsync(io.d_ind, ind, n, hipMemcpyHostToDevice, L.io_stream) != hipSuccess)) {
+        err = "host to device copy failed";
+        return false;
+    }
+    return true;
+}
+
+template <typename F>
+void timed_call(Library& L, const char* name, long long cells, F&& f) {
+    auto t0 = std::chrono::steady_clock::now();
+    progress(L, name, 0);
+    std::string err;
+    int rc = f(err);
+    if (rc != 0) {
+        L.last_error = err;
+        write_msg(L, std::string(name) + " failed: " + err + "\n");
+        return;
+    }
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    progress(L, name, -1);
+    char b[256];
+    std::snprintf(b, sizeof b, "Done. Average speed: %.0f point/sec.\n", sec > 0 ? cells / sec : 0.0);
+    write_msg(L, b);
+}
+
+}  // namespace
+}  // namespace hpglx
+
+using namespace hpglx;
+
+extern "C" {
+
+// ------------------------------------------------------- reference ABI
+char* hpgl_get_last_exception_message(void) {
+    return (char*)lib().last_error.c_str();
+}
+
+void hpgl_set_output_handler(int (*handler)(char*, void*), void* param) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.out_handler = handler;
+    L.out_param = param;
+}
+
+void hpgl_set_progress_handler(int (*handler)(char*, int, void*), void* param) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.progress_handler = handler;
+    L.progress_param = param;
+}
+
+void hpgl_set_thread_num(int n_threads) {
+    lib().thread_num = n_threads;
+}
+
+int hpgl_get_thread_num(void) {
+    return lib().thread_num;
+}
+
+void hpgl_ordinary_kriging(hpgl_cont_masked_array_t* in, hpgl_ok_params_t* p, hpgl_cont_masked_array_t* out) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(&in->m_shape, err);
+    if (n < 0 || shape_volume(&out->m_shape, err) != n) {
+        L.last_error = err.empty() ? "input and output sizes differ" : err;
+        return;
+    }
+    timed_call(L, "Ordinary Kriging", n, [&](std::string& e) -> int {
+        HostIO io;
+        if (!stage_in(L, io, n, in->m_data, in->m_mask, nullptr, e)) return -1;
+        hpglx_grid_request_t q;
+        std::memset(&q, 0, sizeof q);
+        q.kind = KIND_OK;
+        q.precision = L.default_precision;
+        for (int i = 0; i < 3; ++i) {
+            q.dims[i] = in->m_shape.m_data[i];
+            q.radiuses[i] = p->m_radiuses[i];
+        }
+        q.max_neighbours = p->m_max_neighbours;
+        q.cov.m_covariance_type = p->m_covariance_type;
+        for (int i = 0; i < 3; ++i) {
+            q.cov.m_ranges[i] = p->m_ranges[i];
+            q.cov.m_angles[i] = p->m_angles[i];
+        }
+        q.cov.m_sill = p->m_sill;
+        q.cov.m_nugget = p->m_nugget;
+        q.values = io.d_val;
+        q.mask = io.d_mask;
+        q.out_val = io.d_out;
+        q.out_mask = io.d_out_mask;
+        q.refine_steps = -1;
+        if (run_request(L, &q, L.io_stream, e) != 0) return -1;
+        if (hipMemcpyAsync(out->m_data, io.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipMemcpyAsync(out->m_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipStreamSynchronize(L.io_stream) != hipSuccess) {
+            e = "device to host copy failed";
+            return -1;
+        }
+        return 0;
+    });
+}
+
+void hpgl_simple_kriging(float* input_data, unsigned char* input_mask, hpgl_shape_t* in_shape, hpgl_sk_params_t* p,
+                         float* output_data, unsigned char* output_mask, hpgl_shape_t* out_shape) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(in_shape, err);
+    if (n < 0 || shape_volume(out_shape, err) != n) {
+        L.last_error = err.empty() ? "input and output sizes differ" : err;
+        return;
+    }
+    timed_call(L, "Simple Kriging", n, [&](std::string& e) -> int {
+        HostIO io;
+        if (!stage_in(L, io, n, input_data, input_mask, nullptr, e)) return -1;
+        hpglx_grid_request_t q;
+        std::memset(&q, 0, sizeof q);
+        q.kind = KIND_SK;
+        q.precision = L.default_precision;
+        for (int i = 0; i < 3; ++i) {
+            q.dims[i] = in_shape->m_data[i];
+            q.radiuses[i] = p->m_radiuses[i];
+        }
+        q.max_neighbours = p->m_max_neighbours;
+        q.cov.m_covariance_type = p->m_covariance_type;
+        for (int i = 0; i < 3; ++i) {
+            q.cov.m_ranges[i] = p->m_ranges[i];
+            q.cov.m_angles[i] = p->m_angles[i];
+        }
+        q.cov.m_sill = p->m_sill;
+        q.cov.m_nugget = p->m_nugget;
+        q.auto_mean = p->m_automatic_mean ? 1 : 0;
+        q.mean = p->m_mean;
+        q.values = io.d_val;
+        q.mask = io.d_mask;
+        q.out_val = io.d_out;
+        q.out_mask = io.d_out_mask;
+        q.refine_steps = -1;
+        if (run_request(L, &q, L.io_stream, e) != 0) return -1;
+        if (hipMemcpyAsync(output_data, io.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipMemcpyAsync(output_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipStreamSynchronize(L.io_stream) != hipSuccess) {
+            e = "device to host copy failed";
+            return -1;
+        }
+        return 0;
+    });
+}
+
+void hpgl_median_ik(hpgl_ind_masked_array_t* in, hpgl_median_ik_params_t* p, hpgl_ind_masked_array_t* out) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(&in->m_shape, err);
+    if (n < 0 || shape_volume(&out->m_shape, err) != n) {
+        L.last_error = err.empty() ? "input and output sizes differ" : err;
+        return;
+    }
+    timed_call(L, "Median IK", n, [&](std::string& e) -> int {
+        HostIO io;
+        if (!stage_in(L, io, n, nullptr, in->m_mask, in->m_data, e)) return -1;
+        hpglx_grid_request_t q;
+        std::memset(&q, 0, sizeof q);
+        q.kind = KIND_MIK;
+        q.precision = L.default_precision;
+        for (int i = 0; i < 3; ++i) {
+            q.dims[i] = in->m_shape.m_data[i];
+            q.radiuses[i] = p->m_radiuses[i];
+        }
+        q.max_neighbours = p->m_max_neighbours;
+        q.cov.m_covariance_type = p->m_covariance_type;
+        for (int i = 0; i < 3; ++i) {
+            q.cov.m_ranges[i] = p->m_ranges[i];
+            q.cov.m_angles[i] = p->m_angles[i];
+        }
+        q.cov.m_sill = p->m_sill;
+        q.cov.m_nugget = p->m_nugget;
+        double marg = p->m_marginal_probs[1];
+        q.ind = io.d_ind;
+        q.nind = 1;
+        q.marginal = &marg;
+        q.mask = io.d_mask;
+        q.out_class = io.d_out_class;
+        q.out_mask = io.d_out_mask;
+        q.refine_steps = -1;
+        if (run_request(L, &q, L.io_stream, e) != 0) return -1;
+        if (hipMemcpyAsync(out->m_data, io.d_out_class, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipMemcpyAsync(out->m_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipStreamSynchronize(L.io_stream) != hipSuccess) {
+            e = "device to host copy failed";
+            return -1;
+        }
+        return 0;
+    });
+}
+
+int hpgl_simple_kriging_weights(float* center, float* xs, float* ys, float* zs, int n, hpgl_cov_params_t* params,
+                                float* weights) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (n < 0) return fail(L, "hpgl_simple_kriging_weights: negative neighbour count");
+    CovParams c = to_cov(*params);
+    DevModel m;
+    if (c.type < 0 || c.type > 2) return fail(L, "Unknown covariance type");
+    if (!make_transform(c, m.t)) return fail(L, "All covariance ranges should be non-zero");
+    // simple_kriging.cpp:31-35 empties the weights when the system cannot be
+    // solved (n == 0 included), and api.cpp:399-402 then fails on .at(i).
+    if (n == 0) return 0;
+    m.type = c.type;
+    m.sill = c.sill;
+    m.nugget = c.nugget;
+    m.r0 = c.ranges[0];
+    std::vector<float> host(3 + 3 * (size_t)n);
+    for (int i = 0; i < 3; ++i) host[i] = center[i];
+    for (int i = 0; i < n; ++i) {
+        host[3 + i] = xs[i];
+        host[3 + n + i] = ys[i];
+        host[3 + 2 * (size_t)n + i] = zs[i];
+    }
+    float* d_in = (float*)L.ws.get("skw_in", host.size() * 4, err);
+    double* d_a = (double*)L.ws.get("skw_a", (size_t)n * n * 8, err);
+    double* d_x = (double*)L.ws.get("skw_x", (size_t)n * 8, err);
+    int* d_st = (int*)L.ws.get("skw_st", 4, err);
+    if (!err.empty()) return fail(L, err);
+    std::vector<double> w(n, 0.0);
+    int st = 0;
+    if (hipMemcpy(d_in, host.data(), host.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return fail(L, "copy failed");
+    hipLaunchKernelGGL(k_sk_weights, dim3(1), dim3(256), 0, 0, m, (const float*)d_in, (const float*)(d_in + 3),
+                       (const float*)(d_in + 3 + n), (const float*)(d_in + 3 + 2 * (size_t)n), n, d_a, d_x, d_st);
+    if (hipMemcpy(&st, d_st, 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(w.data(), d_x, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(L, "kernel or copy failed");
+    if (st != 0) return fail(L, "vector::_M_range_check: the kriging system is singular");
+    for (int i = 0; i < n; ++i) weights[i] = (float)w[i];
+    return 0;
+}
+
+// ------------------------------------------------------ device entry points
+int hpglx_krige(const hpglx_grid_request_t* req, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (run_request(L, req, (hipStream_t)stream, err) != 0) return fail(L, err);
+    return 0;
+}
+
+int hpglx_ordinary_kriging(const hpgl_cov_params_t* cov, const int radiuses[3], int max_nb, const int dims[3],
+                           const float* values, const unsigned char* mask, float* out_val, unsigned char* out_mask,
+                           double* out_var, int precision, void* stream) {
+    hpglx_grid_request_t q;
+    std::memset(&q, 0, sizeof q);
+    q.kind = KIND_OK;
+    q.precision = precision;
+    for (int i = 0; i < 3; ++i) {
+        q.dims[i] = dims[i];
+        q.radiuses[i] = radiuses[i];
+    }
+    q.max_neighbours = max_nb;
+    q.cov = *cov;
+    q.values = values;
+    q.mask = mask;
+    q.out_val = out_val;
+    q.out_mask = out_mask;
+    q.out_var = out_var;
+    q.refine_steps = -1;
+    return hpglx_krige(&q, stream);
+}
+
+int hpglx_simple_kriging(const hpgl_cov_params_t* cov, const int radiuses[3], int max_nb, const int dims[3],
+                         const float* values, const unsigned char* mask, int auto_mean, double mean, float* out_val,
+                         unsigned char* out_mask, double* out_var, int precision, void* stream) {
+    hpglx_grid_request_t q;
+    std::memset(&q, 0, sizeof q);
+    q.kind = KIND_SK;
+    q.precision = precision;
+    for (int i = 0; i < 3; ++i) {
+        q.dims[i] = dims[i];
+        q.radiuses[i] = radiuses[i];
+    }
+    q.max_neighbours = max_nb;
+    q.cov = *cov;
+    q.values = values;
+    q.mask = mask;
+    q.auto_mean = auto_mean;
+    q.mean = mean;
+    q.out_val = out_val;
+    q.out_mask = out_mask;
+    q.out_var = out_var;
+    q.refine_steps = -1;
+    return hpglx_krige(&q, stream);
+}
+
+int hpglx_indicator_cdf(const hpgl_cov_params_t* cov, const int radiuses[3], int max_nb, const int dims[3],
+                        const float* zfield, const unsigned char* mask, int nthr, const float* thresholds,
+                        const double* marginal, float* out_cdf, int precision, void* stream) {
+    hpglx_grid_request_t q;
+    std::memset(&q, 0, sizeof q);
+    q.kind = KIND_MIK;
+    q.precision = precision;
+    for (int i = 0; i < 3; ++i) {
+        q.dims[i] = dims[i];
+        q.radiuses[i] = radiuses[i];
+    }
+    q.max_neighbours = max_nb;
+    q.cov = *cov;
+    q.zfield = zfield;
+    q.thresholds = thresholds;
+    q.nind = nthr;
+    q.marginal = marginal;
+    q.mask = mask;
+    q.out_prob = out_cdf;
+    q.order_relations = 1;
+    q.refine_steps = -1;
+    return hpglx_krige(&q, stream);
+}
+
+void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz) {
+    unpack_off(packed, *dx, *dy, *dz);
+}
+
+int hpglx_template(const hpgl_cov_params_t* cov, const int radiuses[3], int* out_xyz, int cap) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    PlanHost h;
+    std::string err;
+    if (!build_plan(h, KIND_OK, to_cov(*cov), radiuses, 1, err)) return fail(L, err);
+    const int n = (int)h.tdx.size();
+    for (int i = 0; i < n && i < cap && out_xyz; ++i) {
+        out_xyz[3 * i + 0] = h.tdx[i];
+        out_xyz[3 * i + 1] = h.tdy[i];
+        out_xyz[3 * i + 2] = h.tdz[i];
+    }
+    return n;
+}
+
+void hpglx_set_profiling(int on) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.profiling = on != 0;
+}
+
+int hpglx_last_timing(float* ms3) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    if (!L.timing.valid) return fail(L, "no profiled call");
+    if (hipEventSynchronize(L.timing.ev[3]) != hipSuccess) return fail(L, "hipEventSynchronize failed");
+    float a = 0, b = 0, c = 0;
+    if (hipEventElapsedTime(&a, L.timing.ev[0], L.timing.ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&b, L.timing.ev[2], L.timing.ev[3]) != hipSuccess ||
+        hipEventElapsedTime(&c, L.timing.ev[0], L.timing.ev[3]) != hipSuccess)
+        return fail(L, "hipEventElapsedTime failed (was profiling on for the last call?)");
+    ms3[0] = a;
+    ms3[1] = b;
+    ms3[2] = c;
+    return 0;
+}
+
+int hpglx_last_call_stats(unsigned int* fallbacks, int* cluster_cells) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    std::string err;
+    int* counters = (int*)L.ws.get("counters", 64, err);
+    if (!counters) return fail(L, err);
+    int h[4] = {0, 0, 0, 0};
+    if (L.done && hipEventSynchronize(L.done) != hipSuccess) return fail(L, "hipEventSynchronize failed");
+    if (hipMemcpy(h, counters, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) return fail(L, "copy failed");
+    if (fallbacks) *fallbacks = (unsigned)h[3];
+    if (cluster_cells) *cluster_cells = h[0];
+    return 0;
+}
+
+void hpglx_set_default_precision(int precision) {
+    lib().default_precision = precision == PREC_MIXED ? PREC_MIXED : PREC_FP64;
+}
+
+void hpglx_host_introsort_perm(const double* keys, int n, int* perm) {
+    struct HostStore {
+        struct Item {
+            double key;
+            int off;
+        };
+        std::vector<Item> v;
+        double key(int i) const { return v[i].key; }
+        Item get(int i) const { return v[i]; }
+        void put(int i, const Item& it) { v[i] = it; }
+        void swap(int i, int j) { std::swap(v[i], v[j]); }
+    } st;
+    st.v.resize(n > 0 ? n : 0);
+    for (int i = 0; i < n; ++i) st.v[i] = HostStore::Item{keys[i], i};
+    Introsort<HostStore> s(st);
+    s.sort(n);
+    for (int i = 0; i < n; ++i) perm[i] = st.v[i].off;
+}
+
+const char* hpglx_last_error(void) {
+    return lib().last_error.c_str();
+}
+
+const char* hpglx_version(void) {
+    return "hpgl-mi355x 0.1 (gfx950)";
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// introsort.h -- the permutation libstdc++'s std::sort produces, computed on
+// the GPU (and on the host, for tests).
+//
+// The reference orders cluster-path candidates with std::sort on a
+// "covariance descending" comparator (sugarbox_indexed_neighbour_lookup.h:100,
+// entry_t at :81-105).  The sort is not stable and exact covariance ties are
+// common, so which tied candidates land in the first max_neighbours slots is
+// decided by the library's algorithm.  To reproduce the neighbour sets the
+// same steps have to run: libstdc++'s introsort as published in GCC's
+// bits/stl_algo.h / bits/stl_heap.h (GCC 11):
+//   - introsort loop while a range holds more than 16 elements, depth limit
+//     2*floor(log2 n); pivot = median of (first+1, mid, last-1) moved to
+//     first, then an unguarded Hoare partition of [first+1, last);
+//     recursion on the right part, iteration on the left;
+//   - heapsort (make_heap + sort_heap) once the depth limit is spent;
+//   - a final insertion sort: guarded on the first 16 elements, unguarded on
+//     the rest.
+// This file is an independent implementation of those steps over a
+// structure-of-arrays store; its output is checked against std::sort itself
+// on tie-heavy inputs by the test suite.
+#pragma once
+
+#if defined(__HIPCC__)
+#define HPGLX_HD __host__ __device__
+#else
+#define HPGLX_HD
+#endif
+
+namespace hpglx {
+
+// Store concept: key(i) -> double, swap(i, j), get(i) -> Item, put(i, Item);
+// "less" is key(a) > key(b) (larger covariance first).
+template <typename Store>
+struct Introsort {
+    typedef typename Store::Item Item;
+    Store& s;
+    HPGLX_HD explicit Introsort(Store& st) : s(st) {}
+
+    HPGLX_HD bool lt_ii(int a, int b) const { return s.key(a) > s.key(b); }
+    HPGLX_HD static bool lt_vv(const Item& a, const Item& b) { return a.key > b.key; }
+
+    HPGLX_HD void move_median_to_first(int result, int a, int b, int c) {
+        if (lt_ii(a, b)) {
+            if (lt_ii(b, c))
+                s.swap(result, b);
+            else if (lt_ii(a, c))
+                s.swap(result, c);
+            else
+                s.swap(result, a);
+        } else if (lt_ii(a, c))
+            s.swap(result, a);
+        else if (lt_ii(b, c))
+            s.swap(result, c);
+        else
+            s.swap(result, b);
+    }
+
+    HPGLX_HD int unguarded_partition(int first, int last, int pivot) {
+        while (true) {
+            while (lt_ii(first, pivot)) ++first;
+            --last;
+            while (lt_ii(pivot, last)) --last;
+            if (!(first < last)) return first;
+            s.swap(first, last);
+            ++first;
+        }
+    }
+
+    // Heap helpers on [first, first + len).
+    HPGLX_HD void push_heap(int first, int hole, int top, Item value) {
+        int parent = (hole - 1) / 2;
+        while (hole > top && lt_vv(s.get(first + parent), value)) {
+            s.put(first + hole, s.get(first + parent));
+            hole = parent;
+            parent = (hole - 1) / 2;
+        }
+        s.put(first + hole, value);
+    }
+
+    HPGLX_HD void adjust_heap(int first, int hole, int len, Item value) {
+        const int top = hole;
+        int child = hole;
+        while (child < (len - 1) / 2) {
+            child = 2 * (child + 1);
+            if (lt_ii(first + child, first + (child - 1))) child--;
+            s.put(first + hole, s.get(first + child));
+            hole = child;
+        }
+        if ((len & 1) == 0 && child == (len - 2) / 2) {
+            child = 2 * (child + 1);
+            s.put(first + hole, s.get(first + (child - 1)));
+            hole = child - 1;
+        }
+        push_heap(first, hole, top, value);
+    }
+
+    HPGLX_HD void make_heap(int first, int last) {
+        const int len = last - first;
+        if (len < 2) return;
+        int parent = (len - 2) / 2;
+        while (true) {
+            Item v = s.get(first + parent);
+            adjust_heap(first, parent, len, v);
+            if (parent == 0) return;
+            parent--;
+        }
+    }
+
+    HPGLX_HD void pop_heap(int first, int last, int result) {
+        Item v = s.get(result);
+        s.put(result, s.get(first));
+        adjust_heap(first, 0, last - first, v);
+    }
+
+    HPGLX_HD void heap_sort(int first, int last) {
+        // heap_select with middle == last reduces to make_heap.
+        make_heap(first, last);
+        while (last - first > 1) {
+            --last;
+            pop_heap(first, last, last);
+        }
+    }
+
+    HPGLX_HD void unguarded_linear_insert(int last) {
+        Item v = s.get(last);
+        int next = last - 1;
+        while (lt_vv(v, s.get(next))) {
+            s.put(last, s.get(next));
+            last = next;
+            --next;
+        }
+        s.put(last, v);
+    }
+
+    HPGLX_HD void insertion_sort(int first, int last) {
+        if (first == last) return;
+        for (int i = first + 1; i != last; ++i) {
+            if (lt_ii(i, first)) {
+                Item v = s.get(i);
+                for (int k = i; k > first; --k) s.put(k, s.get(k - 1));
+                s.put(first, v);
+            } else {
+                unguarded_linear_insert(i);
+            }
+        }
+    }
+
+    HPGLX_HD static int lg(int n) {
+        int r = 0;
+        while (n > 1) {
+            n >>= 1;
+            ++r;
+        }
+        return r;
+    }
+
+    // Recursion of the right-hand part is replaced by an explicit stack of
+    // (first, last, depth) ranges, visited in the same order.
+    HPGLX_HD void sort(int n) {
+        if (n <= 0) return;
+        const int kThreshold = 16;
+        struct Range { int first, last, depth; };
+        Range stack[64];
+        int sp = 0;
+        // introsort_loop(first, last, depth): while (len > 16) {...; recurse(cut, last); last = cut;}
+        // The recursive call on [cut, last) must complete before the loop
+        // continues on [first, cut); a LIFO stack of pending left parts does that.
+        int first = 0, last = n, depth = 2 * lg(n);
+        while (true) {
+            while (last - first > kThreshold) {
+                if (depth == 0) {
+                    heap_sort(first, last);
+                    last = first;  // nothing left in this range
+                    break;
+                }
+                --depth;
+                const int mid = first + (last - first) / 2;
+                move_median_to_first(first, first + 1, mid, last - 1);
+                const int cut = unguarded_partition(first + 1, last, first);
+                // Defer [first, cut) with the current depth, descend into [cut, last).
+                stack[sp].first = first;
+                stack[sp].last = cut;
+                stack[sp].depth = depth;
+                ++sp;
+                first = cut;
+            }
+            if (sp == 0) break;
+            --sp;
+            first = stack[sp].first;
+            last = stack[sp].last;
+            depth = stack[sp].depth;
+        }
+        // final_insertion_sort
+        if (n > kThreshold) {
+            insertion_sort(0, kThreshold);
+            for (int i = kThreshold; i != n; ++i) unguarded_linear_insert(i);
+        } else {
+            insertion_sort(0, n);
+        }
+    }
+};
+
+}  // namespace hpglx

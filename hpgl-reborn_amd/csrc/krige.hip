@@ -1,0 +1,873 @@
+// krige.hip -- gfx950 kernels of the grid-kriging hot path.
+//
+// Pipeline for one call (all on one stream, no host synchronisation):
+//   k_prepare       padded 0/1 mask, per-cluster informed counts
+//   k_cluster_flags per cluster: limit exceeded / any of its 27 neighbours exceeded
+//   k_tdelta        template offsets -> deltas in the padded mask
+//   k_search_tmpl   template walk, one lane per cell (cells in template-path clusters)
+//   k_cluster_*     node lists of non-exceeded clusters, ascending node order
+//   k_search_cluster cluster path, one lane per remaining cell: gather, filter,
+//                   libstdc++-identical introsort, first max_neighbours
+//   k_solve         covariance assembly + Cholesky + SK/OK solve + combine,
+//                   one group of G lanes per cell (lane r owns row r)
+//
+// Reference semantics reproduced (paths relative to the reference repository):
+//   template walk   src/geo_bsd/hpgl/sugarbox_neighbour_lookup.h:37-62
+//   path selection  src/geo_bsd/hpgl/sugarbox_indexed_neighbour_lookup.h:133-140,
+//                   src/geo_bsd/hpgl/clusterizer.cpp:169-189
+//   cluster path    src/geo_bsd/hpgl/sugarbox_indexed_neighbour_lookup.h:141-190,
+//                   src/geo_bsd/hpgl/clusterizer.cpp:144-167
+//   weights         src/geo_bsd/hpgl/my_kriging_weights.h:60-196 (SK), :198-361 (OK)
+//   combine         src/geo_bsd/hpgl/combiner.h:10-25 (float accumulation in
+//                   neighbour order for OK/SK, double for median IK)
+//   failure policy  src/geo_bsd/hpgl/cont_kriging.h:73-105, median_ik.cpp:60-78
+//   order relations src/geo_bsd/hpgl/indicator_kriging.h:34-59
+#include "introsort.h"
+#include "krige_kernels.h"
+#include "krige_launch.h"
+
+#include <vector>
+
+namespace hpglx {
+
+// ------------------------------------------------------------------ helpers
+__global__ void k_fill_u8(unsigned char* p, long long n, unsigned char v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, long long n, int v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__device__ __forceinline__ void cell_xyz(const GridDims& g, long long cell, int& x, int& y, int& z) {
+    const long long nxy = (long long)g.nx * g.ny;
+    z = (int)(cell / nxy);
+    const long long rem = cell - (long long)z * nxy;
+    y = (int)(rem / g.nx);
+    x = (int)(rem - (long long)y * g.nx);
+}
+
+__device__ __forceinline__ int cluster_of(const GridDims& g, int x, int y, int z) {
+    return ((z / g.rz) * g.cy + (y / g.ry)) * g.cx + (x / g.rx);
+}
+
+// ------------------------------------------------------------------ prepare
+__global__ void k_prepare(GridDims g, const unsigned char* mask, unsigned char* pmask, int* cl_count) {
+    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.n) return;
+    if (!mask[cell]) return;
+    int x, y, z;
+    cell_xyz(g, cell, x, y, z);
+    const long long p = ((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
+    pmask[p] = 1;
+    atomicAdd(&cl_count[cluster_of(g, x, y, z)], 1);
+}
+
+// Mean of informed values in double (calc_mean.cpp:6-30); per-block partial
+// sums in a fixed order, so the result is reproducible run to run.
+__global__ void k_mean_partial(long long n, const float* values, const unsigned char* mask, double* psum,
+                               int* pcnt) {
+    __shared__ double ss[256];
+    __shared__ int sc[256];
+    double s = 0;
+    int c = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (mask[i]) {
+            s += values[i];
+            c += 1;
+        }
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            ss[threadIdx.x] += ss[threadIdx.x + o];
+            sc[threadIdx.x] += sc[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        psum[blockIdx.x] = ss[0];
+        pcnt[blockIdx.x] = sc[0];
+    }
+}
+
+__global__ void k_mean_final(int nblocks, const double* psum, const int* pcnt, double* mean_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0;
+    long long c = 0;
+    for (int i = 0; i < nblocks; ++i) {
+        s += psum[i];
+        c += pcnt[i];
+    }
+    mean_out[0] = c == 0 ? 0.0 : s / (double)c;
+}
+
+__global__ void k_cluster_flags(GridDims g, const int* cl_count, long long limit, unsigned char* nbhd_exceeded) {
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncl = g.cx * g.cy * g.cz;
+    if (ci >= ncl) return;
+    const int kx = ci % g.cx, ky = (ci / g.cx) % g.cy, kz = ci / (g.cx * g.cy);
+    unsigned char ex = 0;
+    for (int k = kz - 1; k <= kz + 1; ++k)
+        for (int j = ky - 1; j <= ky + 1; ++j)
+            for (int i = kx - 1; i <= kx + 1; ++i) {
+                if (i < 0 || j < 0 || k < 0 || i >= g.cx || j >= g.cy || k >= g.cz) continue;
+                if ((long long)cl_count[(k * g.cy + j) * g.cx + i] > limit) ex = 1;
+            }
+    nbhd_exceeded[ci] = ex;
+}
+
+__global__ void k_tdelta(GridDims g, int tsize, const int* tdx, const int* tdy, const int* tdz, int* tdelta,
+                         int* tpack) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tsize) return;
+    tdelta[t] = (tdz[t] * g.py + tdy[t]) * g.px + tdx[t];
+    tpack[t] = pack_off(tdx[t], tdy[t], tdz[t]);
+}
+
+// ------------------------------------------------------------ template walk
+// One lane per cell.  All lanes of a wave walk the template in lock step, so
+// each template delta is a wave-uniform (scalar) load and the 64 mask probes
+// of one step hit neighbouring bytes of the padded mask.
+__global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const GridDims& g = a.g;
+    bool active = false;
+    long long pbase = 0;
+    if (cell < g.n) {
+        const bool needs = a.kind == 2 || !a.mask[cell];
+        if (!needs) {
+            a.count[cell] = CNT_INFORMED;
+        } else {
+            int x, y, z;
+            cell_xyz(g, cell, x, y, z);
+            if (a.cl_nbhd_exceeded[cluster_of(g, x, y, z)]) {
+                active = true;
+                pbase = ((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
+            } else {
+                a.count[cell] = CNT_PENDING_CLUSTER;
+                int pos = atomicAdd(a.n_cl_cells, 1);
+                a.cl_cells[pos] = (int)cell;
+            }
+        }
+    }
+    int cnt = 0;
+    const int K = a.kmax;
+    int* out = a.nboff + (active ? cell * K : 0);
+    constexpr int U = 8;
+    for (int t0 = 0; t0 < a.tsize; t0 += U) {
+        const bool want = active && cnt < K;
+        if (!__any(want)) break;
+        unsigned char m[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u;
+            m[u] = 0;
+            if (t < a.tsize) {
+                const int d = a.tdelta[t];
+                if (want) m[u] = a.pmask[pbase + d];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (m[u] && cnt < K) {
+                out[cnt] = a.tpack[t0 + u];
+                ++cnt;
+            }
+        }
+    }
+    if (active) a.count[cell] = cnt;
+}
+
+// --------------------------------------------------------------- clusters
+// Exclusive scan of the node counts of non-exceeded clusters (one block).
+__global__ void k_cluster_scan(int ncl, const int* cl_count, long long limit, int* cl_start, int* total) {
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int per = (ncl + 1023) / 1024;
+    const int b = tid * per, e = min(ncl, b + per);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += ((long long)cl_count[i] > limit) ? 0 : cl_count[i];
+    part[tid] = s;
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int i = 0; i < 1024; ++i) {
+            int v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        *total = acc;
+    }
+    __syncthreads();
+    int acc = part[tid];
+    for (int i = b; i < e; ++i) {
+        cl_start[i] = acc;
+        acc += ((long long)cl_count[i] > limit) ? 0 : cl_count[i];
+    }
+}
+
+__global__ void k_cluster_fill(GridDims g, const unsigned char* mask, const int* cl_count, long long limit,
+                               const int* cl_start, int* cl_cursor, int* cl_nodes) {
+    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.n || !mask[cell]) return;
+    int x, y, z;
+    cell_xyz(g, cell, x, y, z);
+    const int ci = cluster_of(g, x, y, z);
+    if ((long long)cl_count[ci] > limit) return;
+    const int slot = atomicAdd(&cl_cursor[ci], 1);
+    cl_nodes[cl_start[ci] + slot] = (int)cell;
+}
+
+// Nodes were added to the reference clusters in ascending node order
+// (cont_kriging.h:63-69); sort each list back into that order.
+__global__ void k_cluster_sort(int ncl, const int* cl_count, long long limit, const int* cl_start, int* cl_nodes) {
+    int ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= ncl) return;
+    const int c = cl_count[ci];
+    if ((long long)c > limit || c < 2) return;
+    int* v = cl_nodes + cl_start[ci];
+    for (int i = 1; i < c; ++i) {
+        const int x = v[i];
+        int j = i - 1;
+        while (j >= 0 && v[j] > x) {
+            v[j + 1] = v[j];
+            --j;
+        }
+        v[j + 1] = x;
+    }
+}
+
+// Scratch store for the introsort: element i of lane l lives at i*nlanes + l.
+struct LaneStore {
+    struct Item {
+        double key;
+        int off;
+    };
+    double* keys;
+    int* offs;
+    int stride;
+    __device__ double key(int i) const { return keys[(long long)i * stride]; }
+    __device__ Item get(int i) const {
+        Item it;
+        it.key = keys[(long long)i * stride];
+        it.off = offs[(long long)i * stride];
+        return it;
+    }
+    __device__ void put(int i, const Item& it) {
+        keys[(long long)i * stride] = it.key;
+        offs[(long long)i * stride] = it.off;
+    }
+    __device__ void swap(int i, int j) {
+        Item a = get(i), b = get(j);
+        put(i, b);
+        put(j, a);
+    }
+};
+
+// Cluster path, one lane per pending cell, persistent over the cell list.
+__global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const GridDims& g = a.g;
+    const int ncells = *a.n_cl_cells;
+    const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
+    const long long center = ((long long)a.wz * SY + a.wy) * SX + a.wx;
+    LaneStore st;
+    st.keys = a.skey + lane;
+    st.offs = a.soff + lane;
+    st.stride = a.nlanes;
+    for (int item = lane; item < ncells; item += a.nlanes) {
+        const long long cell = a.cl_cells[item];
+        int x, y, z;
+        cell_xyz(g, cell, x, y, z);
+        const int kx = x / g.rx, ky = y / g.ry, kz = z / g.rz;
+        int m = 0;
+        bool overflow = false;
+        for (int k = kz - 1; k <= kz + 1; ++k)
+            for (int j = ky - 1; j <= ky + 1; ++j)
+                for (int i = kx - 1; i <= kx + 1; ++i) {
+                    if (i < 0 || j < 0 || k < 0 || i >= g.cx || j >= g.cy || k >= g.cz) continue;
+                    const int ci = (k * g.cy + j) * g.cx + i;
+                    const int c = a.cl_count[ci];
+                    const int* nodes = a.cl_nodes + a.cl_start[ci];
+                    for (int s = 0; s < c; ++s) {
+                        const int nd = nodes[s];
+                        int nx_, ny_, nz_;
+                        cell_xyz(g, nd, nx_, ny_, nz_);
+                        const int dx = nx_ - x, dy = ny_ - y, dz = nz_ - z;
+                        const double key = a.ctab[center + ((long long)dz * SY + dy) * SX + dx];
+                        if (key > a.threshold) {
+                            if (m < a.capacity) {
+                                LaneStore::Item it;
+                                it.key = key;
+                                it.off = pack_off(dx, dy, dz);
+                                st.put(m, it);
+                                ++m;
+                            } else {
+                                overflow = true;
+                            }
+                        }
+                    }
+                }
+        if (overflow) atomicAdd(a.overflow, 1);
+        Introsort<LaneStore> srt(st);
+        srt.sort(m);
+        const int n = m < a.kmax ? m : a.kmax;
+        for (int i = 0; i < n; ++i) a.nboff[cell * a.kmax + i] = st.get(i).off;
+        a.count[cell] = n;
+    }
+}
+
+// -------------------------------------------------------------------- solve
+template <int G, typename T>
+__device__ __forceinline__ T gbc(T v, int src) {
+    return __shfl(v, src, G);
+}
+
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+// In-place Cholesky of the group's matrix, lane r owning row r.  On return
+// lane r holds L[r][c] in a[c] for c <= r and L[c][r] (column r) for c > r,
+// so both triangular solves need one broadcast per step.  The result is
+// group-uniform: false when a pivot is not positive (dpotrf info > 0).
+template <int NMAX, int G, typename T>
+__device__ __forceinline__ bool chol_rowcol(T (&a)[NMAX], T& dinv, T& minpiv, int r) {
+    bool ok = true;
+    minpiv = T(3.0e38);
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        const T piv = gbc<G>(a[j], j);
+        ok = ok && (piv > T(0));
+        minpiv = piv < minpiv ? piv : minpiv;
+        const T s = sqrt(piv);
+        const T inv = T(1) / s;
+        if (r == j) dinv = inv;
+        const T l = (r == j) ? s : a[j] * inv;
+        if (r >= j) a[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < NMAX; ++c) {
+            const T lc = gbc<G>(l, c);
+            if (r == j)
+                a[c] = lc;
+            else if (r > j)
+                a[c] = a[c] - l * lc;
+        }
+    }
+    return ok;
+}
+
+// Solves L L^T x = x in place (lane r holds x_r); factor from chol_rowcol.
+template <int NMAX, int G, typename T>
+__device__ __forceinline__ T chol_solve(const T (&a)[NMAX], T dinv, T x, int r) {
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        const T y = gbc<G>(x * dinv, j);
+        if (r == j)
+            x = y;
+        else if (r > j)
+            x = x - a[j] * y;
+    }
+#pragma unroll
+    for (int i = NMAX - 1; i >= 0; --i) {
+        const T y = gbc<G>(x * dinv, i);
+        if (r == i)
+            x = y;
+        else if (r < i)
+            x = x - a[i] * y;
+    }
+    return x;
+}
+
+// FP32 solve of an FP64 right-hand side, result promoted back.
+template <int NMAX, int G>
+__device__ __forceinline__ double solve32(const float (&af)[NMAX], float dinv, double rhs, int r) {
+    return (double)chol_solve<NMAX, G, float>(af, dinv, (float)rhs, r);
+}
+
+// b - A x with A the FP64 row set (lane r holds row r).
+template <int NMAX, int G>
+__device__ __forceinline__ double residual(const double (&a64)[NMAX], double b, double x) {
+    double acc = b;
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) acc -= a64[c] * gbc<G>(x, c);
+    return acc;
+}
+
+template <int NMAX, int G, bool MIXED>
+__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
+    const int r = threadIdx.x & (G - 1);
+    const long long cell = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const GridDims& g = a.g;
+    if (cell >= g.n) return;
+    const int cnt = a.count[cell];
+    const int kind = a.kind;
+    if (cnt == CNT_INFORMED) {
+        // cont_kriging.h:97-100: informed cells are copied, mask set to 1.
+        if (r == 0) {
+            const float v = a.values[cell];
+            if (a.out_val) a.out_val[cell] = v;
+            if (a.out_mask) a.out_mask[cell] = 1;
+            if (a.out_var) a.out_var[cell] = 0.0;
+            if (a.out_est64) a.out_est64[cell] = v;
+            if (a.out_status) a.out_status[cell] = ST_INFORMED;
+        }
+        return;
+    }
+    const int n = cnt < 0 ? 0 : cnt;
+    const double mean = (kind == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
+    const float mf = (float)mean;
+
+    int status = ST_KRIGED;
+    double w = 0.0, mu = 0.0, b = 0.0;
+    int dx = 0, dy = 0, dz = 0;
+    if (n == 0) {
+        status = ST_NO_NEIGHBOURS;
+    } else {
+        if (r < n) unpack_off(a.nboff[cell * a.kmax + r], dx, dy, dz);
+        const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
+        const long long center = ((long long)a.wz * SY + a.wy) * SX + a.wx;
+        const int tr = (dz * SY + dy) * SX + dx;
+        double a64[NMAX];
+#pragma unroll
+        for (int c = 0; c < NMAX; ++c) {
+            const int tc = gbc<G>(tr, c);
+            double v = (r == c) ? 1.0 : 0.0;
+            if (r < n && c < n) v = a.ctab[center + tr - tc];
+            a64[c] = v;
+        }
+        b = (r < n) ? a.ctab[center + tr] : 0.0;
+        const double one = (r < n) ? 1.0 : 0.0;
+        double s = 0.0, o = 0.0;
+        bool solved = false;
+        bool use64 = !MIXED;
+        if (MIXED) {
+            float af[NMAX];
+#pragma unroll
+            for (int c = 0; c < NMAX; ++c) af[c] = (float)a64[c];
+            float dinvf = 0.f, minpivf = 0.f;
+            const bool ok32 = chol_rowcol<NMAX, G, float>(af, dinvf, minpivf, r);
+            if (!ok32 || !((double)minpivf >= a.pivot_tol * a.c0)) {
+                use64 = true;
+                if (r == 0 && a.fallback_count) atomicAdd(a.fallback_count, 1u);
+            } else {
+                s = solve32<NMAX, G>(af, dinvf, b, r);
+                if (kind == 0) o = solve32<NMAX, G>(af, dinvf, one, r);
+                for (int it = 0; it < a.refine_steps; ++it) {
+                    s += solve32<NMAX, G>(af, dinvf, residual<NMAX, G>(a64, b, s), r);
+                    if (kind == 0) o += solve32<NMAX, G>(af, dinvf, residual<NMAX, G>(a64, one, o), r);
+                }
+                solved = true;
+            }
+        }
+        if (use64) {
+            double dinv = 0.0, minpiv = 0.0;
+            solved = chol_rowcol<NMAX, G, double>(a64, dinv, minpiv, r);
+            if (solved) {
+                s = chol_solve<NMAX, G, double>(a64, dinv, b, r);
+                if (kind == 0) o = chol_solve<NMAX, G, double>(a64, dinv, one, r);
+            }
+        }
+        if (!solved) {
+            status = ST_SINGULAR;
+        } else if (kind == 0) {
+            const double sum_sk = gsum<G>(s), sum_ones = gsum<G>(o);
+            if (fabs(sum_ones) < 1e-15) {
+                status = ST_SINGULAR;
+            } else {
+                mu = (sum_sk - 1) / sum_ones;
+                w = s - mu * o;
+            }
+        } else {
+            w = s;
+        }
+    }
+
+    const long long node = cell + ((long long)dz * g.ny + dy) * g.nx + dx;
+    if (kind != 2) {
+        const float v = (r < n) ? a.values[node] : 0.0f;
+        const float term = (float)(w * (double)(v - mf));
+        float acc = mf;
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) {
+            const float ti = gbc<G>(term, i);
+            if (i < n) acc += ti;
+        }
+        const double e64 = (double)mf + gsum<G>(w * ((double)v - (double)mf));
+        const double var = a.c0 - gsum<G>(w * b) - mu;
+        if (r == 0) {
+            if (status == ST_KRIGED) {
+                if (a.out_val) a.out_val[cell] = acc;
+                if (a.out_mask) a.out_mask[cell] = 1;
+                if (a.out_var) a.out_var[cell] = var < 0 ? 0.0 : var;
+                if (a.out_est64) a.out_est64[cell] = e64;
+            } else if (kind == 1) {
+                // mean_on_failure
+                if (a.out_val) a.out_val[cell] = mf;
+                if (a.out_mask) a.out_mask[cell] = 1;
+                if (a.out_var) a.out_var[cell] = -1.0;
+                if (a.out_est64) a.out_est64[cell] = (double)mf;
+            } else {
+                // undefined_on_failure: the output keeps the input clone
+                if (a.out_val) a.out_val[cell] = a.values[cell];
+                if (a.out_mask) a.out_mask[cell] = a.mask[cell];
+                if (a.out_var) a.out_var[cell] = -1.0;
+                if (a.out_est64) a.out_est64[cell] = __builtin_nan("");
+            }
+            if (a.out_status) a.out_status[cell] = (unsigned char)status;
+        }
+        return;
+    }
+
+    // Median indicator kriging: one weight set, one combine per threshold.
+    constexpr int KMAXIND = 16;
+    float probs[KMAXIND];
+    const double var = a.c0 - gsum<G>(w * b);
+#pragma unroll
+    for (int k = 0; k < KMAXIND; ++k) {
+        probs[k] = 0.f;
+        if (k < a.nind) {
+            const float pf = (float)a.marg[k];
+            float iv = 0.f;
+            if (r < n) {
+                if (a.zfield)
+                    iv = (a.zfield[node] <= a.thresholds[k]) ? 1.f : 0.f;
+                else
+                    iv = (a.ind[(long long)k * g.n + node] == 1) ? 1.f : 0.f;
+            }
+            const double term = w * (double)(iv - pf);
+            double acc = (double)pf;
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                const double ti = gbc<G>(term, i);
+                if (i < n && status == ST_KRIGED) acc += ti;
+            }
+            const double e64 = (double)pf + (status == ST_KRIGED ? gsum<G>(w * ((double)iv - (double)pf)) : 0.0);
+            probs[k] = (float)acc;
+            if (r == 0) {
+                if (a.out_est64) a.out_est64[(long long)k * g.n + cell] = e64;
+                if (a.out_class) a.out_class[(long long)k * g.n + cell] = ((float)acc <= 0.5f) ? 0 : 1;
+            }
+        }
+    }
+    if (r == 0) {
+        if (a.order_relations) {
+            for (int k = 0; k < KMAXIND; ++k)
+                if (k < a.nind) probs[k] = probs[k] < 0.0f ? 0.0f : (probs[k] > 1.0f ? 1.0f : probs[k]);
+            for (int k = 0; k + 1 < KMAXIND; ++k)
+                if (k + 1 < a.nind && probs[k] > probs[k + 1]) {
+                    const double avg = ((double)probs[k] + (double)probs[k + 1]) / 2.0;
+                    probs[k] = (float)avg;
+                    probs[k + 1] = (float)avg;
+                }
+        }
+        if (a.out_prob)
+            for (int k = 0; k < KMAXIND; ++k)
+                if (k < a.nind) a.out_prob[(long long)k * g.n + cell] = probs[k];
+        if (a.out_mask) a.out_mask[cell] = 1;
+        if (a.out_var) a.out_var[cell] = status == ST_KRIGED ? (var < 0 ? 0.0 : var) : -1.0;
+        if (a.out_status) a.out_status[cell] = (unsigned char)status;
+    }
+}
+
+// Explicit instantiations used by the launcher.
+#define HPGLX_SOLVE(NMAX, G)                                  \
+    template __global__ void k_solve<NMAX, G, false>(SolveArgs); \
+    template __global__ void k_solve<NMAX, G, true>(SolveArgs);
+HPGLX_SOLVE(8, 8)
+HPGLX_SOLVE(16, 16)
+HPGLX_SOLVE(24, 32)
+HPGLX_SOLVE(32, 32)
+#undef HPGLX_SOLVE
+
+// ------------------------------------------------------------------ launcher
+#define HPGLX_CHECK(expr)                                              \
+    do {                                                               \
+        hipError_t e_ = (expr);                                        \
+        if (e_ != hipSuccess) {                                        \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);   \
+            return -1;                                                 \
+        }                                                              \
+    } while (0)
+
+bool DevPlan::upload(std::string& err) {
+    if (ctab) return true;
+    const size_t nt = h.ctab.size();
+    std::vector<float> t32(nt);
+    for (size_t i = 0; i < nt; ++i) t32[i] = (float)h.ctab[i];
+    tsize = (int)h.tdx.size();
+    const size_t tb = (size_t)(tsize > 0 ? tsize : 1) * sizeof(int);
+    if (hipMalloc(&ctab, nt * sizeof(double)) != hipSuccess || hipMalloc(&ctab32, nt * sizeof(float)) != hipSuccess ||
+        hipMalloc(&tdx, tb) != hipSuccess || hipMalloc(&tdy, tb) != hipSuccess || hipMalloc(&tdz, tb) != hipSuccess) {
+        err = "hipMalloc failed for the plan tables";
+        release();
+        return false;
+    }
+    if (hipMemcpy(ctab, h.ctab.data(), nt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ctab32, t32.data(), nt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        (tsize > 0 && (hipMemcpy(tdx, h.tdx.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+                       hipMemcpy(tdy, h.tdy.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+                       hipMemcpy(tdz, h.tdz.data(), tb, hipMemcpyHostToDevice) != hipSuccess))) {
+        err = "hipMemcpy failed for the plan tables";
+        release();
+        return false;
+    }
+    return true;
+}
+
+void DevPlan::release() {
+    if (ctab) (void)hipFree(ctab);
+    if (ctab32) (void)hipFree(ctab32);
+    if (tdx) (void)hipFree(tdx);
+    if (tdy) (void)hipFree(tdy);
+    if (tdz) (void)hipFree(tdz);
+    ctab = nullptr;
+    ctab32 = nullptr;
+    tdx = tdy = tdz = nullptr;
+}
+
+void* Workspace::get(const std::string& name, size_t bytes, std::string& err) {
+    if (bytes == 0) bytes = 256;
+    auto it = bufs.find(name);
+    if (it != bufs.end() && it->second.second >= bytes) return it->second.first;
+    if (it != bufs.end()) {
+        // The previous contents are not needed; callers wait for prior work
+        // before the launcher runs, so freeing here is safe.
+        (void)hipDeviceSynchronize();
+        (void)hipFree(it->second.first);
+        bufs.erase(it);
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        err = "hipMalloc failed for workspace buffer '" + name + "' (" + std::to_string(bytes) + " bytes)";
+        return nullptr;
+    }
+    bufs[name] = std::make_pair(p, bytes);
+    return p;
+}
+
+void Workspace::release() {
+    for (auto& kv : bufs) (void)hipFree(kv.second.first);
+    bufs.clear();
+}
+
+namespace {
+
+template <int NMAX, int G>
+void launch_solve(const SolveArgs& sa, bool mixed, long long ncells, hipStream_t st) {
+    const int block = 256;
+    const long long threads = ncells * G;
+    const unsigned grid = (unsigned)((threads + block - 1) / block);
+    if (mixed)
+        hipLaunchKernelGGL((k_solve<NMAX, G, true>), dim3(grid), dim3(block), 0, st, sa);
+    else
+        hipLaunchKernelGGL((k_solve<NMAX, G, false>), dim3(grid), dim3(block), 0, st, sa);
+}
+
+unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace
+
+int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st, CallTiming* timing,
+                std::string& err) {
+    const PlanHost& h = plan.h;
+    if (!plan.upload(err)) return -1;
+    if (c.nx < 1 || c.ny < 1 || c.nz < 1) {
+        err = "grid dimensions must be positive";
+        return -1;
+    }
+    const long long n = (long long)c.nx * c.ny * c.nz;
+    if (n > 2147483647LL) {
+        err = "grid volume exceeds INT_MAX";
+        return -1;
+    }
+    if (h.kmax > 32) {
+        err = "max_neighbours above 32 is not supported by the GPU solver";
+        return -1;
+    }
+    if (c.kind == KIND_MIK && (c.nind < 1 || c.nind > 16)) {
+        err = "median IK needs between 1 and 16 thresholds";
+        return -1;
+    }
+    GridDims g;
+    g.nx = c.nx;
+    g.ny = c.ny;
+    g.nz = c.nz;
+    g.n = n;
+    g.rx = h.r[0];
+    g.ry = h.r[1];
+    g.rz = h.r[2];
+    g.px = c.nx + 2 * g.rx;
+    g.py = c.ny + 2 * g.ry;
+    g.pz = c.nz + 2 * g.rz;
+    // clusterizer.cpp:87-89: g / r + 2, in double, truncated.
+    g.cx = (int)(c.nx / (double)g.rx + 2);
+    g.cy = (int)(c.ny / (double)g.ry + 2);
+    g.cz = (int)(c.nz / (double)g.rz + 2);
+    const long long P = (long long)g.px * g.py * g.pz;
+    const int ncl = g.cx * g.cy * g.cz;
+    const int K = h.kmax;
+    const int T = plan.tsize;
+
+    auto* pmask = (unsigned char*)ws.get("pmask", (size_t)P, err);
+    auto* cl_count = (int*)ws.get("cl_count", (size_t)ncl * 4, err);
+    auto* cl_start = (int*)ws.get("cl_start", (size_t)ncl * 4, err);
+    auto* cl_cursor = (int*)ws.get("cl_cursor", (size_t)ncl * 4, err);
+    auto* nbhd = (unsigned char*)ws.get("cl_nbhd", (size_t)ncl, err);
+    auto* tdelta = (int*)ws.get("tdelta", (size_t)(T > 0 ? T : 1) * 4, err);
+    auto* tpack = (int*)ws.get("tpack", (size_t)(T > 0 ? T : 1) * 4, err);
+    auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
+    auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
+    auto* cl_cells = (int*)ws.get("cl_cells", (size_t)n * 4, err);
+    auto* cl_nodes = (int*)ws.get("cl_nodes", (size_t)n * 4, err);
+    auto* counters = (int*)ws.get("counters", 64, err);
+    const int mean_blocks = 1024;
+    auto* psum = (double*)ws.get("mean_psum", mean_blocks * 8, err);
+    auto* pcnt = (int*)ws.get("mean_pcnt", mean_blocks * 4, err);
+    auto* dmean = (double*)ws.get("mean", 8, err);
+    auto* dmarg = (double*)ws.get("marg", 16 * 8, err);
+    // Cluster-path scratch: every candidate comes from one of 27 clusters
+    // holding at most `limit` nodes each.
+    const int cap = (int)(27 * (h.limit > 0 ? h.limit : 1));
+    const int nlanes = 32768;
+    auto* skey = (double*)ws.get("skey", (size_t)cap * nlanes * 8, err);
+    auto* soff = (int*)ws.get("soff", (size_t)cap * nlanes * 4, err);
+    if (!err.empty()) return -1;
+    int* n_cl_cells = counters + 0;
+    int* cl_total = counters + 1;
+    int* overflow = counters + 2;
+    unsigned* fallback = (unsigned*)(counters + 3);
+
+    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[0], st));
+
+    const int B = 256;
+    hipLaunchKernelGGL(k_fill_u8, dim3(std::min<long long>(4096, blocks_for(P, B))), dim3(B), 0, st, pmask, P,
+                       (unsigned char)0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long long>(1024, blocks_for(ncl, B))), dim3(B), 0, st, cl_count,
+                       (long long)ncl, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long long>(1024, blocks_for(ncl, B))), dim3(B), 0, st, cl_cursor,
+                       (long long)ncl, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, counters, 16LL, 0);
+    hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n, B)), dim3(B), 0, st, g, c.mask, pmask, cl_count);
+    hipLaunchKernelGGL(k_cluster_flags, dim3(blocks_for(ncl, B)), dim3(B), 0, st, g, (const int*)cl_count,
+                       (long long)h.limit, nbhd);
+    if (T > 0)
+        hipLaunchKernelGGL(k_tdelta, dim3(blocks_for(T, B)), dim3(B), 0, st, g, T, (const int*)plan.tdx,
+                           (const int*)plan.tdy, (const int*)plan.tdz, tdelta, tpack);
+    if (c.kind == KIND_SK && c.auto_mean) {
+        hipLaunchKernelGGL(k_mean_partial, dim3(mean_blocks), dim3(256), 0, st, n, c.values, c.mask, psum, pcnt);
+        hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, st, mean_blocks, (const double*)psum,
+                           (const int*)pcnt, dmean);
+    }
+    if (c.kind == KIND_MIK) {
+        double hm[16] = {0};
+        for (int k = 0; k < c.nind; ++k) hm[k] = c.marg_host[k];
+        HPGLX_CHECK(hipMemcpyAsync(dmarg, hm, sizeof(hm), hipMemcpyHostToDevice, st));
+    }
+
+    SearchArgs sa;
+    sa.g = g;
+    sa.kind = c.kind;
+    sa.kmax = K;
+    sa.mask = c.mask;
+    sa.pmask = pmask;
+    sa.tdelta = tdelta;
+    sa.tpack = tpack;
+    sa.tsize = T;
+    sa.cl_nbhd_exceeded = nbhd;
+    sa.count = count;
+    sa.nboff = nboff;
+    sa.cl_cells = cl_cells;
+    sa.n_cl_cells = n_cl_cells;
+    hipLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, sa);
+
+    hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(1024), 0, st, ncl, (const int*)cl_count, (long long)h.limit,
+                       cl_start, cl_total);
+    hipLaunchKernelGGL(k_cluster_fill, dim3(blocks_for(n, B)), dim3(B), 0, st, g, c.mask, (const int*)cl_count,
+                       (long long)h.limit, (const int*)cl_start, cl_cursor, cl_nodes);
+    hipLaunchKernelGGL(k_cluster_sort, dim3(blocks_for(ncl, B)), dim3(B), 0, st, ncl, (const int*)cl_count,
+                       (long long)h.limit, (const int*)cl_start, cl_nodes);
+    ClusterSearchArgs ca;
+    ca.g = g;
+    ca.kmax = K;
+    ca.cl_cells = cl_cells;
+    ca.n_cl_cells = n_cl_cells;
+    ca.cl_start = cl_start;
+    ca.cl_count = cl_count;
+    ca.cl_nodes = cl_nodes;
+    ca.limit = h.limit;
+    ca.ctab = plan.ctab;
+    ca.wx = h.W[0];
+    ca.wy = h.W[1];
+    ca.wz = h.W[2];
+    ca.threshold = h.c0() / 100;
+    ca.skey = skey;
+    ca.soff = soff;
+    ca.capacity = cap;
+    ca.nlanes = nlanes;
+    ca.count = count;
+    ca.nboff = nboff;
+    ca.overflow = overflow;
+    hipLaunchKernelGGL(k_search_cluster, dim3(nlanes / B), dim3(B), 0, st, ca);
+
+    if (timing && timing->valid) {
+        HPGLX_CHECK(hipEventRecord(timing->ev[1], st));
+    }
+
+    SolveArgs so;
+    so.g = g;
+    so.kind = c.kind;
+    so.kmax = K;
+    so.values = c.values;
+    so.mask = c.mask;
+    so.ind = c.ind;
+    so.zfield = c.zfield;
+    so.thresholds = c.thresholds;
+    so.nind = c.nind;
+    so.mean_dev = (c.kind == KIND_SK && c.auto_mean) ? dmean : nullptr;
+    so.mean = c.mean;
+    so.marg = dmarg;
+    so.count = count;
+    so.nboff = nboff;
+    so.ctab = plan.ctab;
+    so.ctab32 = plan.ctab32;
+    so.wx = h.W[0];
+    so.wy = h.W[1];
+    so.wz = h.W[2];
+    so.c0 = h.c0();
+    so.pivot_tol = c.pivot_tol;
+    so.refine_steps = c.refine_steps;
+    so.out_val = c.out_val;
+    so.out_mask = c.out_mask;
+    so.out_class = c.out_class;
+    so.out_prob = c.out_prob;
+    so.out_var = c.out_var;
+    so.out_est64 = c.out_est64;
+    so.out_status = c.out_status;
+    so.order_relations = c.order_relations;
+    so.fallback_count = fallback;
+    const bool mixed = c.precision == PREC_MIXED;
+    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[2], st));
+    if (K <= 8)
+        launch_solve<8, 8>(so, mixed, n, st);
+    else if (K <= 16)
+        launch_solve<16, 16>(so, mixed, n, st);
+    else if (K <= 24)
+        launch_solve<24, 32>(so, mixed, n, st);
+    else
+        launch_solve<32, 32>(so, mixed, n, st);
+    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[3], st));
+    HPGLX_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace hpglx

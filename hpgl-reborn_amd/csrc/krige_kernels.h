@@ -1,0 +1,105 @@
+// krige_kernels.h -- argument blocks shared by the grid-kriging kernels and
+// their host launcher.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hpglx {
+
+// Per-cell neighbour offsets are packed into one int: dx and dy in 11 bits,
+// dz in 10 bits, each biased (dx + 1024, dy + 1024, dz + 512).
+__host__ __device__ inline int pack_off(int dx, int dy, int dz) {
+    return (dx + 1024) | ((dy + 1024) << 11) | ((dz + 512) << 22);
+}
+__host__ __device__ inline void unpack_off(int p, int& dx, int& dy, int& dz) {
+    dx = (p & 2047) - 1024;
+    dy = ((p >> 11) & 2047) - 1024;
+    dz = ((unsigned)p >> 22) - 512;
+}
+
+// Cell status written by the solve kernel (same codes as the oracle).
+enum CellStatus { ST_KRIGED = 0, ST_NO_NEIGHBOURS = 1, ST_SINGULAR = 2, ST_INFORMED = 3 };
+
+// count[] markers set by the search kernels.
+enum { CNT_INFORMED = -1, CNT_PENDING_CLUSTER = -2 };
+
+struct GridDims {
+    int nx, ny, nz;
+    long long n;
+    int rx, ry, rz;          // search radiuses (= cluster sizes)
+    int px, py, pz;          // padded mask dims (n + 2r)
+    int cx, cy, cz;          // cluster box dims (n / r + 2)
+};
+
+struct SearchArgs {
+    GridDims g;
+    int kind;                      // 0 OK, 1 SK, 2 MIK
+    int kmax;
+    const unsigned char* mask;     // input mask (N)
+    const unsigned char* pmask;    // padded 0/1 mask
+    const int* tdelta;             // template: padded-linear delta
+    const int* tpack;              // template: packed offset
+    int tsize;
+    const unsigned char* cl_nbhd_exceeded;  // per cluster: any of its 27 neighbours exceeded
+    int* count;                    // per cell neighbour count / marker
+    int* nboff;                    // per cell kmax packed offsets
+    int* cl_cells;                 // list of cluster-path cells
+    int* n_cl_cells;               // its length (device counter)
+};
+
+struct ClusterSearchArgs {
+    GridDims g;
+    int kmax;
+    const int* cl_cells;
+    const int* n_cl_cells;
+    const int* cl_start;           // per cluster: start in cl_nodes (exclusive scan)
+    const int* cl_count;           // per cluster: informed count
+    const int* cl_nodes;           // concatenated sorted node lists (non-exceeded clusters)
+    long long limit;
+    const double* ctab;            // covariance table over +-W
+    int wx, wy, wz;                // table half-widths
+    double threshold;              // C(0) / 100
+    double* skey;                  // scratch keys   [capacity][nlanes]
+    int* soff;                     // scratch offsets
+    int capacity;                  // scratch entries per lane
+    int nlanes;                    // total lanes of the launch
+    int* count;
+    int* nboff;
+    int* overflow;                 // set when a candidate list exceeds capacity
+};
+
+struct SolveArgs {
+    GridDims g;
+    int kind;
+    int kmax;
+    const float* values;           // OK/SK input values (N)
+    const unsigned char* mask;     // input mask (N)
+    const unsigned char* ind;      // MIK: indicator data, nind arrays of N
+    const float* zfield;           // MIK CDF mode: continuous field (N), else null
+    const float* thresholds;       // MIK CDF mode: nind thresholds
+    int nind;
+    const double* mean_dev;        // SK automatic mean (device), or null
+    double mean;                   // SK explicit mean
+    const double* marg;            // MIK: nind marginal probabilities (device)
+    const int* count;
+    const int* nboff;
+    const double* ctab;
+    const float* ctab32;
+    int wx, wy, wz;
+    double c0;                     // C(0)
+    double pivot_tol;              // mixed precision: FP64 fallback threshold
+    int refine_steps;
+    // outputs (any may be null)
+    float* out_val;                // OK/SK estimate (N)
+    unsigned char* out_mask;       // OK/SK/MIK mask (N)
+    unsigned char* out_class;      // MIK: indicator class per threshold (nind x N)
+    float* out_prob;               // MIK: probability per threshold (nind x N)
+    double* out_var;               // kriging variance (N), -1 when not kriged
+    double* out_est64;             // estimate in FP64 (nind x N for MIK)
+    unsigned char* out_status;     // CellStatus (N)
+    int order_relations;           // MIK CDF: apply the order-relation correction
+    unsigned int* fallback_count;  // mixed precision: systems re-solved in FP64
+};
+
+}  // namespace hpglx

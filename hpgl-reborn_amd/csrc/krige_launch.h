@@ -1,0 +1,71 @@
+// krige_launch.h -- host launcher of the grid-kriging pipeline.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+
+#include "plan.h"
+
+namespace hpglx {
+
+// A plan mirrored in device memory.
+struct DevPlan {
+    PlanHost h;
+    double* ctab = nullptr;
+    float* ctab32 = nullptr;
+    int* tdx = nullptr;
+    int* tdy = nullptr;
+    int* tdz = nullptr;
+    int tsize = 0;
+    bool upload(std::string& err);
+    void release();
+};
+
+// Grow-only named device buffers reused across calls.
+struct Workspace {
+    std::map<std::string, std::pair<void*, size_t>> bufs;
+    void* get(const std::string& name, size_t bytes, std::string& err);
+    void release();
+};
+
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+struct GridCall {
+    int kind = 0;                 // 0 OK, 1 SK, 2 median IK
+    int precision = PREC_FP64;
+    int nx = 0, ny = 0, nz = 0;
+    const float* values = nullptr;          // OK/SK data (device, N)
+    const unsigned char* mask = nullptr;    // informed mask (device, N)
+    const unsigned char* ind = nullptr;     // median IK indicator data (device, nind x N)
+    const float* zfield = nullptr;          // median IK CDF mode: field (device, N)
+    const float* thresholds = nullptr;      // median IK CDF mode: thresholds (device, nind)
+    int nind = 0;
+    const double* marg_host = nullptr;      // median IK marginal P(I=1) per threshold (host, nind)
+    int auto_mean = 0;                      // SK
+    double mean = 0.0;                      // SK explicit mean
+    float* out_val = nullptr;
+    unsigned char* out_mask = nullptr;
+    unsigned char* out_class = nullptr;
+    float* out_prob = nullptr;
+    double* out_var = nullptr;
+    double* out_est64 = nullptr;
+    unsigned char* out_status = nullptr;
+    int order_relations = 0;
+    int* out_count = nullptr;               // debug: neighbour counts (device, N)
+    int* out_nboff = nullptr;               // debug: packed offsets (device, N x kmax)
+    double pivot_tol = 1e-3;
+    int refine_steps = 2;
+    bool profile = false;
+};
+
+struct CallTiming {
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // start, after search, before solve, end
+    bool valid = false;
+};
+
+int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& call, hipStream_t stream, CallTiming* timing,
+                std::string& err);
+
+}  // namespace hpglx

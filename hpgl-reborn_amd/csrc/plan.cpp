@@ -1,0 +1,189 @@
+// plan.cpp -- builds the search template and covariance table of a plan.
+//
+// Compiled with g++ (not hipcc) and -ffp-contract=off so that every
+// covariance value is the bit-identical double the reference computes:
+// the ordering of the search template and of cluster-path candidates is a
+// sort on these doubles, with frequent exact ties.
+//
+// Reference behaviour followed (paths relative to the reference repository):
+//   transform  src/geo_bsd/hpgl/cov_model.h:16-61  (scale*((rx*ry)*rz), TNT product order)
+//   norm       src/geo_bsd/hpgl/cov_model.h:63-77
+//   models     src/geo_bsd/hpgl/cov_model.h:146-183 (h < 1e-4 -> sill)
+//   template   src/geo_bsd/hpgl/covariance_field.h:61-106 (z,y,x push order,
+//              threshold C(0)/100, std::sort on covariance descending)
+//   clipping   src/geo_bsd/hpgl/precalculated_covariance.h:67-75 (SK),
+//              src/geo_bsd/hpgl/covariance_field.h:144-153 (median IK)
+//   limit      src/geo_bsd/hpgl/sugarbox_indexed_neighbour_lookup.h:126 (|template| / 250)
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+namespace hpglx {
+
+namespace {
+const double kPi = 3.14159265358979323846264338327950288419;
+
+void mul3(const double a[9], const double b[9], double c[9]) {
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0;
+            for (int j = 0; j < 3; ++j) s = s + a[3 * i + j] * b[3 * j + k];
+            c[3 * i + k] = s;
+        }
+}
+}  // namespace
+
+bool make_transform(const CovParams& p, double t[9]) {
+    for (int i = 0; i < 3; ++i)
+        if (p.ranges[i] == 0) return false;
+    auto rad = [](double g) { return (g * kPi) / 180; };
+    double sc[9] = {0}, rz[9] = {0}, ry[9] = {0}, rx[9] = {0};
+    sc[0] = 1.0;
+    sc[4] = p.ranges[0] / p.ranges[1];
+    sc[8] = p.ranges[0] / p.ranges[2];
+    rz[0] = cos(rad(p.angles[0]));
+    rz[4] = cos(rad(p.angles[0]));
+    rz[1] = sin(rad(p.angles[0]));
+    rz[3] = -sin(rad(p.angles[0]));
+    rz[8] = 1;
+    ry[0] = cos(rad(p.angles[1]));
+    ry[8] = cos(rad(p.angles[1]));
+    ry[2] = -sin(rad(p.angles[1]));
+    ry[6] = sin(rad(p.angles[1]));
+    ry[4] = 1;
+    rx[4] = cos(rad(p.angles[2]));
+    rx[8] = cos(rad(p.angles[2]));
+    rx[5] = sin(rad(p.angles[2]));
+    rx[7] = -sin(rad(p.angles[2]));
+    rx[0] = 1;
+    double xy[9], xyz[9];
+    mul3(rx, ry, xy);
+    mul3(xy, rz, xyz);
+    mul3(sc, xyz, t);
+    return true;
+}
+
+double model_cov(const CovParams& p, const double t[9], double v0, double v1, double v2) {
+    double acc = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double d = 0.0;
+        d += v0 * t[3 * i + 0];
+        d += v1 * t[3 * i + 1];
+        d += v2 * t[3 * i + 2];
+        acc += d * d;
+    }
+    double h = sqrt(acc);
+    if (h < 0.0001) return p.sill;
+    if (p.type == 2) return (p.sill - p.nugget) * exp(-3 * pow(h / p.ranges[0], 2));
+    if (p.type == 1) return (p.sill - p.nugget) * exp(-3 * h / p.ranges[0]);
+    if (h > p.ranges[0]) return 0;
+    return (p.sill - p.nugget) * (1 - 1.5 * h / p.ranges[0] + 0.5 * pow(h / p.ranges[0], 3));
+}
+
+double PlanHost::c0() const { return model_cov(cov, transform, 0, 0, 0); }
+
+bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses[3], int kmax, std::string& err) {
+    if (kind != KIND_OK && kind != KIND_SK && kind != KIND_MIK) {
+        err = "unknown kriging kind";
+        return false;
+    }
+    if (cov.type < 0 || cov.type > 2) {
+        err = "Unknown covariance type";
+        return false;
+    }
+    p.kind = kind;
+    p.cov = cov;
+    if (!make_transform(cov, p.transform)) {
+        err = "All covariance ranges should be non-zero";
+        return false;
+    }
+    for (int i = 0; i < 3; ++i) {
+        if (radiuses[i] < 1) {
+            err = "search radiuses must be >= 1 (the reference divides by them)";
+            return false;
+        }
+        if (radiuses[i] > 255) {
+            err = "search radiuses above 255 are not supported";
+            return false;
+        }
+        p.r[i] = radiuses[i];
+    }
+    if (kmax < 1) {
+        err = "max_neighbours must be >= 1";
+        return false;
+    }
+    p.kmax = kmax;
+
+    // Template (covariance_field.h:61-106).  Every provider gives the same
+    // value inside the +-radius box, so the unclipped model is used.
+    const int rx = p.r[0], ry = p.r[1], rz = p.r[2];
+    const int xd = 2 * rx + 1, yd = 2 * ry + 1;
+    std::vector<double> data;
+    data.reserve((size_t)xd * yd * (2 * rz + 1));
+    struct V { int x, y, z; };
+    std::vector<V> vec;
+    const double zero_cov = model_cov(cov, p.transform, 0, 0, 0);
+    const double threshold = zero_cov / 100;
+    for (int z = -rz; z <= rz; ++z)
+        for (int y = -ry; y <= ry; ++y)
+            for (int x = -rx; x <= rx; ++x) {
+                double v = model_cov(cov, p.transform, 0.0 - x, 0.0 - y, 0.0 - z);
+                data.push_back(v);
+                if (v > threshold) vec.push_back(V{-x, -y, -z});
+            }
+    auto key = [&](const V& o) {
+        return data[(size_t)(o.z + rz) * yd * xd + (size_t)(o.y + ry) * xd + (size_t)(o.x + rx)];
+    };
+    std::sort(vec.begin(), vec.end(), [&](const V& a, const V& b) { return key(a) > key(b); });
+    for (size_t i = 0; i < vec.size(); ++i)
+        if (model_cov(cov, p.transform, 0.0 - vec[i].x, 0.0 - vec[i].y, 0.0 - vec[i].z) < threshold) {
+            vec.resize(i);
+            break;
+        }
+    p.tdx.resize(vec.size());
+    p.tdy.resize(vec.size());
+    p.tdz.resize(vec.size());
+    for (size_t i = 0; i < vec.size(); ++i) {
+        p.tdx[i] = vec[i].x;
+        p.tdy[i] = vec[i].y;
+        p.tdz[i] = vec[i].z;
+    }
+    p.limit = (long long)(vec.size() / 250);
+
+    // Covariance table over [-W, W].
+    for (int i = 0; i < 3; ++i) p.W[i] = 4 * p.r[i] - 2 > 2 * p.r[i] ? 4 * p.r[i] - 2 : 2 * p.r[i];
+    const int sx = 2 * p.W[0] + 1, sy = 2 * p.W[1] + 1, sz = 2 * p.W[2] + 1;
+    p.ctab.assign((size_t)sx * sy * sz, 0.0);
+    const bool clip = kind != KIND_OK;
+    auto fill_planes = [&](int z0, int z1) {
+        for (int z = z0; z < z1; ++z) {
+            const int dz = z - p.W[2];
+            for (int y = 0; y < sy; ++y) {
+                const int dy = y - p.W[1];
+                for (int x = 0; x < sx; ++x) {
+                    const int dx = x - p.W[0];
+                    double v;
+                    if (clip && (dx < -rx || dx > rx || dy < -ry || dy > ry || dz < -rz || dz > rz))
+                        v = 0;
+                    else
+                        v = model_cov(cov, p.transform, (double)dx, (double)dy, (double)dz);
+                    p.ctab[((size_t)z * sy + y) * sx + x] = v;
+                }
+            }
+        }
+    };
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+    if ((size_t)sx * sy * sz < 65536) nt = 1;
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t) {
+        const int z0 = (int)((long long)sz * t / nt), z1 = (int)((long long)sz * (t + 1) / nt);
+        pool.emplace_back(fill_planes, z0, z1);
+    }
+    for (auto& th : pool) th.join();
+    return true;
+}
+
+}  // namespace hpglx

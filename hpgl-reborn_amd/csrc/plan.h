@@ -1,0 +1,53 @@
+// plan.h -- host-side, data-independent part of a grid-kriging call.
+//
+// A plan depends only on (kind, covariance model, radiuses, max_neighbours):
+// the ordered search template and the covariance table over integer offsets.
+// It is built once on the host (std::sort for the template order, glibc for
+// the covariance values, so both round exactly as the reference's do) and
+// then reused for every grid and data batch with the same parameters.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hpglx {
+
+enum Kind { KIND_OK = 0, KIND_SK = 1, KIND_MIK = 2 };
+
+struct CovParams {
+    int type;  // 0 spherical, 1 exponential, 2 gaussian
+    double ranges[3];
+    double angles[3];
+    double sill;
+    double nugget;
+};
+
+struct PlanHost {
+    Kind kind;
+    CovParams cov;
+    double transform[9];
+    int r[3];  // search radiuses
+    int kmax;  // max_neighbours
+    // Search template: offsets from the cell, in the reference's order.
+    std::vector<int> tdx, tdy, tdz;
+    // Covariance of the kind's provider over the box [-W, W] (x fastest):
+    // OK unclipped, SK / MIK zero outside the +-radius box.  W = 4r - 2 per
+    // axis covers every neighbour pair of either search path.
+    int W[3];
+    std::vector<double> ctab;
+    // Clusterizer capacity: |template| / 250.
+    long long limit;
+    double c0() const;  // C(0) = the sill
+};
+
+// Builds the plan; returns false with a message on invalid parameters.
+bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses[3], int kmax, std::string& err);
+
+// Evaluates the unclipped covariance model at offset v (used for scattered
+// points); returns NaN when the transform is invalid.
+double model_cov(const CovParams& cov, const double t[9], double v0, double v1, double v2);
+bool make_transform(const CovParams& cov, double t[9]);
+
+}  // namespace hpglx

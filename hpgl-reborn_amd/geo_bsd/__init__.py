@@ -1,0 +1,13 @@
+"""geo_bsd -- HPGL's kriging API (ordinary, simple and median indicator
+kriging on SugarboxGrid properties) computed by hand-written HIP kernels on
+an AMD Instinct MI355X (gfx950).
+
+Import-compatible with the reference package's kriging entry points:
+``from geo_bsd import *`` provides ordinary_kriging, simple_kriging,
+median_ik, simple_kriging_weights, SugarboxGrid, CovarianceModel,
+ContProperty, IndProperty and covariance.  ``geo_bsd.device`` adds the
+device-tensor entry points (variance output, mixed precision, multi-threshold
+indicator CDFs).
+"""
+from .geo import *  # noqa: F401,F403
+from . import validation  # noqa: F401

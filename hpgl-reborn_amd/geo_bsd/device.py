@@ -1,0 +1,221 @@
+"""Device-tensor entry points (PyTorch-ROCm tensors in, tensors out).
+
+These enqueue the HIP pipeline of libhpgl.so on torch's current stream and
+return without synchronising.  Tensors are flat (or any contiguous shape)
+over the grid in x-fastest order -- the order of the reference's F-order
+(nx, ny, nz) arrays.
+
+Beyond the reference's host entry points they return the kriging variance
+(computed but discarded by the reference, my_kriging_weights.h:175-194,
+338-358), offer the mixed-precision solver (FP32 Cholesky with FP64 residual
+refinement and an FP64 fallback) and multi-threshold indicator CDFs.
+"""
+import ctypes as C
+
+import numpy
+
+from .hpgl_wrap import _HPGLX_GRID_REQUEST, _hpgl_so
+
+FP64 = 0
+MIXED = 1
+KIND_OK, KIND_SK, KIND_MIK = 0, 1, 2
+_PREC = {"fp64": FP64, "mixed": MIXED, "fp32+refine": MIXED, FP64: FP64, MIXED: MIXED}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _check_dev(t, dtype, n, name):
+    # Only plain attributes reach _require, so an error report never has to
+    # format a device tensor.
+    _require(t is not None, f"{name} is required")
+    _require(t.device.type == "cuda", f"{name} must be a device tensor")
+    _require(t.dtype == dtype, f"{name} must be {dtype}, got {t.dtype}")
+    _require(t.is_contiguous(), f"{name} must be contiguous")
+    _require(t.numel() == n, f"{name} has {t.numel()} elements, the grid has {n}")
+
+
+def _require(ok, msg):
+    if not ok:
+        raise ValueError(msg)
+
+
+def _call(q, stream):
+    rc = _hpgl_so.hpglx_krige(C.byref(q), C.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("hpglx_krige: " + _hpgl_so.hpglx_last_error().decode())
+
+
+def _dims(grid):
+    if hasattr(grid, "x"):
+        return (int(grid.x), int(grid.y), int(grid.z))
+    return tuple(int(v) for v in grid)
+
+
+def krige(kind, grid, radiuses, max_neighbours, cov_model, mask, values=None, indicators=None, zfield=None,
+          thresholds=None, marginal=None, mean=None, precision="fp64", variance=True, est64=False, status=False,
+          neighbours=False, order_relations=False, pivot_tol=1e-3, refine_steps=2, stream=None):
+    """Grid kriging on device tensors; returns a dict of result tensors.
+
+    kind: KIND_OK, KIND_SK or KIND_MIK.  For MIK give ``indicators`` (uint8,
+    nind x N, value 1 = indicator set) or ``zfield`` + ``thresholds``
+    (indicator_k = z <= thresholds[k]), and ``marginal`` (nind floats).
+    """
+    torch = _torch()
+    nx, ny, nz = _dims(grid)
+    n = nx * ny * nz
+    _check_dev(mask, torch.uint8, n, "mask")
+    dev = mask.device
+    q = _HPGLX_GRID_REQUEST()
+    q.kind = int(kind)
+    q.precision = _PREC[precision]
+    q.dims[:] = [nx, ny, nz]
+    q.radiuses[:] = [int(r) for r in radiuses]
+    q.max_neighbours = int(max_neighbours)
+    q.cov.covariance_type = int(cov_model.type)
+    q.cov.ranges[:] = [float(r) for r in cov_model.ranges]
+    q.cov.angles[:] = [float(a) for a in cov_model.angles]
+    q.cov.sill = float(cov_model.sill)
+    q.cov.nugget = float(cov_model.nugget)
+    q.mask = _ptr(mask)
+    out = {}
+    keep = []
+    if kind in (KIND_OK, KIND_SK):
+        _check_dev(values, torch.float32, n, "values")
+        q.values = _ptr(values)
+        out["value"] = torch.empty(n, dtype=torch.float32, device=dev)
+        q.out_val = _ptr(out["value"])
+        nout = 1
+    else:
+        if zfield is not None:
+            _check_dev(zfield, torch.float32, n, "zfield")
+            thr = thresholds if torch.is_tensor(thresholds) else torch.tensor(
+                numpy.asarray(thresholds, numpy.float32), device=dev)
+            thr = thr.to(device=dev, dtype=torch.float32).contiguous()
+            keep.append(thr)
+            q.zfield = _ptr(zfield)
+            q.thresholds = _ptr(thr)
+            nind = thr.numel()
+        else:
+            nind = indicators.numel() // n
+            _check_dev(indicators, torch.uint8, nind * n, "indicators")
+            q.ind = _ptr(indicators)
+        marg = numpy.ascontiguousarray(numpy.asarray(marginal, numpy.float64).reshape(-1))
+        if marg.size != nind:
+            raise ValueError("one marginal probability per indicator is required")
+        keep.append(marg)
+        q.marginal = marg.ctypes.data_as(C.POINTER(C.c_double))
+        q.nind = nind
+        q.order_relations = 1 if order_relations else 0
+        out["prob"] = torch.empty((nind, n), dtype=torch.float32, device=dev)
+        out["class"] = torch.empty((nind, n), dtype=torch.uint8, device=dev)
+        q.out_prob = _ptr(out["prob"])
+        q.out_class = _ptr(out["class"])
+        nout = nind
+    if kind == KIND_SK:
+        q.auto_mean = 1 if mean is None else 0
+        q.mean = 0.0 if mean is None else float(mean)
+    out["mask"] = torch.empty(n, dtype=torch.uint8, device=dev)
+    q.out_mask = _ptr(out["mask"])
+    if variance:
+        out["variance"] = torch.empty(n, dtype=torch.float64, device=dev)
+        q.out_var = _ptr(out["variance"])
+    if est64:
+        out["est64"] = torch.empty((nout, n) if kind == KIND_MIK else n, dtype=torch.float64, device=dev)
+        q.out_est64 = _ptr(out["est64"])
+    if status:
+        out["status"] = torch.empty(n, dtype=torch.uint8, device=dev)
+        q.out_status = _ptr(out["status"])
+    if neighbours:
+        out["count"] = torch.empty(n, dtype=torch.int32, device=dev)
+        out["nboff"] = torch.empty((n, int(max_neighbours)), dtype=torch.int32, device=dev)
+        q.out_count = _ptr(out["count"])
+        q.out_nboff = _ptr(out["nboff"])
+    q.pivot_tol = float(pivot_tol)
+    q.refine_steps = int(refine_steps)
+    if stream is None:
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    _call(q, stream)
+    return out
+
+
+def ordinary_kriging(values, mask, grid, radiuses, max_neighbours, cov_model, precision="fp64", **kw):
+    """OK on device tensors (work of geo.ordinary_kriging, geo.py:572-607)."""
+    return krige(KIND_OK, grid, radiuses, max_neighbours, cov_model, mask, values=values, precision=precision, **kw)
+
+
+def simple_kriging(values, mask, grid, radiuses, max_neighbours, cov_model, mean=None, precision="fp64", **kw):
+    """SK on device tensors (work of geo.simple_kriging, geo.py:609-649)."""
+    return krige(KIND_SK, grid, radiuses, max_neighbours, cov_model, mask, values=values, mean=mean,
+                 precision=precision, **kw)
+
+
+def indicator_cdf(zfield, mask, grid, thresholds, marginal, radiuses, max_neighbours, cov_model, precision="fp64",
+                  order_relations=True, **kw):
+    """Median IK with several thresholds sharing one model: one solve per cell,
+    one CDF value per threshold, order relations corrected."""
+    return krige(KIND_MIK, grid, radiuses, max_neighbours, cov_model, mask, zfield=zfield, thresholds=thresholds,
+                 marginal=marginal, precision=precision, order_relations=order_relations, **kw)
+
+
+def unpack_offsets(nboff):
+    """Decodes packed neighbour offsets into (dx, dy, dz) integer arrays."""
+    p = numpy.asarray(nboff).astype(numpy.int64)
+    dx = (p & 2047) - 1024
+    dy = ((p >> 11) & 2047) - 1024
+    dz = ((p >> 22) & 1023) - 512
+    return dx, dy, dz
+
+
+def set_profiling(on):
+    _hpgl_so.hpglx_set_profiling(1 if on else 0)
+
+
+def last_timing():
+    """Milliseconds (search, solve, whole call) of the last profiled call."""
+    ms = (C.c_float * 3)()
+    if _hpgl_so.hpglx_last_timing(ms) != 0:
+        raise RuntimeError(_hpgl_so.hpglx_last_error().decode())
+    return tuple(float(v) for v in ms)
+
+
+def last_call_stats():
+    fb = C.c_uint(0)
+    cl = C.c_int(0)
+    if _hpgl_so.hpglx_last_call_stats(C.byref(fb), C.byref(cl)) != 0:
+        raise RuntimeError(_hpgl_so.hpglx_last_error().decode())
+    return {"fp64_fallbacks": fb.value, "cluster_path_cells": cl.value}
+
+
+def set_default_precision(precision):
+    """Precision of the host-array entry points (ordinary_kriging etc.)."""
+    _hpgl_so.hpglx_set_default_precision(_PREC[precision])
+
+
+def search_template(cov_model, radiuses):
+    """The ordered search template as an (n, 3) int array."""
+    from .hpgl_wrap import _HPGL_COV_PARAMS
+    cp = _HPGL_COV_PARAMS(covariance_type=int(cov_model.type), ranges=tuple(float(r) for r in cov_model.ranges),
+                          angles=tuple(float(a) for a in cov_model.angles), sill=float(cov_model.sill),
+                          nugget=float(cov_model.nugget))
+    r = (C.c_int * 3)(*[int(v) for v in radiuses])
+    n = _hpgl_so.hpglx_template(C.byref(cp), r, None, 0)
+    if n < 0:
+        raise RuntimeError(_hpgl_so.hpglx_last_error().decode())
+    out = numpy.zeros((n, 3), numpy.int32)
+    _hpgl_so.hpglx_template(C.byref(cp), r, out.ctypes.data_as(C.c_void_p), n)
+    return out
+
+
+def host_introsort_perm(keys):
+    """Host run of the device introsort (slot -> input position)."""
+    keys = numpy.ascontiguousarray(keys, numpy.float64)
+    perm = numpy.zeros(keys.size, numpy.int32)
+    _hpgl_so.hpglx_host_introsort_perm(keys.ctypes.data_as(C.c_void_p), keys.size, perm.ctypes.data_as(C.c_void_p))
+    return perm

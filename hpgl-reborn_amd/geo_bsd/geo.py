@@ -1,0 +1,321 @@
+"""geo_bsd kriging API on the MI355X library.
+
+Drop-in for the reference's grid-kriging functions (src/geo_bsd/geo.py):
+same names, arguments, property containers and results.  Arrays are numpy
+in Fortran order (x fastest), exactly as the reference expects; the native
+calls go through the reference's C ABI, now served by libhpgl.so (HIP).
+
+Differences from the reference, all deliberate:
+  * a native error (for example a zero covariance range) raises RuntimeError
+    instead of being stored silently (the reference's hpgl_wrap.py never
+    reads hpgl_get_last_exception_message for these calls);
+  * median_ik returns the same indicators, with cells that cannot be kriged
+    classified from the marginal probability (the reference reads an
+    uninitialised value there, median_ik.cpp:65-77).
+"""
+import ctypes as C
+
+import numpy
+
+from . import validation
+from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_MEDIAN_IK_PARAMS, _HPGL_OK_PARAMS,
+                        _HPGL_SHAPE, _HPGL_SK_PARAMS, _HPGL_COV_PARAMS, _hpgl_so, hpgl_output_handler,
+                        hpgl_progress_handler, last_error)
+from .validation import (CriticalValidationError, GridValidator, ParameterValidator, ValidationError,
+                         ValidationWarning)
+
+__all__ = ["ContProperty", "IndProperty", "SugarboxGrid", "CovarianceModel", "covariance", "ordinary_kriging",
+           "simple_kriging", "median_ik", "simple_kriging_weights", "calc_mean", "set_output_handler",
+           "set_progress_handler", "set_thread_num", "get_thread_num", "append_mask", "ValidationError",
+           "CriticalValidationError", "ValidationWarning"]
+
+
+def checkFWA(a):
+    """geo.py:140-144: Fortran-order, writable, aligned."""
+    assert a.flags["F"] and a.flags["W"] and a.flags["A"]
+
+
+class ContProperty:
+    """geo.py:215-239."""
+
+    def __init__(self, data, mask):
+        self.data = numpy.require(data, "float32", "F")
+        self.mask = numpy.require(mask, "uint8", "F")
+
+    def validate(self):
+        checkFWA(self.data)
+        checkFWA(self.mask)
+        assert self.data.shape == self.mask.shape
+
+    def fix_shape(self, grid):
+        if self.data.ndim != 3 and self.data.size == grid.x * grid.y * grid.z:
+            self.data = self.data.reshape((grid.x, grid.y, grid.z), order="F")
+        if self.mask.ndim != 3 and self.mask.size == grid.x * grid.y * grid.z:
+            self.mask = self.mask.reshape((grid.x, grid.y, grid.z), order="F")
+
+    def __getitem__(self, idx):
+        if idx == 0:
+            return self.data
+        if idx == 1:
+            return self.mask
+        raise RuntimeError("Index out of range.")
+
+
+class IndProperty:
+    """geo.py:241-260."""
+
+    def __init__(self, data, mask, indicator_count):
+        self.data = numpy.require(data, "uint8", "F")
+        self.mask = numpy.require(mask, "uint8", "F")
+        self.indicator_count = indicator_count
+        if numpy.sum(numpy.bitwise_and((numpy.asarray(mask) > 0), (numpy.asarray(data) >= indicator_count))) > 0:
+            raise RuntimeError("Property contains some indicators outside of [0..%s] range." % (indicator_count - 1))
+        assert numpy.asarray(data).shape == numpy.asarray(mask).shape
+
+    def validate(self):
+        checkFWA(self.data)
+        checkFWA(self.mask)
+        assert self.data.shape == self.mask.shape
+
+    def __getitem__(self, idx):
+        if idx == 0:
+            return self.data
+        if idx == 1:
+            return self.mask
+        if idx == 2:
+            return self.indicator_count
+        raise RuntimeError("Index out of range.")
+
+
+def append_mask(prop, mask):
+    """geo.py:270-273."""
+    infs = prop[1]
+    infs &= mask
+    infs.choose(-99, prop[0], out=prop[0])
+
+
+class covariance:
+    """geo.py:275-278."""
+    spherical = 0
+    exponential = 1
+    gaussian = 2
+
+
+class SugarboxGrid:
+    """geo.py:281-287."""
+
+    def __init__(self, x, y, z):
+        GridValidator.validate_grid_dimensions(x, y, z)
+        self.x = x
+        self.y = y
+        self.z = z
+
+
+class CovarianceModel:
+    """geo.py:289-298."""
+
+    def __init__(self, type=0, ranges=(0, 0, 0), angles=(0.0, 0.0, 0.0), sill=1.0, nugget=0.0):
+        self.type = type
+        self.ranges = ranges
+        self.angles = angles
+        self.sill = sill
+        self.nugget = nugget
+        ParameterValidator.validate_covariance_parameters(sill, nugget, ranges, angles)
+
+
+def _clone_prop(prop):
+    """geo.py:372-380: F-order copies of data and mask."""
+    data2 = prop.data.copy("F")
+    mask2 = prop.mask.copy("F")
+    if isinstance(prop, IndProperty):
+        return IndProperty(data2, mask2, prop.indicator_count)
+    return ContProperty(data2, mask2)
+
+
+def accepts_tuple(arg_name, arg_pos):
+    """geo.py:390-410: (data, mask) or (data, mask, count) tuples become properties."""
+    def tuple_to_prop(t):
+        if isinstance(t, tuple):
+            if len(t) == 3:
+                return IndProperty(*t)
+            if len(t) == 2:
+                return ContProperty(*t)
+            assert False
+        assert isinstance(t, (ContProperty, IndProperty))
+        return t
+
+    def decorator(f):
+        def new_f(*args, **kargs):
+            if arg_name in kargs:
+                kargs[arg_name] = tuple_to_prop(kargs[arg_name])
+            elif len(args) > arg_pos:
+                args = args[:arg_pos] + (tuple_to_prop(args[arg_pos]),) + args[arg_pos + 1:]
+            else:
+                assert False
+            return f(*args, **kargs)
+        new_f.__name__ = f.__name__
+        new_f.__doc__ = f.__doc__
+        return new_f
+    return decorator
+
+
+def _shape(dims):
+    sh = _HPGL_SHAPE()
+    for i in range(3):
+        sh.data[i] = int(dims[i])
+    sh.strides[0] = 1
+    sh.strides[1] = int(dims[0])
+    sh.strides[2] = int(dims[0]) * int(dims[1])
+    return sh
+
+
+def _grid_arrays(prop, grid):
+    n = grid.x * grid.y * grid.z
+    if prop.data.size != n:
+        raise RuntimeError("Invalid data size. Size of data = %s. Size of grid = %s" % (prop.data.size, n))
+    return _shape((grid.x, grid.y, grid.z))
+
+
+def _check_native(name):
+    msg = last_error()
+    if msg:
+        raise RuntimeError(f"{name}: {msg}")
+
+
+def _validate_common(grid, radiuses, max_neighbours, cov_model):
+    GridValidator.validate_grid_dimensions(grid.x, grid.y, grid.z)
+    valid_radiuses = ParameterValidator.validate_radius(radiuses, "radiuses")
+    ParameterValidator.validate_max_neighbors(max_neighbours)
+    ParameterValidator.validate_covariance_parameters(cov_model.sill, cov_model.nugget, cov_model.ranges,
+                                                      cov_model.angles)
+    return tuple(int(r) for r in valid_radiuses)
+
+
+def calc_mean(prop):
+    """geo.py:560-570."""
+    informed = numpy.asarray(prop.mask).reshape(-1, order="F") == 1
+    if not informed.any():
+        raise ValueError("calc_mean: no informed values (all masked)")
+    vals = numpy.asarray(prop.data).reshape(-1, order="F")[informed]
+    s = 0
+    for v in vals:
+        s += v
+    return s / int(informed.sum())
+
+
+@accepts_tuple("prop", 0)
+def ordinary_kriging(prop, grid, radiuses, max_neighbours, cov_model):
+    """Ordinary kriging of every uninformed cell (geo.py:572-607)."""
+    valid_radiuses = _validate_common(grid, radiuses, max_neighbours, cov_model)
+    out_prop = _clone_prop(prop)
+    okp = _HPGL_OK_PARAMS(covariance_type=cov_model.type, ranges=tuple(cov_model.ranges),
+                          angles=tuple(cov_model.angles), sill=cov_model.sill, nugget=cov_model.nugget,
+                          radiuses=valid_radiuses, max_neighbours=max_neighbours)
+    sh = _grid_arrays(prop, grid)
+    inp = _HPGL_CONT_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_float)),
+                                  mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
+    outp = _HPGL_CONT_MASKED_ARRAY(data=out_prop.data.ctypes.data_as(C.POINTER(C.c_float)),
+                                   mask=out_prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
+    _hpgl_so.hpgl_ordinary_kriging(C.byref(inp), C.byref(okp), C.byref(outp))
+    _check_native("ordinary_kriging")
+    return out_prop
+
+
+@accepts_tuple("prop", 0)
+def simple_kriging(prop, grid, radiuses, max_neighbours, cov_model, mean=None):
+    """Simple kriging of every uninformed cell (geo.py:609-649); mean=None
+    uses the mean of the informed values."""
+    valid_radiuses = _validate_common(grid, radiuses, max_neighbours, cov_model)
+    out_prop = _clone_prop(prop)
+    skp = _HPGL_SK_PARAMS(covariance_type=cov_model.type, ranges=tuple(cov_model.ranges),
+                          angles=tuple(cov_model.angles), sill=cov_model.sill, nugget=cov_model.nugget,
+                          radiuses=valid_radiuses, max_neighbours=max_neighbours, automatic_mean=(mean is None),
+                          mean=(mean if mean is not None else 0))
+    sh = _grid_arrays(prop, grid)
+    _hpgl_so.hpgl_simple_kriging(prop.data, prop.mask, C.byref(sh), C.byref(skp), out_prop.data, out_prop.mask,
+                                 C.byref(sh))
+    _check_native("simple_kriging")
+    return out_prop
+
+
+@accepts_tuple("prop", 0)
+def median_ik(prop, grid, marginal_probs, radiuses, max_neighbours, cov_model):
+    """Median indicator kriging with two indicators (geo.py:698-738)."""
+    valid_radiuses = _validate_common(grid, radiuses, max_neighbours, cov_model)
+    if len(marginal_probs) != 2:
+        raise ValueError("median_ik: marginal_probs must have exactly 2 elements")
+    for i, p in enumerate(marginal_probs):
+        ParameterValidator.validate_probability(p, f"marginal_probs[{i}]")
+    out_prop = _clone_prop(prop)
+    mp = _HPGL_MEDIAN_IK_PARAMS(covariance_type=cov_model.type, ranges=tuple(cov_model.ranges),
+                                angles=tuple(cov_model.angles), sill=cov_model.sill, nugget=cov_model.nugget,
+                                radiuses=valid_radiuses, max_neighbours=max_neighbours,
+                                marginal_probs=tuple(marginal_probs))
+    sh = _shape((grid.x, grid.y, grid.z))
+    inp = _HPGL_IND_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                 mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                 indicator_count=prop.indicator_count)
+    outp = _HPGL_IND_MASKED_ARRAY(data=out_prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                  mask=out_prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                  indicator_count=out_prop.indicator_count)
+    _hpgl_so.hpgl_median_ik(C.byref(inp), C.byref(mp), C.byref(outp))
+    _check_native("median_ik")
+    return out_prop
+
+
+def simple_kriging_weights(center_point, n_x, n_y, n_z, ranges=(100000, 100000, 100000), sill=1,
+                           cov_type=covariance.exponential, nugget=None, angles=None):
+    """SK weights of scattered neighbours around a point (geo.py:897-935)."""
+    if angles is None:
+        angles = (0, 0, 0)
+    if nugget is None:
+        nugget = 0
+    ParameterValidator.validate_covariance_parameters(sill, nugget, ranges, angles)
+    if len(n_x) != len(n_y) or len(n_x) != len(n_z):
+        raise RuntimeError("Invalid pointset. %s,%s,%s." % (len(n_x), len(n_y), len(n_z)))
+    for name, arr in (("n_x", n_x), ("n_y", n_y), ("n_z", n_z)):
+        a = numpy.asarray(arr, dtype="float32")
+        if numpy.any(numpy.isnan(a)) or numpy.any(numpy.isinf(a)):
+            raise ValueError(f"simple_kriging_weights: {name} contains NaN or infinite values")
+    covp = _HPGL_COV_PARAMS(covariance_type=cov_type, ranges=tuple(ranges), angles=tuple(angles), sill=sill,
+                            nugget=nugget)
+    weights = numpy.array([0] * len(n_x), dtype="float32")
+    rc = _hpgl_so.hpgl_simple_kriging_weights((C.c_float * 3)(*center_point), numpy.array(n_x, dtype="float32"),
+                                              numpy.array(n_y, dtype="float32"), numpy.array(n_z, dtype="float32"),
+                                              len(n_x), C.byref(covp), weights)
+    if rc != 0:
+        raise RuntimeError("simple_kriging_weights: " + last_error())
+    return weights
+
+
+_handlers = {}
+
+
+def set_output_handler(handler, param):
+    """geo.py:947-956."""
+    if handler is None:
+        _hpgl_so.hpgl_set_output_handler(C.cast(None, hpgl_output_handler), None)
+        _handlers.pop("output", None)
+    else:
+        h = hpgl_output_handler(handler)
+        _handlers["output"] = h
+        _hpgl_so.hpgl_set_output_handler(h, param)
+
+
+def set_progress_handler(handler, param):
+    """geo.py:958-966."""
+    if handler is None:
+        _hpgl_so.hpgl_set_progress_handler(C.cast(None, hpgl_progress_handler), None)
+        _handlers.pop("progress", None)
+    else:
+        h = hpgl_progress_handler(handler)
+        _handlers["progress"] = h
+        _hpgl_so.hpgl_set_progress_handler(h, param)
+
+
+def set_thread_num(n):
+    _hpgl_so.hpgl_set_thread_num(int(n))
+
+
+def get_thread_num():
+    return _hpgl_so.hpgl_get_thread_num()

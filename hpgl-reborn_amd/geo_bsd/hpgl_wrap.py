@@ -1,0 +1,136 @@
+"""ctypes binding of libhpgl.so (the MI355X kriging library).
+
+Mirrors the reference binding (src/geo_bsd/hpgl_wrap.py:40-455): the same C
+struct layouts, and the library is loaded from this package directory only
+(hpgl_wrap.py:187-290).  The library is the HIP build; there is no CPU
+fallback, and importing this module fails loudly when it is missing.
+"""
+import ctypes as C
+import os
+
+import numpy
+import numpy.ctypeslib as NC
+
+hpgl_output_handler = C.CFUNCTYPE(C.c_int, C.c_char_p, C.py_object)
+hpgl_progress_handler = C.CFUNCTYPE(C.c_int, C.c_char_p, C.c_int, C.py_object)
+
+
+class _HPGL_SHAPE(C.Structure):
+    _fields_ = [("data", C.c_int * 3), ("strides", C.c_int * 3)]
+
+
+class _HPGL_CONT_MASKED_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("mask", C.POINTER(C.c_ubyte)), ("shape", _HPGL_SHAPE)]
+
+
+class _HPGL_IND_MASKED_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_ubyte)), ("mask", C.POINTER(C.c_ubyte)), ("shape", _HPGL_SHAPE),
+                ("indicator_count", C.c_int)]
+
+
+class _HPGL_OK_PARAMS(C.Structure):
+    _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double), ("radiuses", C.c_int * 3),
+                ("max_neighbours", C.c_int)]
+
+
+class _HPGL_SK_PARAMS(C.Structure):
+    _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double), ("radiuses", C.c_int * 3),
+                ("max_neighbours", C.c_int), ("automatic_mean", C.c_ubyte), ("mean", C.c_double)]
+
+
+class _HPGL_MEDIAN_IK_PARAMS(C.Structure):
+    _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double), ("radiuses", C.c_int * 3),
+                ("max_neighbours", C.c_int), ("marginal_probs", C.c_double * 2)]
+
+
+class __hpgl_cov_params_t(C.Structure):
+    _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double)]
+
+
+_HPGL_COV_PARAMS = __hpgl_cov_params_t
+
+
+class _HPGLX_GRID_REQUEST(C.Structure):
+    """hpglx_grid_request_t (include/hpglx.h)."""
+    _fields_ = [
+        ("kind", C.c_int), ("precision", C.c_int), ("dims", C.c_int * 3), ("radiuses", C.c_int * 3),
+        ("max_neighbours", C.c_int), ("cov", _HPGL_COV_PARAMS),
+        ("values", C.c_void_p), ("mask", C.c_void_p), ("ind", C.c_void_p), ("zfield", C.c_void_p),
+        ("thresholds", C.c_void_p), ("nind", C.c_int), ("marginal", C.POINTER(C.c_double)),
+        ("auto_mean", C.c_int), ("mean", C.c_double),
+        ("out_val", C.c_void_p), ("out_mask", C.c_void_p), ("out_class", C.c_void_p), ("out_prob", C.c_void_p),
+        ("out_var", C.c_void_p), ("out_est64", C.c_void_p), ("out_status", C.c_void_p),
+        ("order_relations", C.c_int), ("out_count", C.c_void_p), ("out_nboff", C.c_void_p),
+        ("pivot_tol", C.c_double), ("refine_steps", C.c_int)]
+
+
+LIB_NAME = "libhpgl.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: build it with `python hpgl-reborn_amd/build.py` "
+                      "(there is no CPU fallback)")
+    return C.CDLL(LIB_PATH)
+
+
+_hpgl_so = _load()
+
+_hpgl_so.hpgl_get_last_exception_message.restype = C.c_char_p
+_hpgl_so.hpgl_get_last_exception_message.argtypes = []
+_hpgl_so.hpgl_set_output_handler.restype = None
+_hpgl_so.hpgl_set_output_handler.argtypes = [hpgl_output_handler, C.py_object]
+_hpgl_so.hpgl_set_progress_handler.restype = None
+_hpgl_so.hpgl_set_progress_handler.argtypes = [hpgl_progress_handler, C.py_object]
+_hpgl_so.hpgl_set_thread_num.restype = None
+_hpgl_so.hpgl_set_thread_num.argtypes = [C.c_int]
+_hpgl_so.hpgl_get_thread_num.restype = C.c_int
+_hpgl_so.hpgl_get_thread_num.argtypes = []
+
+_hpgl_so.hpgl_ordinary_kriging.restype = None
+_hpgl_so.hpgl_ordinary_kriging.argtypes = [C.POINTER(_HPGL_CONT_MASKED_ARRAY), C.POINTER(_HPGL_OK_PARAMS),
+                                           C.POINTER(_HPGL_CONT_MASKED_ARRAY)]
+_hpgl_so.hpgl_simple_kriging.restype = None
+_hpgl_so.hpgl_simple_kriging.argtypes = [
+    NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]), NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
+    C.POINTER(_HPGL_SHAPE), C.POINTER(_HPGL_SK_PARAMS),
+    NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]), NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
+    C.POINTER(_HPGL_SHAPE)]
+_hpgl_so.hpgl_simple_kriging_weights.restype = C.c_int
+_hpgl_so.hpgl_simple_kriging_weights.argtypes = [
+    (C.c_float * 3), NC.ndpointer(dtype=numpy.float32), NC.ndpointer(dtype=numpy.float32),
+    NC.ndpointer(dtype=numpy.float32), C.c_int, C.POINTER(__hpgl_cov_params_t), NC.ndpointer(dtype=numpy.float32)]
+_hpgl_so.hpgl_median_ik.restype = None
+_hpgl_so.hpgl_median_ik.argtypes = [C.POINTER(_HPGL_IND_MASKED_ARRAY), C.POINTER(_HPGL_MEDIAN_IK_PARAMS),
+                                    C.POINTER(_HPGL_IND_MASKED_ARRAY)]
+
+_hpgl_so.hpglx_krige.restype = C.c_int
+_hpgl_so.hpglx_krige.argtypes = [C.POINTER(_HPGLX_GRID_REQUEST), C.c_void_p]
+_hpgl_so.hpglx_template.restype = C.c_int
+_hpgl_so.hpglx_template.argtypes = [C.POINTER(__hpgl_cov_params_t), C.c_int * 3, C.c_void_p, C.c_int]
+_hpgl_so.hpglx_set_profiling.restype = None
+_hpgl_so.hpglx_set_profiling.argtypes = [C.c_int]
+_hpgl_so.hpglx_last_timing.restype = C.c_int
+_hpgl_so.hpglx_last_timing.argtypes = [C.POINTER(C.c_float)]
+_hpgl_so.hpglx_last_call_stats.restype = C.c_int
+_hpgl_so.hpglx_last_call_stats.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_int)]
+_hpgl_so.hpglx_set_default_precision.restype = None
+_hpgl_so.hpglx_set_default_precision.argtypes = [C.c_int]
+_hpgl_so.hpglx_host_introsort_perm.restype = None
+_hpgl_so.hpglx_host_introsort_perm.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+_hpgl_so.hpglx_last_error.restype = C.c_char_p
+_hpgl_so.hpglx_last_error.argtypes = []
+_hpgl_so.hpglx_version.restype = C.c_char_p
+_hpgl_so.hpglx_version.argtypes = []
+_hpgl_so.hpglx_unpack_offset.restype = None
+_hpgl_so.hpglx_unpack_offset.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+
+
+def last_error():
+    msg = _hpgl_so.hpgl_get_last_exception_message()
+    return msg.decode("utf-8", errors="replace") if msg else ""

@@ -1,0 +1,124 @@
+/* hpgl.h -- the reference's kriging C ABI, served by the MI355X library.
+ *
+ * libhpgl.so exports these symbols with the reference's names, struct
+ * layouts and calling convention, so the reference's ctypes binding
+ * (src/geo_bsd/hpgl_wrap.py) and geo.py load it unchanged.  Each entry point
+ * takes host buffers, copies them to the GPU, runs the HIP pipeline and copies
+ * the result back before returning.
+ *
+ * Citations are to the reference repository (itohnobue/hpgl-reborn).
+ */
+#ifndef HPGL_MI355X_HPGL_H
+#define HPGL_MI355X_HPGL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/geo_bsd/hpgl/api.h:18-22 */
+struct hpgl_shape_t {
+    int m_data[3];
+    int m_strides[3];
+};
+
+/* src/geo_bsd/hpgl/api.h:24-29 */
+struct hpgl_cont_masked_array_t {
+    float* m_data;
+    unsigned char* m_mask;
+    struct hpgl_shape_t m_shape;
+};
+
+/* src/geo_bsd/hpgl/api.h:31-37 */
+struct hpgl_ind_masked_array_t {
+    unsigned char* m_data;
+    unsigned char* m_mask;
+    struct hpgl_shape_t m_shape;
+    int m_indicator_count;
+};
+
+/* src/geo_bsd/hpgl/api.h:51-60 */
+struct hpgl_ok_params_t {
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+    int m_radiuses[3];
+    int m_max_neighbours;
+};
+
+/* src/geo_bsd/hpgl/api.h:62-73 */
+struct hpgl_sk_params_t {
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+    int m_radiuses[3];
+    int m_max_neighbours;
+    unsigned char m_automatic_mean;
+    double m_mean;
+};
+
+/* src/geo_bsd/hpgl/api.h:75-82 */
+struct hpgl_cov_params_t {
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+};
+
+/* src/geo_bsd/hpgl/api.h:98-108 */
+struct hpgl_median_ik_params_t {
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+    int m_radiuses[3];
+    int m_max_neighbours;
+    double m_marginal_probs[2];
+};
+
+/* Replaces api.cpp:38-41.  Message of the last failed call ("" if none). */
+char* hpgl_get_last_exception_message(void);
+
+/* Replace output.cpp:58-68.  Messages and progress of the kriging calls are
+ * routed to these callbacks; without them messages go to stdout prefixed
+ * "[LOG2]" as in the reference. */
+void hpgl_set_output_handler(int (*handler)(char* str, void* param), void* param);
+void hpgl_set_progress_handler(int (*handler)(char* stage, int percentage, void* param), void* param);
+
+/* Replace api.cpp:43-51.  Kept for ABI compatibility: the value is stored and
+ * returned; the GPU path does not use host threads. */
+void hpgl_set_thread_num(int n_threads);
+int hpgl_get_thread_num(void);
+
+/* Replaces api.cpp:253-293 (geo.ordinary_kriging).  Output buffers hold a
+ * clone of the input on entry; kriged cells get the estimate and mask 1,
+ * cells that cannot be kriged keep the clone. */
+void hpgl_ordinary_kriging(struct hpgl_cont_masked_array_t* input_data, struct hpgl_ok_params_t* params,
+                           struct hpgl_cont_masked_array_t* output_data);
+
+/* Replaces api.cpp:323-354 (geo.simple_kriging).  Cells that cannot be
+ * kriged get the mean. */
+void hpgl_simple_kriging(float* input_data, unsigned char* input_mask, struct hpgl_shape_t* input_data_shape,
+                         struct hpgl_sk_params_t* params, float* output_data, unsigned char* output_mask,
+                         struct hpgl_shape_t* output_data_shape);
+
+/* Replaces api.cpp:356-410 (geo.simple_kriging_weights): SK weights for
+ * float coordinates with the unclipped model.  Returns 0, or -1 on error. */
+int hpgl_simple_kriging_weights(float* center_coords, float* neighbours_x, float* neighbours_y, float* neighbours_z,
+                                int neighbours_count, struct hpgl_cov_params_t* params, float* weights);
+
+/* Replaces api.cpp:552-601 (geo.median_ik): every cell gets indicator 0/1
+ * (1 when the kriged P(data == 1) exceeds 0.5) and mask 1. */
+void hpgl_median_ik(struct hpgl_ind_masked_array_t* in_data, struct hpgl_median_ik_params_t* params,
+                    struct hpgl_ind_masked_array_t* out_data);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPGL_MI355X_HPGL_H */

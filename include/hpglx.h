@@ -1,0 +1,114 @@
+/* hpglx.h -- device-pointer entry points of the MI355X kriging library.
+ *
+ * These take device buffers (for example torch tensors' data_ptr()) and a
+ * HIP stream, enqueue the whole pipeline on that stream and return without
+ * synchronising.  They extend the reference ABI (include/hpgl.h) with what
+ * the reference computes but does not return (kriging variance, neighbour
+ * lists), with a mixed-precision solver and with multi-threshold indicator
+ * kriging.  Each cites the reference entry point whose work it performs.
+ *
+ * All entry points return 0 on success and -1 on error; hpglx_last_error()
+ * then holds the message.  Calls are stream-ordered: buffers handed to a call
+ * may be reused once the call's work on its stream has completed.  The
+ * library keeps one workspace per process; a call waits for the previous
+ * call's work before reusing it, so concurrent calls on different streams
+ * are serialised on the device.
+ */
+#ifndef HPGL_MI355X_HPGLX_H
+#define HPGL_MI355X_HPGLX_H
+
+#include "hpgl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { HPGLX_KIND_OK = 0, HPGLX_KIND_SK = 1, HPGLX_KIND_MIK = 2 };
+enum { HPGLX_FP64 = 0, HPGLX_MIXED = 1 };
+enum { HPGLX_KRIGED = 0, HPGLX_NO_NEIGHBOURS = 1, HPGLX_SINGULAR = 2, HPGLX_INFORMED = 3 };
+
+/* One grid-kriging request.  Arrays are device pointers over the grid in
+ * x-fastest order (the reference's F-order property arrays); any output may
+ * be null.  `marginal` is a host array. */
+typedef struct hpglx_grid_request_t {
+    int kind;                  /* HPGLX_KIND_* */
+    int precision;             /* HPGLX_FP64 or HPGLX_MIXED (FP32 Cholesky + FP64 refinement) */
+    int dims[3];               /* nx, ny, nz */
+    int radiuses[3];
+    int max_neighbours;        /* 1..64 */
+    struct hpgl_cov_params_t cov;
+    const float* values;       /* OK/SK: data */
+    const unsigned char* mask; /* nonzero = informed */
+    const unsigned char* ind;  /* MIK: nind indicator arrays (value 1 = indicator set) */
+    const float* zfield;       /* MIK threshold mode: field, indicator_k = (z <= thresholds[k]) */
+    const float* thresholds;   /* MIK threshold mode: device array of nind floats */
+    int nind;                  /* MIK: number of indicators / thresholds, 1..16 */
+    const double* marginal;    /* MIK: host array, marginal P(indicator_k = 1) (the SK mean) */
+    int auto_mean;             /* SK: mean of the informed values */
+    double mean;               /* SK: explicit mean */
+    float* out_val;            /* OK/SK estimate; cells not kriged follow the reference policy */
+    unsigned char* out_mask;
+    unsigned char* out_class;  /* MIK: nind x N indicator 0/1 (prob > 0.5) */
+    float* out_prob;           /* MIK: nind x N probabilities */
+    double* out_var;           /* kriging variance; -1 where not kriged, 0 on informed cells */
+    double* out_est64;         /* estimate accumulated in FP64 (nind x N for MIK) */
+    unsigned char* out_status; /* HPGLX_KRIGED ... HPGLX_INFORMED */
+    int order_relations;       /* MIK: correct the CDF (indicator_kriging.h:34-59) */
+    int* out_count;            /* neighbour count per cell (-1 on informed OK/SK cells) */
+    int* out_nboff;            /* N x max_neighbours packed offsets, see hpglx_unpack_offset */
+    double pivot_tol;          /* MIXED: re-solve in FP64 when min pivot < tol * C(0); <= 0 -> 1e-3 */
+    int refine_steps;          /* MIXED: FP64 residual refinement steps; < 0 -> 2 */
+} hpglx_grid_request_t;
+
+/* The work of hpgl_ordinary_kriging / hpgl_simple_kriging / hpgl_median_ik
+ * (api.cpp:253-293, 323-354, 552-601) on device buffers, enqueued on
+ * `stream` (a hipStream_t; null = the default stream). */
+int hpglx_krige(const hpglx_grid_request_t* req, void* stream);
+
+/* Convenience forms of hpglx_krige. */
+int hpglx_ordinary_kriging(const struct hpgl_cov_params_t* cov, const int radiuses[3], int max_neighbours,
+                           const int dims[3], const float* values, const unsigned char* mask, float* out_val,
+                           unsigned char* out_mask, double* out_var, int precision, void* stream);
+int hpglx_simple_kriging(const struct hpgl_cov_params_t* cov, const int radiuses[3], int max_neighbours,
+                         const int dims[3], const float* values, const unsigned char* mask, int auto_mean,
+                         double mean, float* out_val, unsigned char* out_mask, double* out_var, int precision,
+                         void* stream);
+/* Multi-threshold median IK on a continuous field: one weight solve per cell,
+ * nthr combines, then the order-relation correction of the CDF. */
+int hpglx_indicator_cdf(const struct hpgl_cov_params_t* cov, const int radiuses[3], int max_neighbours,
+                        const int dims[3], const float* zfield, const unsigned char* mask, int nthr,
+                        const float* thresholds, const double* marginal, float* out_cdf, int precision,
+                        void* stream);
+
+/* Decodes an entry of out_nboff into the neighbour's offset from the cell. */
+void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz);
+
+/* The ordered search template of a model (covariance_field.h:61-106):
+ * returns its length and writes up to `cap` dx,dy,dz triples. */
+int hpglx_template(const struct hpgl_cov_params_t* cov, const int radiuses[3], int* out_xyz, int cap);
+
+/* Profiling: when on, each call records HIP events around the search and
+ * the solve kernels; hpglx_last_timing waits for them and returns
+ * milliseconds {search, solve, whole call}. */
+void hpglx_set_profiling(int on);
+int hpglx_last_timing(float* ms3);
+
+/* Waits for the last call and reports how many systems the mixed-precision
+ * solver re-solved in FP64 and how many cells took the cluster path. */
+int hpglx_last_call_stats(unsigned int* fp64_fallbacks, int* cluster_path_cells);
+
+/* Precision used by the host-buffer entry points of hpgl.h (default FP64). */
+void hpglx_set_default_precision(int precision);
+
+/* Host run of the device introsort (for tests): perm[i] = input position of
+ * the element sorted to slot i, ordering keys descending. */
+void hpglx_host_introsort_perm(const double* keys, int n, int* perm);
+
+const char* hpglx_last_error(void);
+const char* hpglx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPGL_MI355X_HPGLX_H */

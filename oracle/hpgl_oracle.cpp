@@ -1,0 +1,633 @@
+// hpgl_oracle.cpp -- FP64 CPU oracle for the HPGL grid-kriging hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
+// entry point and the CPU-baseline leg of bench.py load it to compare the HIP
+// path against; the product library never links, loads or calls it.
+//
+// It is a clean-room restatement of what the reference computes for
+// geo_bsd.ordinary_kriging / simple_kriging / median_ik, written from a reading
+// of the reference sources (paths relative to the reference repository):
+//   covariance model     src/geo_bsd/hpgl/cov_model.h:17-77, 146-183
+//   clipped tables       src/geo_bsd/hpgl/precalculated_covariance.h:36-75,
+//                        src/geo_bsd/hpgl/covariance_field.h:144-162
+//   search template      src/geo_bsd/hpgl/covariance_field.h:61-106
+//   template walk        src/geo_bsd/hpgl/sugarbox_neighbour_lookup.h:37-62
+//   cluster path         src/geo_bsd/hpgl/sugarbox_indexed_neighbour_lookup.h:120-190,
+//                        src/geo_bsd/hpgl/clusterizer.cpp:14-189
+//   weights              src/geo_bsd/hpgl/my_kriging_weights.h:60-361
+//   combine              src/geo_bsd/hpgl/combiner.h:10-25
+//   drivers              src/geo_bsd/hpgl/cont_kriging.h:39-129,
+//                        ordinary_kriging.cpp:14-48, simple_kriging.cpp:40-75,
+//                        median_ik.cpp:24-90, calc_mean.cpp:6-30
+//   order relations      src/geo_bsd/hpgl/indicator_kriging.h:34-59
+//
+// The reference delegates three pieces of arithmetic to third-party code; the
+// oracle calls or restates them as follows:
+//   * std::sort (libstdc++, GCC 11.4 here) orders the search template and the
+//     cluster-path candidates.  Ties are frequent, so the oracle calls the very
+//     same std::sort with the same comparator on the same input sequence.
+//   * glibc exp/pow/sqrt produce the covariance keys; the oracle evaluates the
+//     same expressions with the same libm (built with -ffp-contract=off).
+//   * LAPACK dpotrf_/dpotrs_ (version unpinned upstream) factor and solve the
+//     SPD systems; the oracle uses a textbook unblocked Cholesky instead, which
+//     agrees with any LAPACK to rounding (checked against numpy in tests).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+const double kPi = 3.14159265358979323846264338327950288419;
+
+struct CovParams {
+    int type;  // 0 spherical, 1 exponential, 2 gaussian (covariance_type.h)
+    double ranges[3];
+    double angles[3];
+    double sill;
+    double nugget;
+};
+
+// cov_model.h:16-61: t = scale * ((rot_x * rot_y) * rot_z), each product a
+// plain i-k-j triple loop with the sum started at 0 (TNT matmult).
+void mat3_mul(const double a[9], const double b[9], double c[9]) {
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double s = 0;
+            for (int j = 0; j < 3; ++j) s = s + a[3 * i + j] * b[3 * j + k];
+            c[3 * i + k] = s;
+        }
+}
+
+bool make_transform(const CovParams& p, double t[9]) {
+    for (int i = 0; i < 3; ++i)
+        if (p.ranges[i] == 0) return false;
+    auto rad = [](double g) { return (g * kPi) / 180; };
+    double sc[9] = {0}, rz[9] = {0}, ry[9] = {0}, rx[9] = {0};
+    sc[0] = 1.0;
+    sc[4] = p.ranges[0] / p.ranges[1];
+    sc[8] = p.ranges[0] / p.ranges[2];
+    rz[0] = cos(rad(p.angles[0]));
+    rz[4] = cos(rad(p.angles[0]));
+    rz[1] = sin(rad(p.angles[0]));
+    rz[3] = -sin(rad(p.angles[0]));
+    rz[8] = 1;
+    ry[0] = cos(rad(p.angles[1]));
+    ry[8] = cos(rad(p.angles[1]));
+    ry[2] = -sin(rad(p.angles[1]));
+    ry[6] = sin(rad(p.angles[1]));
+    ry[4] = 1;
+    rx[4] = cos(rad(p.angles[2]));
+    rx[8] = cos(rad(p.angles[2]));
+    rx[5] = sin(rad(p.angles[2]));
+    rx[7] = -sin(rad(p.angles[2]));
+    rx[0] = 1;
+    double xy[9], xyz[9];
+    mat3_mul(rx, ry, xy);
+    mat3_mul(xy, rz, xyz);
+    mat3_mul(sc, xyz, t);
+    return true;
+}
+
+struct Model {
+    CovParams p;
+    double t[9];
+
+    // cov_model.h:63-77 (transfrom_and_norm) then :146-183 (the three models).
+    double operator()(double v0, double v1, double v2) const {
+        double acc = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double d = 0.0;
+            d += v0 * t[3 * i + 0];
+            d += v1 * t[3 * i + 1];
+            d += v2 * t[3 * i + 2];
+            acc += d * d;
+        }
+        double h = sqrt(acc);
+        if (h < 0.0001) return p.sill;
+        if (p.type == 2) return (p.sill - p.nugget) * exp(-3 * pow(h / p.ranges[0], 2));
+        if (p.type == 1) return (p.sill - p.nugget) * exp(-3 * h / p.ranges[0]);
+        if (h > p.ranges[0]) return 0;
+        return (p.sill - p.nugget) *
+               (1 - 1.5 * h / p.ranges[0] + 0.5 * pow(h / p.ranges[0], 3));
+    }
+};
+
+enum Kind { OK = 0, SK = 1, MIK = 2 };
+
+// The three covariance providers of the drivers.  OK: the model itself,
+// unclipped (ordinary_kriging.cpp:29).  SK: precalculated table over the
+// +-radius box, 0 outside (precalculated_covariance.h:67-75).  MIK: the
+// covariance field, also 0 outside the box (covariance_field.h:144-153).
+// The three are all even functions of the offset, evaluated bit-identically.
+struct Provider {
+    Kind kind;
+    Model m;
+    int r[3];
+    double operator()(int ax, int ay, int az, int bx, int by, int bz) const {
+        int dx = ax - bx, dy = ay - by, dz = az - bz;
+        if (kind != OK) {
+            if (dx < -r[0] || dx > r[0] || dy < -r[1] || dy > r[1] || dz < -r[2] || dz > r[2])
+                return 0;
+        }
+        return m(dx, dy, dz);
+    }
+};
+
+struct Off { int x, y, z; };
+
+// covariance_field.h:61-106.  Offsets -(x,y,z) are pushed for z, y, x
+// ascending over the box when C > C(0)/100; std::sort orders them by the
+// covariance looked up at the stored vector, descending.
+std::vector<Off> build_template(const Provider& cov, const int r[3]) {
+    const int xd = 2 * r[0] + 1, yd = 2 * r[1] + 1;
+    std::vector<double> data;
+    std::vector<Off> vec;
+    double threshold = cov(0, 0, 0, 0, 0, 0) / 100;
+    for (int z = -r[2]; z <= r[2]; ++z)
+        for (int y = -r[1]; y <= r[1]; ++y)
+            for (int x = -r[0]; x <= r[0]; ++x) {
+                double v = cov(0, 0, 0, x, y, z);
+                data.push_back(v);
+                if (v > threshold) vec.push_back(Off{-x, -y, -z});
+            }
+    auto key = [&](const Off& o) {
+        long long idx = (long long)(o.z + r[2]) * yd * xd + (long long)(o.y + r[1]) * xd + (o.x + r[0]);
+        return data[(size_t)idx];
+    };
+    std::sort(vec.begin(), vec.end(), [&](const Off& a, const Off& b) { return key(a) > key(b); });
+    double thr2 = cov(0, 0, 0, 0, 0, 0) / 100;
+    for (size_t i = 0; i < vec.size(); ++i)
+        if (cov(0, 0, 0, vec[i].x, vec[i].y, vec[i].z) < thr2) {
+            vec.resize(i);
+            break;
+        }
+    return vec;
+}
+
+struct Grid {
+    int nx, ny, nz;
+    long long n() const { return (long long)nx * ny * nz; }
+    // sugarbox_grid.h:52-58, sugarbox_grid.cpp:70-74
+    int index(int x, int y, int z) const {
+        if ((unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && (unsigned)z < (unsigned)nz)
+            return z * nx * ny + y * nx + x;
+        return -1;
+    }
+    void loc(int idx, int& x, int& y, int& z) const {
+        x = idx % nx;
+        y = idx % (nx * ny) / nx;
+        z = idx / (nx * ny);
+    }
+};
+
+// clusterizer.cpp:14-189.  Cluster (x/rx, y/ry, z/rz); cluster box dims
+// g/r + 2 (computed in double, truncated); membership test inclusive of the
+// upper bound; a cluster is "exceeded" once it holds more than `limit` nodes.
+struct Clusters {
+    int mx, my, mz;
+    int r[3];
+    size_t limit;
+    std::vector<int> count;
+    std::vector<std::vector<int>> nodes;
+    int index(int cx, int cy, int cz) const {
+        if (!(0 <= cx && cx <= mx && 0 <= cy && cy <= my && 0 <= cz && cz <= mz)) return -1;
+        return cz * mx * my + cy * mx + cx;
+    }
+    bool exceeded(int c) const { return (size_t)count[c] > limit; }
+};
+
+// Cholesky A = L L^T of a row-major SPD matrix (lower triangle used), then
+// the two triangular solves.  Fails like dpotrf when a pivot is <= 0 or NaN.
+bool cholesky(std::vector<double>& a, int n) {
+    for (int j = 0; j < n; ++j) {
+        double d = a[j * n + j];
+        for (int k = 0; k < j; ++k) d -= a[j * n + k] * a[j * n + k];
+        if (!(d > 0)) return false;
+        d = sqrt(d);
+        a[j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double s = a[i * n + j];
+            for (int k = 0; k < j; ++k) s -= a[i * n + k] * a[j * n + k];
+            a[i * n + j] = s / d;
+        }
+    }
+    return true;
+}
+
+void chol_solve(const std::vector<double>& l, int n, std::vector<double>& x) {
+    for (int i = 0; i < n; ++i) {
+        double s = x[i];
+        for (int k = 0; k < i; ++k) s -= l[i * n + k] * x[k];
+        x[i] = s / l[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = x[i];
+        for (int k = i + 1; k < n; ++k) s -= l[k * n + i] * x[k];
+        x[i] = s / l[i * n + i];
+    }
+}
+
+struct Setup {
+    Kind kind;
+    Provider cov;
+    Grid g;
+    int r[3];
+    int kmax;
+    std::vector<Off> tmpl;
+    Clusters cl;
+    const unsigned char* mask;
+    bool informed(int idx) const {
+        if (idx < 0 || idx >= (int)g.n()) return false;
+        return mask[idx] != 0;
+    }
+};
+
+bool setup(Setup& s, int kind, const CovParams* cp, const int radiuses[3], int kmax, int nx, int ny,
+           int nz, const unsigned char* mask) {
+    s.kind = (Kind)kind;
+    s.cov.kind = s.kind;
+    s.cov.m.p = *cp;
+    if (!make_transform(*cp, s.cov.m.t)) return false;
+    for (int i = 0; i < 3; ++i) {
+        s.r[i] = radiuses[i];
+        s.cov.r[i] = radiuses[i];
+        if (radiuses[i] < 1) return false;
+    }
+    s.g = Grid{nx, ny, nz};
+    s.kmax = kmax;
+    s.mask = mask;
+    s.tmpl = build_template(s.cov, s.r);
+    Clusters& c = s.cl;
+    for (int i = 0; i < 3; ++i) c.r[i] = s.r[i];
+    c.mx = (int)(nx / (double)s.r[0] + 2);
+    c.my = (int)(ny / (double)s.r[1] + 2);
+    c.mz = (int)(nz / (double)s.r[2] + 2);
+    // sugarbox_indexed_neighbour_lookup.h:126 (MAGIC_NUMBER_2 = 250)
+    c.limit = s.tmpl.size() / 250;
+    size_t nc = (size_t)c.mx * c.my * c.mz;
+    c.count.assign(nc, 0);
+    c.nodes.assign(nc, std::vector<int>());
+    for (int idx = 0; idx < (int)s.g.n(); ++idx) {
+        if (!mask[idx]) continue;
+        int x, y, z;
+        s.g.loc(idx, x, y, z);
+        int ci = c.index(x / s.r[0], y / s.r[1], z / s.r[2]);
+        c.count[ci]++;
+        if ((size_t)c.count[ci] <= c.limit) c.nodes[ci].push_back(idx);
+    }
+    return true;
+}
+
+struct Entry {
+    int index;
+    double cov;
+};
+
+// Neighbour search for one cell; returns the neighbour node indices in the
+// reference's order and whether the cluster path was taken.
+void find(const Setup& s, int node, std::vector<int>& out, bool& cluster_path) {
+    out.clear();
+    int cx, cy, cz;
+    s.g.loc(node, cx, cy, cz);
+    const Clusters& c = s.cl;
+    int kx = cx / s.r[0], ky = cy / s.r[1], kz = cz / s.r[2];
+    bool exceeded = false;
+    for (int k = kz - 1; k <= kz + 1; ++k)
+        for (int j = ky - 1; j <= ky + 1; ++j)
+            for (int i = kx - 1; i <= kx + 1; ++i) {
+                int ci = c.index(i, j, k);
+                if (ci >= 0 && c.exceeded(ci)) exceeded = true;
+            }
+    cluster_path = !exceeded;
+    if (exceeded) {
+        for (size_t t = 0; t < s.tmpl.size() && (int)out.size() < s.kmax; ++t) {
+            const Off& o = s.tmpl[t];
+            int idx = s.g.index(cx + o.x, cy + o.y, cz + o.z);
+            if (s.informed(idx)) out.push_back(idx);
+        }
+        return;
+    }
+    std::vector<Entry> cand;
+    double thr = s.cov(cx, cy, cz, cx, cy, cz) / 100;
+    for (int k = kz - 1; k <= kz + 1; ++k)
+        for (int j = ky - 1; j <= ky + 1; ++j)
+            for (int i = kx - 1; i <= kx + 1; ++i) {
+                int ci = c.index(i, j, k);
+                if (ci < 0) continue;
+                for (int nd : c.nodes[ci]) {
+                    int x, y, z;
+                    s.g.loc(nd, x, y, z);
+                    double v = s.cov(cx, cy, cz, x, y, z);
+                    if (v > thr) cand.push_back(Entry{nd, v});
+                }
+            }
+    std::sort(cand.begin(), cand.end(), [](const Entry& a, const Entry& b) { return a.cov > b.cov; });
+    int m = std::min((int)cand.size(), s.kmax);
+    for (int i = 0; i < m; ++i) out.push_back(cand[i].index);
+}
+
+// Status codes shared with the HIP path.
+enum Status { ST_KRIGED = 0, ST_NO_NEIGHBOURS = 1, ST_SINGULAR = 2, ST_INFORMED = 3 };
+
+struct CellResult {
+    int status;
+    double var;
+    std::vector<double> w;  // final weights
+};
+
+// my_kriging_weights.h:60-196 (SK) and :198-361 (OK), with variance.
+CellResult weights(const Setup& s, int node, const std::vector<int>& nb) {
+    CellResult res;
+    int n = (int)nb.size();
+    res.var = -1;
+    if (n <= 0) {
+        res.status = ST_NO_NEIGHBOURS;
+        return res;
+    }
+    int cx, cy, cz;
+    s.g.loc(node, cx, cy, cz);
+    std::vector<int> xs(n), ys(n), zs(n);
+    for (int i = 0; i < n; ++i) s.g.loc(nb[i], xs[i], ys[i], zs[i]);
+    std::vector<double> a((size_t)n * n), b(n);
+    for (int i = 0; i < n; ++i) {
+        for (int j = i; j < n; ++j) {
+            a[i * n + j] = s.cov(xs[i], ys[i], zs[i], xs[j], ys[j], zs[j]);
+            a[j * n + i] = a[i * n + j];
+        }
+        b[i] = s.cov(xs[i], ys[i], zs[i], cx, cy, cz);
+    }
+    if (!cholesky(a, n)) {
+        res.status = ST_SINGULAR;
+        return res;
+    }
+    double c0 = s.cov(cx, cy, cz, cx, cy, cz);
+    if (s.kind != OK) {
+        res.w = b;
+        chol_solve(a, n, res.w);
+        double v = c0;
+        for (int i = 0; i < n; ++i) v -= res.w[i] * b[i];
+        res.var = v < 0 ? 0 : v;
+        res.status = ST_KRIGED;
+        return res;
+    }
+    std::vector<double> sw = b, ones(n, 1.0);
+    chol_solve(a, n, sw);
+    chol_solve(a, n, ones);
+    double sum_sk = 0, sum_ones = 0;
+    for (int k = 0; k < n; ++k) {
+        sum_sk += sw[k];
+        sum_ones += ones[k];
+    }
+    if (std::fabs(sum_ones) < 1e-15) {
+        res.status = ST_SINGULAR;
+        return res;
+    }
+    double mu = (sum_sk - 1) / sum_ones;
+    res.w.resize(n);
+    for (int k = 0; k < n; ++k) res.w[k] = sw[k] - mu * ones[k];
+    double v = c0;
+    for (int i = 0; i < n; ++i) v -= res.w[i] * b[i];
+    v -= mu;
+    res.var = v < 0 ? 0 : v;
+    res.status = ST_KRIGED;
+    return res;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+    int type;
+    double ranges[3];
+    double angles[3];
+    double sill;
+    double nugget;
+} orc_cov_t;
+
+// Search template for the given model and radiuses.  Returns the number of
+// offsets (or -1 on invalid parameters); writes dx,dy,dz triples when `out`
+// is non-null (capacity `cap` triples) and the covariance keys when `keys`
+// is non-null.
+int orc_template(const orc_cov_t* cp, const int radiuses[3], int* out, double* keys, int cap) {
+    Setup s;
+    static const unsigned char dummy = 0;
+    if (!setup(s, OK, (const CovParams*)cp, radiuses, 1, 1, 1, 1, &dummy)) return -1;
+    int n = (int)s.tmpl.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (out) {
+            out[3 * i + 0] = s.tmpl[i].x;
+            out[3 * i + 1] = s.tmpl[i].y;
+            out[3 * i + 2] = s.tmpl[i].z;
+        }
+        if (keys) keys[i] = s.cov.m(-s.tmpl[i].x, -s.tmpl[i].y, -s.tmpl[i].z);
+    }
+    return n;
+}
+
+// Covariance between two integer grid offsets' difference (dx,dy,dz) for the
+// unclipped model; NaN on invalid parameters.
+double orc_cov(const orc_cov_t* cp, double dx, double dy, double dz) {
+    Model m;
+    m.p = *(const CovParams*)cp;
+    if (!make_transform(m.p, m.t)) return NAN;
+    return m(dx, dy, dz);
+}
+
+// Grid kriging over all cells.
+//   kind      0 = ordinary, 1 = simple, 2 = median indicator
+//   values    N floats (OK/SK: data; MIK: ignored, see ind)
+//   ind       MIK only: nind arrays of N bytes, indicator value per cell
+//   mask      N bytes, nonzero = informed
+//   means     SK: means[0] is the mean (auto_mean: computed as in
+//             calc_mean.cpp); MIK: means[k] is marginal P(I_k = 1)
+// Outputs (each may be null):
+//   out_val   N floats (OK/SK), the reference's float-accumulated estimate
+//   out_mask  N bytes
+//   out_prob  nind*N doubles (MIK), the double-accumulated probability
+//   out_est64 nind*N doubles, the estimate m + sum w (v - m) in double
+//   out_var   N doubles, kriging variance (-1 when not kriged)
+//   out_count N ints, number of neighbours
+//   out_nb    N*kmax ints, neighbour node indices in order, -1 padded
+//   out_path  N bytes, 1 where the cluster path was taken
+//   out_status N ints, see Status
+// Returns 0, or -1 on invalid parameters.
+int orc_krige(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax, int nx, int ny, int nz,
+              const float* values, const unsigned char* ind, int nind, const unsigned char* mask,
+              const double* means, int auto_mean, double* mean_used, float* out_val,
+              unsigned char* out_mask, double* out_prob, double* out_est64, double* out_var,
+              int* out_count, int* out_nb, unsigned char* out_path, int* out_status) {
+    Setup s;
+    if (kmax < 1) return -1;
+    if (!setup(s, kind, (const CovParams*)cp, radiuses, kmax, nx, ny, nz, mask)) return -1;
+    const int N = (int)s.g.n();
+    if (kind == MIK && nind < 1) return -1;
+    double mean = 0;
+    if (kind == SK) {
+        if (auto_mean) {
+            // calc_mean.cpp:6-30: serial double sum over informed cells.
+            int cnt = 0;
+            double sum = 0;
+            for (int i = 0; i < N; ++i)
+                if (mask[i]) {
+                    sum += values[i];
+                    cnt += 1;
+                }
+            mean = cnt == 0 ? 0 : sum / cnt;
+        } else {
+            mean = means[0];
+        }
+        if (mean_used) *mean_used = mean;
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int idx = 0; idx < N; ++idx) {
+        std::vector<int> nb;
+        bool cpath = false;
+        if (out_count) out_count[idx] = 0;
+        if (out_path) out_path[idx] = 0;
+        if (out_nb)
+            for (int i = 0; i < kmax; ++i) out_nb[(size_t)idx * kmax + i] = -1;
+        if (kind != MIK && mask[idx]) {
+            // cont_kriging.h:97-100: informed cells are copied.
+            if (out_val) out_val[idx] = values[idx];
+            if (out_mask) out_mask[idx] = 1;
+            if (out_var) out_var[idx] = 0;
+            if (out_est64) out_est64[idx] = values[idx];
+            if (out_status) out_status[idx] = ST_INFORMED;
+            continue;
+        }
+        find(s, idx, nb, cpath);
+        if (out_count) out_count[idx] = (int)nb.size();
+        if (out_path) out_path[idx] = cpath ? 1 : 0;
+        if (out_nb)
+            for (size_t i = 0; i < nb.size(); ++i) out_nb[(size_t)idx * kmax + i] = nb[i];
+        CellResult cr;
+        if (nb.empty()) {
+            cr.status = ST_NO_NEIGHBOURS;
+            cr.var = -1;
+        } else {
+            cr = weights(s, idx, nb);
+        }
+        if (out_status) out_status[idx] = cr.status;
+        if (out_var) out_var[idx] = cr.status == ST_KRIGED ? cr.var : -1;
+        int n = (int)nb.size();
+        if (kind == MIK) {
+            // median_ik.cpp:60-78 with combine<uchar,double>.  On failure the
+            // reference leaves `prob` uninitialised; the oracle defines it as
+            // the marginal probability (the SK mean).
+            for (int k = 0; k < nind; ++k) {
+                const unsigned char* d = ind + (size_t)k * N;
+                float mf = (float)means[k];
+                double prob = (double)mf;
+                double e64 = (double)mf;
+                if (cr.status == ST_KRIGED) {
+                    for (int i = 0; i < n; ++i) {
+                        float iv = d[nb[i]] == 1 ? 1.0f : 0.0f;
+                        prob += (double)(cr.w[i] * (double)(iv - mf));
+                        e64 += cr.w[i] * ((double)iv - (double)mf);
+                    }
+                }
+                if (out_prob) out_prob[(size_t)k * N + idx] = prob;
+                if (out_est64) out_est64[(size_t)k * N + idx] = e64;
+            }
+            if (out_val) out_val[idx] = 0;
+            if (out_mask) out_mask[idx] = 1;
+            continue;
+        }
+        float mf = kind == SK ? (float)mean : 0.0f;
+        if (cr.status == ST_KRIGED) {
+            // combiner.h:10-25 with result_t = float.
+            float res = mf;
+            double e64 = (double)mf;
+            for (int i = 0; i < n; ++i) {
+                float v = values[nb[i]];
+                res += (float)(cr.w[i] * (v - mf));
+                e64 += cr.w[i] * ((double)v - (double)mf);
+            }
+            if (out_val) out_val[idx] = res;
+            if (out_mask) out_mask[idx] = 1;
+            if (out_est64) out_est64[idx] = e64;
+        } else if (kind == SK) {
+            // mean_on_failure (cont_kriging.h:80-94)
+            if (out_val) out_val[idx] = mf;
+            if (out_mask) out_mask[idx] = 1;
+            if (out_est64) out_est64[idx] = mf;
+        } else {
+            // undefined_on_failure: the output keeps the clone of the input.
+            if (out_val) out_val[idx] = values[idx];
+            if (out_mask) out_mask[idx] = mask[idx];
+            if (out_est64) out_est64[idx] = NAN;
+        }
+    }
+    return 0;
+}
+
+// indicator_kriging.h:34-59: clamp to [0,1], then one forward pass that
+// replaces each inverted adjacent pair by its average.  Operates in place on
+// `cells` consecutive CDFs of length k (float, as indicator_probability_t).
+void orc_order_relations(float* probs, int k, long long cells) {
+    for (long long c = 0; c < cells; ++c) {
+        float* p = probs + c * k;
+        for (int i = 0; i < k; ++i) {
+            if (p[i] < 0.0)
+                p[i] = 0.0f;
+            else if (p[i] > 1.0)
+                p[i] = 1.0f;
+        }
+        for (int i = 0; i + 1 < k; ++i)
+            if (p[i] > p[i + 1]) {
+                double avg = (p[i] + p[i + 1]) / 2.0;
+                p[i] = (float)avg;
+                p[i + 1] = (float)avg;
+            }
+    }
+}
+
+// Simple kriging weights at arbitrary float coordinates
+// (simple_kriging.cpp:18-38 via api.cpp:356-410): unclipped model.
+// Returns 0 on success, 1 when the system is singular (weights zeroed),
+// -1 on invalid parameters.
+int orc_sk_weights(const orc_cov_t* cp, const float center[3], const float* xs, const float* ys,
+                   const float* zs, int n, double* weights, double* variance) {
+    Model m;
+    m.p = *(const CovParams*)cp;
+    if (!make_transform(m.p, m.t)) return -1;
+    if (n <= 0) return 1;
+    std::vector<double> a((size_t)n * n), b(n);
+    auto c = [&](double ax, double ay, double az, double bx, double by, double bz) {
+        return m(ax - bx, ay - by, az - bz);
+    };
+    for (int i = 0; i < n; ++i) {
+        for (int j = i; j < n; ++j) {
+            a[i * n + j] = c(xs[i], ys[i], zs[i], xs[j], ys[j], zs[j]);
+            a[j * n + i] = a[i * n + j];
+        }
+        b[i] = c(xs[i], ys[i], zs[i], center[0], center[1], center[2]);
+    }
+    if (!cholesky(a, n)) {
+        for (int i = 0; i < n; ++i) weights[i] = 0;
+        return 1;
+    }
+    std::vector<double> w = b;
+    chol_solve(a, n, w);
+    double v = c(center[0], center[1], center[2], center[0], center[1], center[2]);
+    for (int i = 0; i < n; ++i) {
+        weights[i] = w[i];
+        v -= w[i] * b[i];
+    }
+    if (variance) *variance = v < 0 ? 0 : v;
+    return 0;
+}
+
+// libstdc++ std::sort permutation for a key array under "greater key first"
+// ordering: perm[i] = original position of the element sorted to slot i.
+void orc_std_sort_perm(const double* keys, int n, int* perm) {
+    std::vector<Entry> e(n);
+    for (int i = 0; i < n; ++i) e[i] = Entry{i, keys[i]};
+    std::sort(e.begin(), e.end(), [](const Entry& a, const Entry& b) { return a.cov > b.cov; });
+    for (int i = 0; i < n; ++i) perm[i] = e[i].index;
+}
+
+}  // extern "C"

@@ -1,0 +1,243 @@
+"""ctypes front-end to the CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+* ``libhpgl_oracle.so`` -- the clean-room FP64 restatement in hpgl_oracle.cpp.
+* ``_ref/libhpgl_ref.so`` -- the reference's own kriging core compiled from its
+  sources by ``make ref`` (present only where the reference tree was available
+  at build time; tests skip what needs it otherwise).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+import this module.  The product path never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libhpgl_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhpgl_ref.so")
+
+ST_KRIGED, ST_NO_NEIGHBOURS, ST_SINGULAR, ST_INFORMED = 0, 1, 2, 3
+OK, SK, MIK = 0, 1, 2
+
+
+class CovT(C.Structure):
+    _fields_ = [("type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double)]
+
+
+def make_cov(type_, ranges, angles=(0.0, 0.0, 0.0), sill=1.0, nugget=0.0):
+    return CovT(int(type_), (C.c_double * 3)(*ranges), (C.c_double * 3)(*angles), float(sill), float(nugget))
+
+
+def build(ref=False):
+    """Compile the oracle (and the reference checker when ``ref``)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+_orc = None
+_ref = None
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        _orc = C.CDLL(ORACLE_SO)
+        _orc.orc_template.restype = C.c_int
+        _orc.orc_cov.restype = C.c_double
+        _orc.orc_cov.argtypes = [C.POINTER(CovT), C.c_double, C.c_double, C.c_double]
+        _orc.orc_krige.restype = C.c_int
+        _orc.orc_sk_weights.restype = C.c_int
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_SO)
+        _ref.ref_quiet()
+    return _ref
+
+
+# ------------------------------------------------------------------ oracle
+def template(cov, radiuses):
+    r = (C.c_int * 3)(*radiuses)
+    n = lib().orc_template(C.byref(cov), r, None, None, 0)
+    if n < 0:
+        raise ValueError("invalid covariance parameters")
+    out = np.zeros((n, 3), np.int32)
+    keys = np.zeros(n, np.float64)
+    lib().orc_template(C.byref(cov), r, _p(out, C.c_int), _p(keys, C.c_double), n)
+    return out, keys
+
+
+def cov_value(cov, dx, dy, dz):
+    return lib().orc_cov(C.byref(cov), dx, dy, dz)
+
+
+def krige(kind, cov, radiuses, max_nb, shape, mask, values=None, indicators=None, means=None,
+          auto_mean=False, want_neighbours=True):
+    """Run the oracle over a whole grid.  Arrays are flat, x fastest."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    mask = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+    assert mask.size == n
+    if values is None:
+        values = np.zeros(n, np.float32)
+    values = np.ascontiguousarray(values, np.float32).reshape(-1)
+    nind = 0
+    ind = None
+    if kind == MIK:
+        ind = np.ascontiguousarray(indicators, np.uint8).reshape(-1, n)
+        nind = ind.shape[0]
+    m = np.zeros(max(nind, 1), np.float64)
+    if means is not None:
+        m[:len(np.atleast_1d(means))] = np.atleast_1d(means)
+    mean_used = C.c_double(0)
+    out = dict(
+        value=np.zeros(n, np.float32), mask=np.zeros(n, np.uint8),
+        var=np.zeros(n, np.float64), count=np.zeros(n, np.int32),
+        path=np.zeros(n, np.uint8), status=np.zeros(n, np.int32),
+        est64=np.zeros(max(nind, 1) * n, np.float64))
+    if kind == MIK:
+        out["prob"] = np.zeros(nind * n, np.float64)
+    if want_neighbours:
+        out["nb"] = np.zeros(n * max_nb, np.int32)
+    rc = lib().orc_krige(
+        C.c_int(kind), C.byref(cov), (C.c_int * 3)(*radiuses), C.c_int(max_nb), C.c_int(nx), C.c_int(ny),
+        C.c_int(nz), _p(values, C.c_float), _p(ind, C.c_ubyte), C.c_int(nind), _p(mask, C.c_ubyte),
+        _p(m, C.c_double), C.c_int(1 if auto_mean else 0), C.byref(mean_used), _p(out["value"], C.c_float),
+        _p(out["mask"], C.c_ubyte), _p(out.get("prob"), C.c_double), _p(out["est64"], C.c_double),
+        _p(out["var"], C.c_double), _p(out["count"], C.c_int), _p(out.get("nb"), C.c_int),
+        _p(out["path"], C.c_ubyte), _p(out["status"], C.c_int))
+    if rc != 0:
+        raise ValueError("oracle rejected the parameters")
+    if want_neighbours:
+        out["nb"] = out["nb"].reshape(n, max_nb)
+    if kind == MIK:
+        out["prob"] = out["prob"].reshape(nind, n)
+        out["est64"] = out["est64"].reshape(nind, n)
+    out["mean"] = mean_used.value
+    return out
+
+
+def order_relations(probs):
+    """indicator_kriging.h:34-59 on an (cells, K) float32 array, in place copy."""
+    p = np.ascontiguousarray(probs, np.float32).copy()
+    cells, k = p.shape
+    lib().orc_order_relations(_p(p, C.c_float), C.c_int(k), C.c_longlong(cells))
+    return p
+
+
+def sk_weights(cov, center, xs, ys, zs):
+    n = len(xs)
+    w = np.zeros(n, np.float64)
+    var = C.c_double(0)
+    rc = lib().orc_sk_weights(C.byref(cov), (C.c_float * 3)(*center), _p(np.ascontiguousarray(xs, np.float32), C.c_float),
+                              _p(np.ascontiguousarray(ys, np.float32), C.c_float), _p(np.ascontiguousarray(zs, np.float32), C.c_float),
+                              C.c_int(n), _p(w, C.c_double), C.byref(var))
+    return rc, w, var.value
+
+
+def std_sort_perm(keys):
+    keys = np.ascontiguousarray(keys, np.float64)
+    perm = np.zeros(keys.size, np.int32)
+    lib().orc_std_sort_perm(_p(keys, C.c_double), C.c_int(keys.size), _p(perm, C.c_int))
+    return perm
+
+
+# --------------------------------------------------------------- reference
+def _cov_args(cov):
+    return (C.c_int(cov.type), (C.c_double * 3)(*cov.ranges), (C.c_double * 3)(*cov.angles),
+            C.c_double(cov.sill), C.c_double(cov.nugget))
+
+
+def ref_ordinary_kriging(cov, radiuses, max_nb, shape, values, mask, threads=1):
+    r = ref_lib()
+    r.ref_set_threads(C.c_int(threads))
+    v = np.ascontiguousarray(values, np.float32).reshape(-1).copy()
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1).copy()
+    ov, om = v.copy(), m.copy()
+    rc = r.ref_ordinary_kriging(_p(v, C.c_float), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], *_cov_args(cov),
+                                (C.c_int * 3)(*radiuses), C.c_int(max_nb), _p(ov, C.c_float), _p(om, C.c_ubyte))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return ov, om
+
+
+def ref_simple_kriging(cov, radiuses, max_nb, shape, values, mask, mean=None, threads=1):
+    r = ref_lib()
+    r.ref_set_threads(C.c_int(threads))
+    v = np.ascontiguousarray(values, np.float32).reshape(-1).copy()
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1).copy()
+    ov, om = v.copy(), m.copy()
+    rc = r.ref_simple_kriging(_p(v, C.c_float), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], *_cov_args(cov),
+                              (C.c_int * 3)(*radiuses), C.c_int(max_nb), C.c_int(1 if mean is None else 0),
+                              C.c_double(0.0 if mean is None else mean), _p(ov, C.c_float), _p(om, C.c_ubyte))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return ov, om
+
+
+def ref_median_ik(cov, radiuses, max_nb, shape, data, mask, marginal_probs, threads=1):
+    r = ref_lib()
+    r.ref_set_threads(C.c_int(threads))
+    d = np.ascontiguousarray(data, np.uint8).reshape(-1).copy()
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1).copy()
+    od, om = d.copy(), m.copy()
+    rc = r.ref_median_ik(_p(d, C.c_ubyte), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], *_cov_args(cov),
+                         (C.c_int * 3)(*radiuses), C.c_int(max_nb), C.c_double(marginal_probs[0]),
+                         C.c_double(marginal_probs[1]), _p(od, C.c_ubyte), _p(om, C.c_ubyte))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return od, om
+
+
+def ref_neighbours(kind, cov, radiuses, max_nb, shape, mask, threads=8):
+    r = ref_lib()
+    r.ref_set_threads(C.c_int(threads))
+    n = int(np.prod(shape))
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1).copy()
+    counts = np.zeros(n, np.int32)
+    idx = np.zeros(n * max_nb, np.int32)
+    rc = r.ref_neighbours(C.c_int(kind), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], *_cov_args(cov),
+                          (C.c_int * 3)(*radiuses), C.c_int(max_nb), _p(counts, C.c_int), _p(idx, C.c_int))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return counts, idx.reshape(n, max_nb)
+
+
+def ref_template(cov, radiuses):
+    r = ref_lib()
+    cap = 1
+    for x in radiuses:
+        cap *= 2 * x + 1
+    out = np.zeros((cap, 3), np.int32)
+    n = r.ref_template(*_cov_args(cov), (C.c_int * 3)(*radiuses), _p(out, C.c_int), C.c_int(cap))
+    if n < 0:
+        raise RuntimeError("reference raised")
+    return out[:n]
+
+
+def ref_sk_weights(cov, center, xs, ys, zs):
+    r = ref_lib()
+    n = len(xs)
+    w = np.zeros(n, np.float64)
+    var = C.c_double(0)
+    got = r.ref_sk_weights(*_cov_args(cov), (C.c_float * 3)(*center), _p(np.ascontiguousarray(xs, np.float32), C.c_float),
+                           _p(np.ascontiguousarray(ys, np.float32), C.c_float), _p(np.ascontiguousarray(zs, np.float32), C.c_float),
+                           C.c_int(n), _p(w, C.c_double), C.byref(var))
+    return got, w, var.value

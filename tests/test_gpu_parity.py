@@ -1,0 +1,269 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Neighbour sets are compared element by element (bit-exact); estimates within
+1e-5 relative (float32 outputs) and, for the FP64 accumulations, 1e-9; the
+mixed-precision solver within 1e-6 absolute of the FP64 oracle (values are
+O(0.2)).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KIND = {"ok": 0, "sk": 1, "mik": 2}
+
+
+def _wl(name, scale):
+    from geo_bsd import synthetic as S
+    return {"sk": S.sk_c1, "ok": S.ok_c2, "mik": S.mik_c3, "wells": S.ok_wells_c4}[name](scale)
+
+
+def _cov(orc, wl):
+    c = wl.cov
+    return orc.make_cov(c["type"], c["ranges"], c["angles"], c["sill"], c["nugget"])
+
+
+def _model(wl):
+    from geo_bsd import CovarianceModel
+    return CovarianceModel(**wl.cov)
+
+
+def _oracle(orc, wl, thr_index=None):
+    kind = KIND[wl.kind]
+    cov = _cov(orc, wl)
+    if kind == 2:
+        ind = np.stack([(wl.values <= t).astype(np.uint8) for t in wl.thresholds])
+        return orc.krige(2, cov, wl.radiuses, wl.k, wl.shape, wl.mask, indicators=ind, means=wl.levels)
+    return orc.krige(kind, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values,
+                     means=[wl.get("mean", 0.0)], auto_mean=False)
+
+
+def _device(wl, precision="fp64"):
+    import torch
+    from geo_bsd import device as D
+    dev = torch.device("cuda:0")
+    m = torch.from_numpy(wl.mask).to(dev)
+    kind = KIND[wl.kind]
+    print(f"[gpu] {wl.name} shape={wl.shape} k={wl.k} precision={precision}", flush=True)
+    if kind == 2:
+        z = torch.from_numpy(wl.values).to(dev)
+        out = D.krige(2, wl.shape, wl.radiuses, wl.k, _model(wl), m, zfield=z, thresholds=wl.thresholds,
+                      marginal=wl.levels, precision=precision, est64=True, status=True, neighbours=True)
+    else:
+        v = torch.from_numpy(wl.values).to(dev)
+        out = D.krige(kind, wl.shape, wl.radiuses, wl.k, _model(wl), m, values=v, mean=wl.get("mean"),
+                      precision=precision, est64=True, status=True, neighbours=True)
+    torch.cuda.synchronize()
+    return {k: t.cpu().numpy() for k, t in out.items()}
+
+
+def _node_lists(wl, count, nboff):
+    from geo_bsd import device as D
+    nx, ny, _ = wl.shape
+    n = count.size
+    dx, dy, dz = D.unpack_offsets(nboff)
+    cells = np.arange(n, dtype=np.int64)[:, None]
+    nodes = cells + (dz * ny + dy) * nx + dx
+    k = nboff.shape[1]
+    valid = np.arange(k)[None, :] < np.maximum(count, 0)[:, None]
+    return np.where(valid, nodes, -1)
+
+
+CASES = [("sk", 1.0), ("ok", 0.4), ("mik", 0.25), ("wells", 0.25)]
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_neighbour_sets_bit_exact(orc, name, scale):
+    wl = _wl(name, scale)
+    o = _oracle(orc, wl)
+    g = _device(wl)
+    searched = o["status"] != orc.ST_INFORMED
+    assert np.array_equal(g["count"][searched], o["count"][searched])
+    nodes = _node_lists(wl, g["count"], g["nboff"])
+    assert np.array_equal(nodes[searched], o["nb"][searched].astype(np.int64))
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_estimates_fp64(orc, name, scale):
+    wl = _wl(name, scale)
+    o = _oracle(orc, wl)
+    g = _device(wl)
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    assert np.array_equal(g["mask"], o["mask"])
+    kr = o["status"] == orc.ST_KRIGED
+    if wl.kind == "mik":
+        assert np.allclose(g["est64"][:, kr], o["est64"][:, kr], rtol=0, atol=1e-9)
+        # reference probabilities are double-accumulated; outputs are float32
+        assert np.allclose(g["prob"][:, kr], o["prob"][:, kr].astype(np.float32), rtol=1e-5, atol=1e-6)
+    else:
+        ref = o["value"]
+        got = g["value"]
+        assert np.all(np.abs(got - ref) <= 1e-5 * np.maximum(1.0, np.abs(ref)))
+        assert np.allclose(g["est64"][kr], o["est64"][kr], rtol=1e-9, atol=1e-12)
+        # failure cells follow the reference policy exactly
+        assert np.array_equal(got[~kr], ref[~kr])
+    assert np.allclose(g["variance"][kr], o["var"][kr], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_mixed_precision_error(orc, name, scale):
+    wl = _wl(name, scale)
+    o = _oracle(orc, wl)
+    g = _device(wl, "mixed")
+    kr = o["status"] == orc.ST_KRIGED
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    err = np.abs(g["est64"][..., kr] - o["est64"][..., kr])
+    assert err.max() <= 1e-6, err.max()
+    assert np.allclose(g["variance"][kr], o["var"][kr], rtol=1e-6, atol=1e-7)
+
+
+def test_full_size_ok_fp64_and_mixed(orc):
+    """500k-cell ordinary kriging at its full size, both precisions."""
+    wl = _wl("ok", 1.0)
+    o = _oracle(orc, wl)
+    g = _device(wl)
+    kr = o["status"] == orc.ST_KRIGED
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    assert np.array_equal(_node_lists(wl, g["count"], g["nboff"])[kr], o["nb"][kr].astype(np.int64))
+    assert np.all(np.abs(g["value"] - o["value"]) <= 1e-5 * np.maximum(1.0, np.abs(o["value"])))
+    gm = _device(wl, "mixed")
+    assert np.abs(gm["est64"][kr] - o["est64"][kr]).max() <= 1e-6
+
+
+def test_reproducible_bitwise():
+    wl = _wl("ok", 0.3)
+    a = _device(wl)
+    b = _device(wl)
+    assert np.array_equal(a["value"], b["value"]) and np.array_equal(a["variance"], b["variance"])
+
+
+# ------------------------------------------------- reference ABI (host arrays)
+def test_geo_ordinary_kriging_matches_oracle(orc):
+    import geo_bsd
+    wl = _wl("ok", 0.3)
+    nx, ny, nz = wl.shape
+    prop = geo_bsd.ContProperty(wl.values.reshape((nx, ny, nz), order="F"), wl.mask.reshape((nx, ny, nz), order="F"))
+    out = geo_bsd.ordinary_kriging(prop, geo_bsd.SugarboxGrid(nx, ny, nz), wl.radiuses, wl.k, _model(wl))
+    o = _oracle(orc, wl)
+    got = out.data.reshape(-1, order="F")
+    assert np.array_equal(out.mask.reshape(-1, order="F"), o["mask"])
+    assert np.all(np.abs(got - o["value"]) <= 1e-5 * np.maximum(1.0, np.abs(o["value"])))
+    # the input property is not modified (geo.py clones it)
+    assert np.array_equal(prop.data.reshape(-1, order="F"), wl.values)
+
+
+def test_geo_simple_kriging_auto_mean_matches_oracle(orc):
+    import geo_bsd
+    wl = _wl("sk", 0.6)
+    nx, ny, nz = wl.shape
+    prop = (wl.values.reshape((nx, ny, nz), order="F"), wl.mask.reshape((nx, ny, nz), order="F"))
+    out = geo_bsd.simple_kriging(prop, geo_bsd.SugarboxGrid(nx, ny, nz), wl.radiuses, wl.k, _model(wl))
+    o = orc.krige(1, _cov(orc, wl), wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, auto_mean=True)
+    got = out.data.reshape(-1, order="F")
+    assert np.array_equal(out.mask.reshape(-1, order="F"), o["mask"])
+    assert np.all(np.abs(got - o["value"]) <= 1e-5 * np.maximum(1.0, np.abs(o["value"])))
+
+
+def test_golden_problem_7_2_on_gpu(golden_7_2):
+    """Reference-held SK output (solved_problems_book 7.2) through geo_bsd on the GPU."""
+    import geo_bsd
+    g = golden_7_2
+    cov = geo_bsd.CovarianceModel(type=g["cov"]["type"], ranges=tuple(g["cov"]["ranges"]), sill=g["cov"]["sill"])
+    for run in g["runs"]:
+        data = np.full(20, g["undefined"], np.float32)
+        mask = np.zeros(20, np.uint8)
+        data[run["cells"]] = run["values"]
+        mask[run["cells"]] = 1
+        out = geo_bsd.simple_kriging((data, mask), geo_bsd.SugarboxGrid(*g["grid"]), tuple(g["radiuses"]),
+                                     g["max_neighbours"], cov)
+        # cell 10 is a tie resolved by the C++ library's unstable sort; see
+        # test_oracle_cpu.test_golden_problem_7_2
+        keep = np.arange(20) != 10
+        exp_ = np.asarray(run["expected"], np.float32)
+        assert np.allclose(out.data[keep], exp_[keep], rtol=0, atol=2e-7), run["name"]
+        assert out.mask.all()
+
+
+def test_geo_median_ik_matches_oracle(orc):
+    import geo_bsd
+    wl = _wl("mik", 0.2)
+    nx, ny, nz = wl.shape
+    ind = (wl.values <= wl.thresholds[2]).astype(np.uint8)
+    prop = geo_bsd.IndProperty(ind.reshape((nx, ny, nz), order="F"), wl.mask.reshape((nx, ny, nz), order="F"), 2)
+    out = geo_bsd.median_ik(prop, geo_bsd.SugarboxGrid(nx, ny, nz), (0.5, 0.5), wl.radiuses, wl.k, _model(wl))
+    o = orc.krige(2, _cov(orc, wl), wl.radiuses, wl.k, wl.shape, wl.mask, indicators=ind[None], means=[0.5])
+    cls = (o["prob"][0].astype(np.float32) > 0.5).astype(np.uint8)
+    assert np.array_equal(out.data.reshape(-1, order="F"), cls)
+    assert out.mask.all()
+
+
+def test_simple_kriging_weights_matches_oracle(orc):
+    import geo_bsd
+    rng = np.random.default_rng(7)
+    for n in (1, 5, 24, 80):
+        xs, ys, zs = (rng.uniform(0, 20, n).astype(np.float32) for _ in range(3))
+        center = (10.0, 10.0, 10.0)
+        for t in (0, 1, 2):
+            w = geo_bsd.simple_kriging_weights(center, xs, ys, zs, ranges=(15, 12, 9), sill=1.0, cov_type=t,
+                                               nugget=0.1)
+            rc, wo, _ = orc.sk_weights(orc.make_cov(t, (15, 12, 9), (0, 0, 0), 1.0, 0.1), center, xs, ys, zs)
+            assert rc == 0
+            assert np.allclose(w, wo.astype(np.float32), rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------- edge cases
+def _ok_small(values, mask, shape, k=4, kind=0, mean=None):
+    import torch
+    from geo_bsd import CovarianceModel, device as D
+    dev = torch.device("cuda:0")
+    cm = CovarianceModel(type=0, ranges=(4, 4, 2), sill=1.0, nugget=0.1)
+    out = D.krige(kind, shape, (4, 4, 2), k, cm, torch.from_numpy(mask).to(dev),
+                  values=torch.from_numpy(values).to(dev), mean=mean, status=True)
+    torch.cuda.synchronize()
+    return {k2: t.cpu().numpy() for k2, t in out.items()}
+
+
+def test_all_uninformed_ok_keeps_clone():
+    shape = (6, 5, 3)
+    vals = np.full(90, -99.0, np.float32)
+    r = _ok_small(vals, np.zeros(90, np.uint8), shape)
+    assert (r["mask"] == 0).all() and (r["value"] == -99.0).all() and (r["status"] == 1).all()
+
+
+def test_all_uninformed_sk_gives_mean():
+    shape = (6, 5, 3)
+    vals = np.full(90, -99.0, np.float32)
+    r = _ok_small(vals, np.zeros(90, np.uint8), shape, kind=1, mean=0.25)
+    assert (r["mask"] == 1).all() and np.all(r["value"] == np.float32(0.25))
+
+
+def test_constant_and_zero_fields():
+    shape = (10, 9, 4)
+    rng = np.random.default_rng(3)
+    mask = (rng.random(360) < 0.2).astype(np.uint8)
+    for c in (42.0, 0.0):
+        vals = np.where(mask == 1, c, -99.0).astype(np.float32)
+        r = _ok_small(vals, mask, shape)
+        kr = r["status"] == 0
+        assert kr.any()
+        assert np.allclose(r["value"][kr], c, atol=1e-4)
+        assert np.array_equal(r["value"][mask == 1], vals[mask == 1])
+
+
+def test_single_informed_cell():
+    r = _ok_small(np.array([42.5], np.float32), np.array([1], np.uint8), (1, 1, 1))
+    assert r["value"][0] == np.float32(42.5) and r["mask"][0] == 1
+
+
+def test_invalid_parameters_raise():
+    import torch
+    from geo_bsd import CovarianceModel, device as D
+    dev = torch.device("cuda:0")
+    m = torch.ones(8, dtype=torch.uint8, device=dev)
+    v = torch.zeros(8, dtype=torch.float32, device=dev)
+    with pytest.raises(RuntimeError):
+        D.krige(0, (2, 2, 2), (1, 1, 1), 4, CovarianceModel(type=0, ranges=(0, 1, 1)), m, values=v)
+    with pytest.raises(RuntimeError):
+        D.krige(0, (2, 2, 2), (0, 1, 1), 4, CovarianceModel(type=0, ranges=(1, 1, 1)), m, values=v)

@@ -1,0 +1,347 @@
+"""Pins the CPU oracle: reference golden vectors, the reference's own compiled
+core (where it was built), closed forms, brute force and library routines."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+
+def _wl(name, scale):
+    from geo_bsd import synthetic as S
+    return {"sk": S.sk_c1, "ok": S.ok_c2, "mik": S.mik_c3, "wells": S.ok_wells_c4}[name](scale)
+
+
+def _cov(orc, wl):
+    c = wl.cov
+    return orc.make_cov(c["type"], c["ranges"], c["angles"], c["sill"], c["nugget"])
+
+
+# ------------------------------------------------------------- golden vector
+# Cell 10 of the 7.2 grid has two data at distance 3 (cells 7 and 13) competing
+# for the second neighbour slot.  The search template holds offsets -3 and +3
+# with equal covariance, and std::sort is not stable, so the winner is decided
+# by the C++ library's sort.  The book's output (2009 build) took cell 7; the
+# reference built with libstdc++ takes cell 13, and so do the oracle and the
+# GPU (checked bit-exactly against that build elsewhere).  The stored value of
+# cell 10 is therefore checked against SK with neighbours {8, 7}.
+TIE_CELL = 10
+TIE_NEIGHBOURS_BOOK = [8, 7]
+
+
+def golden_runs(g):
+    for run in g["runs"]:
+        data = np.full(20, g["undefined"], np.float32)
+        mask = np.zeros(20, np.uint8)
+        data[run["cells"]] = run["values"]
+        mask[run["cells"]] = 1
+        yield run, data, mask
+
+
+def test_golden_problem_7_2(orc, golden_7_2):
+    """solved_problems_book 7.2: SK, exponential (5,1,1), k=2, automatic mean."""
+    g = golden_7_2
+    cov = orc.make_cov(g["cov"]["type"], g["cov"]["ranges"], g["cov"]["angles"], g["cov"]["sill"], g["cov"]["nugget"])
+    for run, data, mask in golden_runs(g):
+        o = orc.krige(1, cov, g["radiuses"], g["max_neighbours"], g["grid"], mask, values=data, auto_mean=True)
+        exp_ = np.asarray(run["expected"], np.float32)
+        keep = np.arange(20) != TIE_CELL
+        # printed with 8 decimals: half a unit of the last place plus float32 rounding
+        assert np.allclose(o["value"][keep], exp_[keep], rtol=0, atol=1e-7), run["name"]
+        assert o["mask"].all()
+        assert list(o["nb"][TIE_CELL]) == [8, 13]
+        # the book's cell 10 = SK with the other tied neighbour
+        m = np.float32(o["mean"])
+        rc, w, _ = orc.sk_weights(cov, (TIE_CELL, 0, 0), np.array(TIE_NEIGHBOURS_BOOK, np.float32),
+                                  np.zeros(2, np.float32), np.zeros(2, np.float32))
+        alt = m + sum(np.float32(w[i] * (data[c] - m)) for i, c in enumerate(TIE_NEIGHBOURS_BOOK))
+        assert rc == 0 and abs(alt - exp_[TIE_CELL]) < 1e-7
+
+
+def test_golden_hand_values(orc, golden_7_2):
+    """Three cells of the 7.2 vector checked by hand from the closed forms."""
+    run = golden_7_2["runs"][0]
+    mean = np.float32(np.mean(np.asarray(run["values"], np.float32).astype(np.float64)))
+    # cell 19: no datum within radius 5 -> the mean
+    assert abs(run["expected"][19] - mean) < 1e-7
+    # cell 18: one neighbour (cell 13, distance 5): w = C(5)/C(0) = exp(-3)
+    v13 = np.float32(run["values"][run["cells"].index(13)])
+    assert abs(run["expected"][18] - (mean + math.exp(-3) * (v13 - mean))) < 1e-6
+
+
+# ----------------------------------------------------------- covariance forms
+@pytest.mark.parametrize("t", [0, 1, 2])
+def test_covariance_closed_forms(orc, t):
+    sill, nug, r0 = 2.0, 0.3, 8.0
+    cov = orc.make_cov(t, (r0, 4.0, 2.0), (0, 0, 0), sill, nug)
+    for h in (0.5, 3.0, 7.9, 8.0, 11.0):
+        x = h / r0
+        if t == 0:
+            exp_ = 0.0 if h > r0 else (sill - nug) * (1 - 1.5 * x + 0.5 * x ** 3)
+        elif t == 1:
+            exp_ = (sill - nug) * math.exp(-3 * h / r0)
+        else:
+            exp_ = (sill - nug) * math.exp(-3 * x * x)
+        assert orc.cov_value(cov, h, 0, 0) == pytest.approx(exp_, rel=1e-14, abs=1e-300)
+        # anisotropy: an offset along y is scaled by r0 / r1 = 2, along z by r0 / r2 = 4
+        assert orc.cov_value(cov, 0, h / 2, 0) == pytest.approx(exp_, rel=1e-14, abs=1e-300)
+        assert orc.cov_value(cov, 0, 0, h / 4) == pytest.approx(exp_, rel=1e-14, abs=1e-300)
+    # h < 1e-4 -> the sill, even with a nugget
+    assert orc.cov_value(cov, 0, 0, 0) == sill
+    assert orc.cov_value(cov, 5e-5, 0, 0) == sill
+
+
+def test_covariance_rotation(orc):
+    """A 90 degree azimuth swaps the roles of x and y (cov_model.h:37-41)."""
+    a = orc.make_cov(1, (10.0, 5.0, 5.0), (0, 0, 0), 1.0, 0.0)
+    b = orc.make_cov(1, (10.0, 5.0, 5.0), (90.0, 0, 0), 1.0, 0.0)
+    for d in (1.0, 3.0, 6.0):
+        assert orc.cov_value(b, d, 0, 0) == pytest.approx(orc.cov_value(a, 0, d, 0), rel=1e-12)
+        assert orc.cov_value(b, 0, d, 0) == pytest.approx(orc.cov_value(a, d, 0, 0), rel=1e-12)
+
+
+def test_zero_range_rejected(orc):
+    with pytest.raises(ValueError):
+        orc.template(orc.make_cov(0, (0.0, 1.0, 1.0)), (2, 2, 2))
+
+
+# ------------------------------------------------------------------ template
+@pytest.mark.parametrize("name", ["sk", "ok", "mik", "wells"])
+def test_template_properties(orc, name):
+    wl = _wl(name, 0.25)
+    cov = _cov(orc, wl)
+    t, keys = orc.template(cov, wl.radiuses)
+    thr = wl.cov["sill"] / 100
+    assert np.all(np.diff(keys) <= 0), "not sorted by covariance, descending"
+    assert np.all(keys > thr)
+    # exactly the box offsets above the threshold (brute force)
+    rx, ry, rz = wl.radiuses
+    want = {(x, y, z) for x in range(-rx, rx + 1) for y in range(-ry, ry + 1) for z in range(-rz, rz + 1)
+            if orc.cov_value(cov, x, y, z) > thr}
+    assert set(map(tuple, t.tolist())) == want
+    assert tuple(t[0]) == (0, 0, 0)
+
+
+@pytest.mark.parametrize("name", ["sk", "ok", "mik", "wells"])
+def test_template_matches_reference(ref, name):
+    wl = _wl(name, 0.25)
+    cov = _cov(ref, wl)
+    t, _ = ref.template(cov, wl.radiuses)
+    assert np.array_equal(t, ref.ref_template(cov, wl.radiuses))
+
+
+# ------------------------------------------------------------ neighbour sets
+def test_template_walk_brute_force(orc):
+    """Template path: the first k informed cells in template order."""
+    rng = np.random.default_rng(11)
+    shape = (12, 10, 6)
+    n = 720
+    mask = (rng.random(n) < 0.15).astype(np.uint8)
+    cov = orc.make_cov(1, (4.0, 4.0, 2.0), (0, 0, 0), 1.0, 0.1)
+    k = 6
+    o = orc.krige(0, cov, (4, 4, 2), k, shape, mask, values=np.zeros(n, np.float32))
+    t, _ = orc.template(cov, (4, 4, 2))
+    nx, ny, nz = shape
+    for cell in range(n):
+        if mask[cell] or o["path"][cell]:
+            continue
+        x, y, z = cell % nx, (cell // nx) % ny, cell // (nx * ny)
+        want = []
+        for dx, dy, dz in t:
+            px, py, pz = x + dx, y + dy, z + dz
+            if 0 <= px < nx and 0 <= py < ny and 0 <= pz < nz and mask[pz * nx * ny + py * nx + px]:
+                want.append(pz * nx * ny + py * nx + px)
+                if len(want) == k:
+                    break
+        assert o["count"][cell] == len(want)
+        assert list(o["nb"][cell][:len(want)]) == want
+
+
+CASES = [("sk", 1.0), ("ok", 0.4), ("mik", 0.25), ("wells", 0.25), ("wells", 0.4)]
+
+
+@pytest.mark.parametrize("name,scale", CASES)
+def test_neighbours_match_reference(ref, name, scale):
+    wl = _wl(name, scale)
+    kind = {"ok": 0, "sk": 1, "mik": 2}[wl.kind]
+    o = ref.krige(kind, _cov(ref, wl), wl.radiuses, wl.k, wl.shape, wl.mask, values=np.zeros(wl.mask.size, np.float32),
+                  indicators=wl.mask[None] if kind == 2 else None, means=[0.5])
+    cnt, idx = ref.ref_neighbours(kind, _cov(ref, wl), wl.radiuses, wl.k, wl.shape, wl.mask)
+    assert np.array_equal(cnt, o["count"])
+    assert np.array_equal(idx, o["nb"])
+    if name == "wells":
+        assert o["path"].sum() > 0, "the wells workload must exercise the cluster path"
+
+
+@pytest.mark.parametrize("name,scale", [("sk", 1.0), ("ok", 0.4), ("wells", 0.25)])
+def test_estimates_match_reference_bitwise(ref, name, scale):
+    wl = _wl(name, scale)
+    cov = _cov(ref, wl)
+    if wl.kind == "ok":
+        rv, rm = ref.ref_ordinary_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, threads=4)
+        o = ref.krige(0, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values)
+    else:
+        rv, rm = ref.ref_simple_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, mean=wl.mean, threads=4)
+        o = ref.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, means=[wl.mean])
+    assert np.array_equal(rm, o["mask"])
+    # float32 estimates agree to the last bit on this LAPACK (n <= 32 systems)
+    assert np.array_equal(rv, o["value"])
+
+
+def test_sk_auto_mean_matches_reference(ref):
+    wl = _wl("sk", 0.6)
+    cov = _cov(ref, wl)
+    rv, rm = ref.ref_simple_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, mean=None)
+    o = ref.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, auto_mean=True)
+    assert np.array_equal(rm, o["mask"])
+    assert np.array_equal(rv, o["value"])
+
+
+def test_median_ik_matches_reference(ref):
+    wl = _wl("mik", 0.25)
+    cov = _cov(ref, wl)
+    ind = (wl.values <= wl.thresholds[2]).astype(np.uint8)
+    rd, rm = ref.ref_median_ik(cov, wl.radiuses, wl.k, wl.shape, ind, wl.mask, (0.5, 0.5))
+    o = ref.krige(2, cov, wl.radiuses, wl.k, wl.shape, wl.mask, indicators=ind[None], means=[0.5])
+    kr = o["status"] == ref.ST_KRIGED
+    cls = (o["prob"][0].astype(np.float32) > 0.5).astype(np.uint8)
+    assert np.array_equal(rd[kr], cls[kr])
+    assert rm.all() and o["mask"].all()
+
+
+def test_sk_weights_match_reference(ref):
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 12, 40):
+        xs, ys, zs = (rng.uniform(0, 30, n).astype(np.float32) for _ in range(3))
+        for t in (0, 1, 2):
+            cov = ref.make_cov(t, (20.0, 15.0, 10.0), (30.0, 0, 0), 1.0, 0.2)
+            got, wr, vr = ref.ref_sk_weights(cov, (15, 15, 15), xs, ys, zs)
+            rc, wo, vo = ref.sk_weights(cov, (15, 15, 15), xs, ys, zs)
+            assert got == n and rc == 0
+            assert np.allclose(wr, wo, rtol=1e-10, atol=1e-12)
+            assert vr == pytest.approx(vo, rel=1e-10, abs=1e-12)
+
+
+# ------------------------------------------------------------------ algebra
+def test_sk_weights_vs_numpy(orc):
+    rng = np.random.default_rng(9)
+    cov = orc.make_cov(1, (10.0, 8.0, 6.0), (0, 0, 0), 1.5, 0.2)
+    for n in (1, 4, 24):
+        pts = rng.uniform(0, 10, (n, 3)).astype(np.float32)
+        c = np.array([5, 5, 5], np.float32)
+        a = np.array([[orc.cov_value(cov, *(pts[i].astype(np.float64) - pts[j])) for j in range(n)] for i in range(n)])
+        b = np.array([orc.cov_value(cov, *(pts[i].astype(np.float64) - c)) for i in range(n)])
+        w = np.linalg.solve(a, b)
+        rc, wo, var = orc.sk_weights(cov, c, pts[:, 0], pts[:, 1], pts[:, 2])
+        assert rc == 0
+        assert np.allclose(wo, w, rtol=1e-9, atol=1e-12)
+        assert var == pytest.approx(max(0.0, 1.5 - w @ b), rel=1e-9, abs=1e-12)
+
+
+def test_ok_weights_sum_to_one_and_variance(orc):
+    """OK: Sum w = 1 (unbiasedness), checked through a constant-plus-offset field,
+    and the variance against the bordered-system solution."""
+    rng = np.random.default_rng(2)
+    shape = (9, 9, 4)
+    n = 324
+    mask = (rng.random(n) < 0.2).astype(np.uint8)
+    cov = orc.make_cov(0, (5.0, 5.0, 3.0), (0, 0, 0), 1.0, 0.1)
+    base = rng.normal(0, 1, n).astype(np.float32)
+    o1 = orc.krige(0, cov, (4, 4, 2), 8, shape, mask, values=base)
+    o2 = orc.krige(0, cov, (4, 4, 2), 8, shape, mask, values=base + np.float32(10.0))
+    kr = o1["status"] == orc.ST_KRIGED
+    assert np.allclose(o2["est64"][kr] - o1["est64"][kr], 10.0, atol=1e-9)
+    # bordered system for one cell
+    cell = int(np.flatnonzero(kr)[0])
+    nb = o1["nb"][cell][: o1["count"][cell]]
+    nx, ny = shape[0], shape[1]
+    xyz = lambda i: np.array([i % nx, (i // nx) % ny, i // (nx * ny)], np.float64)
+    m = len(nb)
+    a = np.ones((m + 1, m + 1))
+    a[m, m] = 0
+    for i in range(m):
+        for j in range(m):
+            a[i, j] = orc.cov_value(cov, *(xyz(nb[i]) - xyz(nb[j])))
+    b = np.ones(m + 1)
+    b[:m] = [orc.cov_value(cov, *(xyz(nb[i]) - xyz(cell))) for i in range(m)]
+    sol = np.linalg.solve(a, b)
+    w, lag = sol[:m], sol[m]
+    assert o1["est64"][cell] == pytest.approx(float(w @ base[nb].astype(np.float64)), rel=1e-10)
+    # C(0) - w.b - mu, with mu = -lagrange of the bordered system
+    assert o1["var"][cell] == pytest.approx(max(0.0, 1.0 - w @ b[:m] - lag), rel=1e-9, abs=1e-12)
+
+
+def test_constant_and_zero_fields(orc):
+    rng = np.random.default_rng(4)
+    shape = (10, 8, 5)
+    n = 400
+    mask = (rng.random(n) < 0.1).astype(np.uint8)
+    cov = orc.make_cov(2, (6.0, 6.0, 3.0), (0, 0, 0), 1.0, 0.05)
+    for c in (42.0, 0.0):
+        vals = np.where(mask == 1, c, -99).astype(np.float32)
+        o = orc.krige(0, cov, (5, 5, 2), 10, shape, mask, values=vals)
+        kr = o["status"] == orc.ST_KRIGED
+        assert np.allclose(o["value"][kr], c, atol=0.1 if c else 1e-6)
+        assert np.array_equal(o["value"][mask == 1], vals[mask == 1]) and o["mask"][mask == 1].all()
+
+
+def test_all_uninformed(orc):
+    shape = (5, 4, 3)
+    vals = np.full(60, -99, np.float32)
+    cov = orc.make_cov(0, (3.0, 3.0, 3.0))
+    o = orc.krige(0, cov, (2, 2, 2), 4, shape, np.zeros(60, np.uint8), values=vals)
+    assert not o["mask"].any() and np.all(o["value"] == -99)
+    o = orc.krige(1, cov, (2, 2, 2), 4, shape, np.zeros(60, np.uint8), values=vals, means=[0.3])
+    assert o["mask"].all() and np.all(o["value"] == np.float32(0.3))
+
+
+def test_single_informed_cell(orc):
+    o = orc.krige(0, orc.make_cov(0, (3.0, 3.0, 3.0)), (1, 1, 1), 4, (1, 1, 1), np.ones(1, np.uint8),
+                  values=np.array([42.5], np.float32))
+    assert o["value"][0] == np.float32(42.5) and o["mask"][0] == 1
+
+
+def test_order_relations(orc):
+    p = np.array([[-0.1, 0.2, 0.1, 1.3, 0.9],
+                  [0.1, 0.3, 0.5, 0.7, 0.9],
+                  [0.5, 0.4, 0.3, 0.2, 0.1]], np.float32)
+    q = orc.order_relations(p)
+    assert np.allclose(q[0], [0.0, 0.15, 0.15, 0.95, 0.95])
+    assert np.allclose(q[1], p[1])
+    # one forward pass, as in indicator_kriging.h:50-58 (not a full isotonic fit)
+    assert np.allclose(q[2], [0.45, 0.375, 0.2875, 0.19375, 0.19375])
+
+
+# --------------------------------------------------------------- std::sort
+def test_library_introsort_matches_std_sort(orc):
+    """The GPU's sort (run on the host) permutes exactly like libstdc++ std::sort."""
+    from geo_bsd import device as D
+    rng = np.random.default_rng(0)
+    for trial in range(3000):
+        n = int(rng.integers(0, 300)) if trial % 10 else int(rng.integers(300, 2500))
+        nv = int(rng.integers(1, 6)) if trial % 3 else int(rng.integers(1, n + 2))
+        keys = rng.integers(0, nv, n).astype(np.float64)
+        if trial % 7 == 0:
+            keys = np.sort(keys)
+        if trial % 11 == 0:
+            keys = np.sort(keys)[::-1].copy()
+        assert np.array_equal(orc.std_sort_perm(keys), D.host_introsort_perm(keys)), (trial, n, nv)
+
+
+def test_library_introsort_heap_fallback(orc):
+    """Inputs that drive introsort past its depth limit (heapsort branch)."""
+    from geo_bsd import device as D
+    for n in (64, 257, 1000, 4097):
+        # organ-pipe and sawtooth patterns with ties defeat median-of-three
+        keys = np.concatenate([np.arange(n // 2), np.arange(n - n // 2)[::-1]]).astype(np.float64)
+        saw = (np.arange(n) % 7).astype(np.float64)
+        for k in (keys, saw, -keys):
+            assert np.array_equal(orc.std_sort_perm(k), D.host_introsort_perm(k))
+    # a median-of-3 killer sequence for the (first+1, mid, last-1) pivot
+    n = 2048
+    a = np.zeros(n)
+    idx = list(range(n))
+    for i in range(n):
+        a[i] = n - i
+    for k in (a, np.roll(a, 5), a[::-1].copy()):
+        assert np.array_equal(orc.std_sort_perm(k), D.host_introsort_perm(k))
