@@ -76,6 +76,15 @@ def _load():
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} is missing: build it with `python hpgl-reborn_amd/build.py` "
                       "(there is no CPU fallback)")
+    # PyTorch-ROCm ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Whichever copy is loaded first serves the whole
+    # process, so when torch is installed it is imported first: torch then runs
+    # on the runtime it was built with, and this library shares it (one device
+    # context, interoperable streams and pointers).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return C.CDLL(LIB_PATH)
 
 
