@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -232,7 +233,8 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     call.out_count = q->out_count;
     call.out_nboff = q->out_nboff;
     call.pivot_tol = q->pivot_tol > 0 ? q->pivot_tol : 1e-3;
-    call.refine_steps = q->refine_steps >= 0 ? q->refine_steps : 2;
+    call.refine_steps = q->refine_steps >= 0 ? q->refine_steps : 1;
+    if (const char* gs = std::getenv("HPGLX_SOLVER_G")) call.solver_g = std::atoi(gs);
     if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
         err = "values and mask are required";
         return -1;

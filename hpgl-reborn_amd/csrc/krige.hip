@@ -25,6 +25,7 @@
 #include "introsort.h"
 #include "krige_kernels.h"
 #include "krige_launch.h"
+#include "solve.h"
 
 #include <vector>
 
@@ -186,8 +187,10 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
 
 // --------------------------------------------------------------- clusters
 // Exclusive scan of the node counts of non-exceeded clusters (one block).
-__global__ void k_cluster_scan(int ncl, const int* cl_count, long long limit, int* cl_start, int* total) {
+__global__ void k_cluster_scan(int ncl, const int* cl_count, long long limit, int* cl_start, int* total,
+                               const int* n_cl_cells) {
     __shared__ int part[1024];
+    if (*n_cl_cells == 0) return;
     const int tid = threadIdx.x;
     const int per = (ncl + 1023) / 1024;
     const int b = tid * per, e = min(ncl, b + per);
@@ -213,7 +216,8 @@ __global__ void k_cluster_scan(int ncl, const int* cl_count, long long limit, in
 }
 
 __global__ void k_cluster_fill(GridDims g, const unsigned char* mask, const int* cl_count, long long limit,
-                               const int* cl_start, int* cl_cursor, int* cl_nodes) {
+                               const int* cl_start, int* cl_cursor, int* cl_nodes, const int* n_cl_cells) {
+    if (*n_cl_cells == 0) return;
     long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= g.n || !mask[cell]) return;
     int x, y, z;
@@ -226,7 +230,9 @@ __global__ void k_cluster_fill(GridDims g, const unsigned char* mask, const int*
 
 // Nodes were added to the reference clusters in ascending node order
 // (cont_kriging.h:63-69); sort each list back into that order.
-__global__ void k_cluster_sort(int ncl, const int* cl_count, long long limit, const int* cl_start, int* cl_nodes) {
+__global__ void k_cluster_sort(int ncl, const int* cl_count, long long limit, const int* cl_start, int* cl_nodes,
+                               const int* n_cl_cells) {
+    if (*n_cl_cells == 0) return;
     int ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= ncl) return;
     const int c = cl_count[ci];
@@ -324,269 +330,16 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
 }
 
 // -------------------------------------------------------------------- solve
-template <int G, typename T>
-__device__ __forceinline__ T gbc(T v, int src) {
-    return __shfl(v, src, G);
-}
-
-template <int G>
-__device__ __forceinline__ double gsum(double v) {
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-    return v;
-}
-
-// In-place Cholesky of the group's matrix, lane r owning row r.  On return
-// lane r holds L[r][c] in a[c] for c <= r and L[c][r] (column r) for c > r,
-// so both triangular solves need one broadcast per step.  The result is
-// group-uniform: false when a pivot is not positive (dpotrf info > 0).
-template <int NMAX, int G, typename T>
-__device__ __forceinline__ bool chol_rowcol(T (&a)[NMAX], T& dinv, T& minpiv, int r) {
-    bool ok = true;
-    minpiv = T(3.0e38);
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-        const T piv = gbc<G>(a[j], j);
-        ok = ok && (piv > T(0));
-        minpiv = piv < minpiv ? piv : minpiv;
-        const T s = sqrt(piv);
-        const T inv = T(1) / s;
-        if (r == j) dinv = inv;
-        const T l = (r == j) ? s : a[j] * inv;
-        if (r >= j) a[j] = l;
-#pragma unroll
-        for (int c = j + 1; c < NMAX; ++c) {
-            const T lc = gbc<G>(l, c);
-            if (r == j)
-                a[c] = lc;
-            else if (r > j)
-                a[c] = a[c] - l * lc;
-        }
-    }
-    return ok;
-}
-
-// Solves L L^T x = x in place (lane r holds x_r); factor from chol_rowcol.
-template <int NMAX, int G, typename T>
-__device__ __forceinline__ T chol_solve(const T (&a)[NMAX], T dinv, T x, int r) {
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-        const T y = gbc<G>(x * dinv, j);
-        if (r == j)
-            x = y;
-        else if (r > j)
-            x = x - a[j] * y;
-    }
-#pragma unroll
-    for (int i = NMAX - 1; i >= 0; --i) {
-        const T y = gbc<G>(x * dinv, i);
-        if (r == i)
-            x = y;
-        else if (r < i)
-            x = x - a[i] * y;
-    }
-    return x;
-}
-
-// FP32 solve of an FP64 right-hand side, result promoted back.
-template <int NMAX, int G>
-__device__ __forceinline__ double solve32(const float (&af)[NMAX], float dinv, double rhs, int r) {
-    return (double)chol_solve<NMAX, G, float>(af, dinv, (float)rhs, r);
-}
-
-// b - A x with A the FP64 row set (lane r holds row r).
-template <int NMAX, int G>
-__device__ __forceinline__ double residual(const double (&a64)[NMAX], double b, double x) {
-    double acc = b;
-#pragma unroll
-    for (int c = 0; c < NMAX; ++c) acc -= a64[c] * gbc<G>(x, c);
-    return acc;
-}
-
-template <int NMAX, int G, bool MIXED>
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-    const int r = threadIdx.x & (G - 1);
-    const long long cell = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const GridDims& g = a.g;
-    if (cell >= g.n) return;
-    const int cnt = a.count[cell];
-    const int kind = a.kind;
-    if (cnt == CNT_INFORMED) {
-        // cont_kriging.h:97-100: informed cells are copied, mask set to 1.
-        if (r == 0) {
-            const float v = a.values[cell];
-            if (a.out_val) a.out_val[cell] = v;
-            if (a.out_mask) a.out_mask[cell] = 1;
-            if (a.out_var) a.out_var[cell] = 0.0;
-            if (a.out_est64) a.out_est64[cell] = v;
-            if (a.out_status) a.out_status[cell] = ST_INFORMED;
-        }
-        return;
-    }
-    const int n = cnt < 0 ? 0 : cnt;
-    const double mean = (kind == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
-    const float mf = (float)mean;
-
-    int status = ST_KRIGED;
-    double w = 0.0, mu = 0.0, b = 0.0;
-    int dx = 0, dy = 0, dz = 0;
-    if (n == 0) {
-        status = ST_NO_NEIGHBOURS;
-    } else {
-        if (r < n) unpack_off(a.nboff[cell * a.kmax + r], dx, dy, dz);
-        const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
-        const long long center = ((long long)a.wz * SY + a.wy) * SX + a.wx;
-        const int tr = (dz * SY + dy) * SX + dx;
-        double a64[NMAX];
-#pragma unroll
-        for (int c = 0; c < NMAX; ++c) {
-            const int tc = gbc<G>(tr, c);
-            double v = (r == c) ? 1.0 : 0.0;
-            if (r < n && c < n) v = a.ctab[center + tr - tc];
-            a64[c] = v;
-        }
-        b = (r < n) ? a.ctab[center + tr] : 0.0;
-        const double one = (r < n) ? 1.0 : 0.0;
-        double s = 0.0, o = 0.0;
-        bool solved = false;
-        bool use64 = !MIXED;
-        if (MIXED) {
-            float af[NMAX];
-#pragma unroll
-            for (int c = 0; c < NMAX; ++c) af[c] = (float)a64[c];
-            float dinvf = 0.f, minpivf = 0.f;
-            const bool ok32 = chol_rowcol<NMAX, G, float>(af, dinvf, minpivf, r);
-            if (!ok32 || !((double)minpivf >= a.pivot_tol * a.c0)) {
-                use64 = true;
-                if (r == 0 && a.fallback_count) atomicAdd(a.fallback_count, 1u);
-            } else {
-                s = solve32<NMAX, G>(af, dinvf, b, r);
-                if (kind == 0) o = solve32<NMAX, G>(af, dinvf, one, r);
-                for (int it = 0; it < a.refine_steps; ++it) {
-                    s += solve32<NMAX, G>(af, dinvf, residual<NMAX, G>(a64, b, s), r);
-                    if (kind == 0) o += solve32<NMAX, G>(af, dinvf, residual<NMAX, G>(a64, one, o), r);
-                }
-                solved = true;
-            }
-        }
-        if (use64) {
-            double dinv = 0.0, minpiv = 0.0;
-            solved = chol_rowcol<NMAX, G, double>(a64, dinv, minpiv, r);
-            if (solved) {
-                s = chol_solve<NMAX, G, double>(a64, dinv, b, r);
-                if (kind == 0) o = chol_solve<NMAX, G, double>(a64, dinv, one, r);
-            }
-        }
-        if (!solved) {
-            status = ST_SINGULAR;
-        } else if (kind == 0) {
-            const double sum_sk = gsum<G>(s), sum_ones = gsum<G>(o);
-            if (fabs(sum_ones) < 1e-15) {
-                status = ST_SINGULAR;
-            } else {
-                mu = (sum_sk - 1) / sum_ones;
-                w = s - mu * o;
-            }
-        } else {
-            w = s;
-        }
-    }
-
-    const long long node = cell + ((long long)dz * g.ny + dy) * g.nx + dx;
-    if (kind != 2) {
-        const float v = (r < n) ? a.values[node] : 0.0f;
-        const float term = (float)(w * (double)(v - mf));
-        float acc = mf;
-#pragma unroll
-        for (int i = 0; i < NMAX; ++i) {
-            const float ti = gbc<G>(term, i);
-            if (i < n) acc += ti;
-        }
-        const double e64 = (double)mf + gsum<G>(w * ((double)v - (double)mf));
-        const double var = a.c0 - gsum<G>(w * b) - mu;
-        if (r == 0) {
-            if (status == ST_KRIGED) {
-                if (a.out_val) a.out_val[cell] = acc;
-                if (a.out_mask) a.out_mask[cell] = 1;
-                if (a.out_var) a.out_var[cell] = var < 0 ? 0.0 : var;
-                if (a.out_est64) a.out_est64[cell] = e64;
-            } else if (kind == 1) {
-                // mean_on_failure
-                if (a.out_val) a.out_val[cell] = mf;
-                if (a.out_mask) a.out_mask[cell] = 1;
-                if (a.out_var) a.out_var[cell] = -1.0;
-                if (a.out_est64) a.out_est64[cell] = (double)mf;
-            } else {
-                // undefined_on_failure: the output keeps the input clone
-                if (a.out_val) a.out_val[cell] = a.values[cell];
-                if (a.out_mask) a.out_mask[cell] = a.mask[cell];
-                if (a.out_var) a.out_var[cell] = -1.0;
-                if (a.out_est64) a.out_est64[cell] = __builtin_nan("");
-            }
-            if (a.out_status) a.out_status[cell] = (unsigned char)status;
-        }
-        return;
-    }
-
-    // Median indicator kriging: one weight set, one combine per threshold.
-    constexpr int KMAXIND = 16;
-    float probs[KMAXIND];
-    const double var = a.c0 - gsum<G>(w * b);
-#pragma unroll
-    for (int k = 0; k < KMAXIND; ++k) {
-        probs[k] = 0.f;
-        if (k < a.nind) {
-            const float pf = (float)a.marg[k];
-            float iv = 0.f;
-            if (r < n) {
-                if (a.zfield)
-                    iv = (a.zfield[node] <= a.thresholds[k]) ? 1.f : 0.f;
-                else
-                    iv = (a.ind[(long long)k * g.n + node] == 1) ? 1.f : 0.f;
-            }
-            const double term = w * (double)(iv - pf);
-            double acc = (double)pf;
-#pragma unroll
-            for (int i = 0; i < NMAX; ++i) {
-                const double ti = gbc<G>(term, i);
-                if (i < n && status == ST_KRIGED) acc += ti;
-            }
-            const double e64 = (double)pf + (status == ST_KRIGED ? gsum<G>(w * ((double)iv - (double)pf)) : 0.0);
-            probs[k] = (float)acc;
-            if (r == 0) {
-                if (a.out_est64) a.out_est64[(long long)k * g.n + cell] = e64;
-                if (a.out_class) a.out_class[(long long)k * g.n + cell] = ((float)acc <= 0.5f) ? 0 : 1;
-            }
-        }
-    }
-    if (r == 0) {
-        if (a.order_relations) {
-            for (int k = 0; k < KMAXIND; ++k)
-                if (k < a.nind) probs[k] = probs[k] < 0.0f ? 0.0f : (probs[k] > 1.0f ? 1.0f : probs[k]);
-            for (int k = 0; k + 1 < KMAXIND; ++k)
-                if (k + 1 < a.nind && probs[k] > probs[k + 1]) {
-                    const double avg = ((double)probs[k] + (double)probs[k + 1]) / 2.0;
-                    probs[k] = (float)avg;
-                    probs[k + 1] = (float)avg;
-                }
-        }
-        if (a.out_prob)
-            for (int k = 0; k < KMAXIND; ++k)
-                if (k < a.nind) a.out_prob[(long long)k * g.n + cell] = probs[k];
-        if (a.out_mask) a.out_mask[cell] = 1;
-        if (a.out_var) a.out_var[cell] = status == ST_KRIGED ? (var < 0 ? 0.0 : var) : -1.0;
-        if (a.out_status) a.out_status[cell] = (unsigned char)status;
-    }
-}
-
-// Explicit instantiations used by the launcher.
-#define HPGLX_SOLVE(NMAX, G)                                  \
+// (solve.h)
+#define HPGLX_SOLVE(NMAX, G)                                     \
     template __global__ void k_solve<NMAX, G, false>(SolveArgs); \
     template __global__ void k_solve<NMAX, G, true>(SolveArgs);
-HPGLX_SOLVE(8, 8)
-HPGLX_SOLVE(16, 16)
-HPGLX_SOLVE(24, 32)
-HPGLX_SOLVE(32, 32)
+HPGLX_SOLVE(8, 4)
+HPGLX_SOLVE(16, 4)
+HPGLX_SOLVE(24, 4)
+HPGLX_SOLVE(24, 8)
+HPGLX_SOLVE(32, 4)
+HPGLX_SOLVE(32, 8)
 #undef HPGLX_SOLVE
 
 // ------------------------------------------------------------------ launcher
@@ -665,12 +418,37 @@ namespace {
 template <int NMAX, int G>
 void launch_solve(const SolveArgs& sa, bool mixed, long long ncells, hipStream_t st) {
     const int block = 256;
-    const long long threads = ncells * G;
-    const unsigned grid = (unsigned)((threads + block - 1) / block);
+    const unsigned grid = (unsigned)((ncells * G + block - 1) / block);
     if (mixed)
         hipLaunchKernelGGL((k_solve<NMAX, G, true>), dim3(grid), dim3(block), 0, st, sa);
     else
         hipLaunchKernelGGL((k_solve<NMAX, G, false>), dim3(grid), dim3(block), 0, st, sa);
+}
+
+// FP64 form over a device-sized cell list (fallback of the mixed form).
+template <int NMAX, int G>
+void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    const int block = 256;
+    long long blocks = (max_cells * G + block - 1) / block;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((k_solve<NMAX, G, false>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+}
+
+// Group width per (NMAX, precision): G lanes share one cell.
+template <int NMAX, int G64, int G32>
+void dispatch_solve(const SolveArgs& sa, bool mixed, long long n, hipStream_t st) {
+    if (mixed) {
+        launch_solve<NMAX, G32>(sa, true, n, st);
+        SolveArgs fb = sa;
+        fb.cell_list = sa.fallback_cells;
+        fb.n_list = sa.n_fallback;
+        fb.fallback_cells = nullptr;
+        fb.n_fallback = nullptr;
+        launch_solve_list<NMAX, G64>(fb, n, st);
+    } else {
+        launch_solve<NMAX, G64>(sa, false, n, st);
+    }
 }
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -745,7 +523,9 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     int* n_cl_cells = counters + 0;
     int* cl_total = counters + 1;
     int* overflow = counters + 2;
-    unsigned* fallback = (unsigned*)(counters + 3);
+    int* n_fallback = counters + 3;
+    auto* fb_cells = (int*)ws.get("fb_cells", (size_t)n * 4, err);
+    if (!err.empty()) return -1;
 
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[0], st));
 
@@ -791,11 +571,11 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     hipLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, sa);
 
     hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(1024), 0, st, ncl, (const int*)cl_count, (long long)h.limit,
-                       cl_start, cl_total);
+                       cl_start, cl_total, (const int*)n_cl_cells);
     hipLaunchKernelGGL(k_cluster_fill, dim3(blocks_for(n, B)), dim3(B), 0, st, g, c.mask, (const int*)cl_count,
-                       (long long)h.limit, (const int*)cl_start, cl_cursor, cl_nodes);
+                       (long long)h.limit, (const int*)cl_start, cl_cursor, cl_nodes, (const int*)n_cl_cells);
     hipLaunchKernelGGL(k_cluster_sort, dim3(blocks_for(ncl, B)), dim3(B), 0, st, ncl, (const int*)cl_count,
-                       (long long)h.limit, (const int*)cl_start, cl_nodes);
+                       (long long)h.limit, (const int*)cl_start, cl_nodes, (const int*)n_cl_cells);
     ClusterSearchArgs ca;
     ca.g = g;
     ca.kmax = K;
@@ -854,17 +634,32 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.out_est64 = c.out_est64;
     so.out_status = c.out_status;
     so.order_relations = c.order_relations;
-    so.fallback_count = fallback;
+    so.cell_list = nullptr;
+    so.n_list = nullptr;
+    so.fallback_cells = fb_cells;
+    so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[2], st));
+    const int gsel = c.solver_g;
     if (K <= 8)
-        launch_solve<8, 8>(so, mixed, n, st);
+        dispatch_solve<8, 4, 4>(so, mixed, n, st);
     else if (K <= 16)
-        launch_solve<16, 16>(so, mixed, n, st);
-    else if (K <= 24)
-        launch_solve<24, 32>(so, mixed, n, st);
-    else
-        launch_solve<32, 32>(so, mixed, n, st);
+        dispatch_solve<16, 4, 4>(so, mixed, n, st);
+    else if (K <= 24) {
+        if (gsel == 8)
+            dispatch_solve<24, 8, 8>(so, mixed, n, st);
+        else if (gsel == 4)
+            dispatch_solve<24, 4, 4>(so, mixed, n, st);
+        else
+            dispatch_solve<24, 8, 4>(so, mixed, n, st);
+    } else {
+        if (gsel == 8)
+            dispatch_solve<32, 8, 8>(so, mixed, n, st);
+        else if (gsel == 4)
+            dispatch_solve<32, 4, 4>(so, mixed, n, st);
+        else
+            dispatch_solve<32, 8, 4>(so, mixed, n, st);
+    }
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[3], st));
     HPGLX_CHECK(hipGetLastError());
     return 0;

@@ -99,7 +99,10 @@ struct SolveArgs {
     double* out_est64;             // estimate in FP64 (nind x N for MIK)
     unsigned char* out_status;     // CellStatus (N)
     int order_relations;           // MIK CDF: apply the order-relation correction
-    unsigned int* fallback_count;  // mixed precision: systems re-solved in FP64
+    const int* cell_list;          // when set: krige only these cells
+    const int* n_list;             // its length (device)
+    int* fallback_cells;           // mixed precision: cells queued for the FP64 form
+    int* n_fallback;               // its length (device counter)
 };
 
 }  // namespace hpglx

@@ -56,7 +56,8 @@ struct GridCall {
     int* out_count = nullptr;               // debug: neighbour counts (device, N)
     int* out_nboff = nullptr;               // debug: packed offsets (device, N x kmax)
     double pivot_tol = 1e-3;
-    int refine_steps = 2;
+    int refine_steps = 1;
+    int solver_g = 0;                       // lanes per cell (4 or 8); 0 = default
     bool profile = false;
 };
 

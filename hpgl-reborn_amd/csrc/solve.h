@@ -1,0 +1,521 @@
+// solve.h -- per-cell covariance assembly, Cholesky, SK/OK solve and combine.
+//
+// Layout: a group of G lanes (G = 4 or 8) kriges one cell.  Lane t owns the
+// rows r = G*s + t of the n x n system (s = 0 .. NMAX/G - 1 "slots") and keeps,
+// for slot s, the columns 0 .. G*s + G - 1 in registers: a ragged lower
+// triangle with static indices, about NMAX^2 / (2G) + NMAX/2 values per lane.
+//   * Right-looking Cholesky: at step j the owner of row j broadcasts the
+//     pivot; every lane scales its column-j entries; then for each c > j the
+//     entry L[c][j] is broadcast from its owner and every slot holding column
+//     c does one FMA.  One broadcast feeds up to NMAX/G FMAs.
+//   * Forward solve: column-oriented, one broadcast per step.
+//   * Backward solve: each lane forms the partial dot product of its rows
+//     below i with column i, the group sums it, the owner divides.
+// Broadcasts and sums use DPP quad permutes (G = 4) or ds_swizzle (G = 8):
+// no LDS traffic, no address arithmetic.  Entries above the diagonal in a
+// lane's slot width are written with garbage and never read.
+//
+// FP64 form: everything in double.  Mixed form: the factor and both
+// triangular solves in float; residuals b - A x in double from the FP64
+// covariance table, corrections from the float factor; a cell whose
+// smallest float pivot falls under pivot_tol * C(0), or whose float factor
+// fails, is queued for the FP64 form.
+//
+// Reference algebra (src/geo_bsd/hpgl/my_kriging_weights.h):
+//   SK (:60-196)  A w = b, var = C(0) - w.b
+//   OK (:198-361) A s = b, A o = 1, mu = (sum s - 1) / sum o, w = s - mu o,
+//                 singular when |sum o| < 1e-15, var = C(0) - w.b - mu
+//   failure       Cholesky pivot <= 0 (dpotrf info > 0)
+// Combine (src/geo_bsd/hpgl/combiner.h:10-25): m + sum (float)(w_i (v_i - m))
+// accumulated in float in neighbour order (double for median IK).
+#pragma once
+
+#include <type_traits>
+
+#include "krige_kernels.h"
+
+namespace hpglx {
+
+// ------------------------------------------------------- group primitives
+template <int G>
+struct Grp;
+
+template <>
+struct Grp<4> {
+    // lane q of each quad to all four lanes (DPP quad_perm [q,q,q,q])
+    static __device__ __forceinline__ int bcast(int v, int q) {
+        switch (q & 3) {
+            case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xf, 0xf, true);
+            case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xf, 0xf, true);
+            case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xf, 0xf, true);
+            default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xf, 0xf, true);
+        }
+    }
+    // value of lane (t ^ m), m in {1, 2}
+    static __device__ __forceinline__ int xchg(int v, int m) {
+        if (m == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, true);
+        return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, true);
+    }
+};
+
+template <>
+struct Grp<8> {
+    // ds_swizzle bit mode within 32 lanes: lane' = ((lane & and) | or) ^ xor
+    static __device__ __forceinline__ int bcast(int v, int q) {
+        switch (q & 7) {
+            case 0: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (0 << 5));
+            case 1: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (1 << 5));
+            case 2: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (2 << 5));
+            case 3: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (3 << 5));
+            case 4: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (4 << 5));
+            case 5: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (5 << 5));
+            case 6: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (6 << 5));
+            default: return __builtin_amdgcn_ds_swizzle(v, 0x18 | (7 << 5));
+        }
+    }
+    static __device__ __forceinline__ int xchg(int v, int m) {
+        if (m == 1) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (1 << 10));
+        if (m == 2) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (2 << 10));
+        return __builtin_amdgcn_ds_swizzle(v, 0x1F | (4 << 10));
+    }
+};
+
+template <int G>
+__device__ __forceinline__ int gbc(int v, int q) {
+    return Grp<G>::bcast(v, q);
+}
+template <int G>
+__device__ __forceinline__ float gbc(float v, int q) {
+    return __int_as_float(Grp<G>::bcast(__float_as_int(v), q));
+}
+template <int G>
+__device__ __forceinline__ double gbc(double v, int q) {
+    const int lo = Grp<G>::bcast(__double2loint(v), q);
+    const int hi = Grp<G>::bcast(__double2hiint(v), q);
+    return __hiloint2double(hi, lo);
+}
+template <int G>
+__device__ __forceinline__ float gxchg(float v, int m) {
+    return __int_as_float(Grp<G>::xchg(__float_as_int(v), m));
+}
+template <int G>
+__device__ __forceinline__ double gxchg(double v, int m) {
+    const int lo = Grp<G>::xchg(__double2loint(v), m);
+    const int hi = Grp<G>::xchg(__double2hiint(v), m);
+    return __hiloint2double(hi, lo);
+}
+// Butterfly sum: every lane of the group ends with the same bits.
+template <int G, typename T>
+__device__ __forceinline__ T gsum(T v) {
+    v += gxchg<G>(v, 1);
+    v += gxchg<G>(v, 2);
+    if (G == 8) v += gxchg<G>(v, 4);
+    return v;
+}
+
+__device__ __forceinline__ float rsqrt_t(float x) {
+    return __builtin_amdgcn_rsqf(x);
+}
+// FP64 reciprocal square root: hardware estimate + two Newton steps.
+__device__ __forceinline__ double rsqrt_t(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = x * y;
+    double e = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+    h = x * y;
+    e = __builtin_fma(-h, y, 1.0);
+    return __builtin_fma(0.5 * y, e, y);
+}
+
+// Offset of slot s in the ragged row storage.
+template <int G>
+__host__ __device__ constexpr int slot_off(int s) {
+    return G * s * (s + 1) / 2;
+}
+
+// ----------------------------------------------------------- factor/solve
+template <int NMAX, int G, typename T>
+struct Chol {
+    static constexpr int S = NMAX / G;
+    static constexpr int TOT = slot_off<G>(S);
+
+    // In-place Cholesky; D[s] = 1 / L[r_s][r_s].  Group-uniform result.
+    static __device__ __forceinline__ bool factor(T (&L)[TOT], T (&D)[S], T& minpiv, int t) {
+        bool ok = true;
+        minpiv = T(3.0e38);
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            const int q = j % G, sj = j / G;
+            const T piv = gbc<G>(L[slot_off<G>(sj) + j], q);
+            ok = ok && (piv > T(0));
+            minpiv = piv < minpiv ? piv : minpiv;
+            const T rs = rsqrt_t(piv);
+            const T d = piv * rs;
+            if (t == q) D[sj] = rs;
+#pragma unroll
+            for (int s = sj; s < S; ++s) {
+                T& e = L[slot_off<G>(s) + j];
+                if (s == sj)
+                    e = (t == q) ? d : e * rs;
+                else
+                    e = e * rs;
+            }
+#pragma unroll
+            for (int c = j + 1; c < NMAX; ++c) {
+                const T lc = gbc<G>(L[slot_off<G>(c / G) + j], c % G);
+#pragma unroll
+                for (int s = c / G; s < S; ++s) L[slot_off<G>(s) + c] -= L[slot_off<G>(s) + j] * lc;
+            }
+        }
+        return ok;
+    }
+
+    // z <- L^-1 z (column-oriented, one broadcast per step)
+    template <typename V>
+    static __device__ __forceinline__ void forward(const T (&L)[TOT], const T (&D)[S], V (&z)[S], int t) {
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            const int q = j % G, sj = j / G;
+            const V yj = gbc<G>(z[sj] * (V)D[sj], q);
+#pragma unroll
+            for (int s = sj; s < S; ++s) {
+                const V upd = z[s] - (V)L[slot_off<G>(s) + j] * yj;
+                if (s == sj)
+                    z[s] = (t == q) ? yj : ((t > q) ? upd : z[s]);
+                else
+                    z[s] = upd;
+            }
+        }
+    }
+
+    // x <- L^-T x (partial dot products, group sum, owner divides)
+    template <typename V>
+    static __device__ __forceinline__ void backward(const T (&L)[TOT], const T (&D)[S], V (&x)[S], int t) {
+#pragma unroll
+        for (int i = NMAX - 1; i >= 0; --i) {
+            const int q = i % G, si = i / G;
+            V p = V(0);
+#pragma unroll
+            for (int s = si; s < S; ++s) {
+                const V prod = (V)L[slot_off<G>(s) + i] * x[s];
+                if (s == si)
+                    p += (t > q) ? prod : V(0);
+                else
+                    p += prod;
+            }
+            p = gsum<G>(p);
+            if (t == q) x[si] = (x[si] - p) * (V)D[si];
+        }
+    }
+};
+
+template <int G>
+__device__ __forceinline__ void write_informed(const SolveArgs& a, long long cell, int t) {
+    if (t != 0) return;
+    const float v = a.values[cell];
+    if (a.out_val) a.out_val[cell] = v;
+    if (a.out_mask) a.out_mask[cell] = 1;
+    if (a.out_var) a.out_var[cell] = 0.0;
+    if (a.out_est64) a.out_est64[cell] = v;
+    if (a.out_status) a.out_status[cell] = ST_INFORMED;
+}
+
+// Kriges one cell with the group of G lanes; t = lane index in the group;
+// xsh = this group's LDS scratch (2 * NMAX doubles) for the mixed form.
+template <int NMAX, int G, bool MIXED>
+__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int t, double* xsh) {
+    using T = typename std::conditional<MIXED, float, double>::type;
+    typedef Chol<NMAX, G, T> CH;
+    constexpr int S = CH::S;
+    const GridDims& g = a.g;
+    const int cnt = a.count[cell];
+    const int kind = a.kind;
+    if (cnt == CNT_INFORMED) {
+        write_informed<G>(a, cell, t);
+        return;
+    }
+    const int n = cnt < 0 ? 0 : cnt;
+    const double mean = (kind == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
+    const float mf = (float)mean;
+    const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
+    const long long center = ((long long)a.wz * SY + a.wy) * SX + a.wx;
+    const int* nb = a.nboff + cell * a.kmax;
+
+    int status = ST_KRIGED;
+    double w[S], bq[S];
+    int dnode[S];
+    double mu = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        w[s] = 0.0;
+        bq[s] = 0.0;
+        dnode[s] = 0;
+    }
+    if (n == 0) {
+        status = ST_NO_NEIGHBOURS;
+    } else {
+        // Table-linear offsets of my rows; column offsets come by broadcast.
+        int tr[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int r = G * s + t;
+            int dx = 0, dy = 0, dz = 0;
+            if (r < n) unpack_off(nb[r], dx, dy, dz);
+            tr[s] = (dz * SY + dy) * SX + dx;
+            dnode[s] = (dz * g.ny + dy) * g.nx + dx;
+        }
+        T L[CH::TOT];
+        T D[S];
+        const T* tab = MIXED ? (const T*)(const void*)a.ctab32 : (const T*)(const void*)a.ctab;
+#pragma unroll
+        for (int c = 0; c < NMAX; ++c) {
+            const int tcol = gbc<G>(tr[c / G], c % G);
+#pragma unroll
+            for (int s = c / G; s < S; ++s) {
+                const int r = G * s + t;
+                T v = (r == c) ? T(1) : T(0);
+                if (r < n && c < n) v = tab[center + tr[s] - tcol];
+                L[slot_off<G>(s) + c] = v;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) bq[s] = (G * s + t < n) ? a.ctab[center + tr[s]] : 0.0;
+        const bool ok_kind = kind == 0;
+        T minpiv;
+        const bool ok = CH::factor(L, D, minpiv, t);
+        if (MIXED && (!ok || !((double)minpiv >= a.pivot_tol * a.c0))) {
+            // Queue the cell for the FP64 form (the whole group leaves together).
+            if (t == 0) {
+                const int pos = atomicAdd(a.n_fallback, 1);
+                a.fallback_cells[pos] = (int)cell;
+            }
+            return;
+        }
+        if (!ok) {
+            status = ST_SINGULAR;
+        } else {
+            double sv[S], ov[S];
+            T zs[S], zo[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                zs[s] = (T)bq[s];
+                zo[s] = (G * s + t < n) ? T(1) : T(0);
+            }
+            CH::forward(L, D, zs, t);
+            CH::backward(L, D, zs, t);
+            if (ok_kind) {
+                CH::forward(L, D, zo, t);
+                CH::backward(L, D, zo, t);
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                sv[s] = (double)zs[s];
+                ov[s] = ok_kind ? (double)zo[s] : 0.0;
+            }
+            if (MIXED) {
+                for (int it = 0; it < a.refine_steps; ++it) {
+                    // residuals b - A s and 1 - A o in FP64; the current
+                    // solutions are shared through LDS, A comes from the table.
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int r = G * s + t;
+                        if (r < n) {
+                            xsh[r] = sv[s];
+                            xsh[NMAX + r] = ov[s];
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    double rs_[S], ro_[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        rs_[s] = bq[s];
+                        ro_[s] = (G * s + t < n) ? 1.0 : 0.0;
+                    }
+                    for (int c = 0; c < n; ++c) {
+                        int dx, dy, dz;
+                        unpack_off(nb[c], dx, dy, dz);
+                        const int tcol = (dz * SY + dy) * SX + dx;
+                        const double xs = xsh[c];
+                        const double xo = xsh[NMAX + c];
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+                            if (G * s + t < n) {
+                                const double av = a.ctab[center + tr[s] - tcol];
+                                rs_[s] -= av * xs;
+                                ro_[s] -= av * xo;
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    T cs[S], co[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        cs[s] = (T)rs_[s];
+                        co[s] = (T)ro_[s];
+                    }
+                    CH::forward(L, D, cs, t);
+                    CH::backward(L, D, cs, t);
+                    if (ok_kind) {
+                        CH::forward(L, D, co, t);
+                        CH::backward(L, D, co, t);
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        sv[s] += (double)cs[s];
+                        ov[s] += ok_kind ? (double)co[s] : 0.0;
+                    }
+                }
+            }
+            if (ok_kind) {
+                double ps = 0.0, po = 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    ps += sv[s];
+                    po += ov[s];
+                }
+                const double sum_sk = gsum<G>(ps), sum_ones = gsum<G>(po);
+                if (fabs(sum_ones) < 1e-15) {
+                    status = ST_SINGULAR;
+                } else {
+                    mu = (sum_sk - 1) / sum_ones;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) w[s] = sv[s] - mu * ov[s];
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) w[s] = sv[s];
+            }
+        }
+    }
+    if (status != ST_KRIGED) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) w[s] = 0.0;
+        mu = 0.0;
+    }
+
+    double pwb = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) pwb += w[s] * bq[s];
+    const double wb = gsum<G>(pwb);
+
+    if (kind != 2) {
+        float term[S];
+        double pe = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const float v = (G * s + t < n) ? a.values[cell + dnode[s]] : 0.0f;
+            term[s] = (float)(w[s] * (double)(v - mf));
+            pe += w[s] * ((double)v - (double)mf);
+        }
+        float acc = mf;
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) {
+            const float ti = gbc<G>(term[i / G], i % G);
+            if (i < n) acc += ti;
+        }
+        const double e64 = (double)mf + gsum<G>(pe);
+        const double var = a.c0 - wb - mu;
+        if (t == 0) {
+            if (status == ST_KRIGED) {
+                if (a.out_val) a.out_val[cell] = acc;
+                if (a.out_mask) a.out_mask[cell] = 1;
+                if (a.out_var) a.out_var[cell] = var < 0 ? 0.0 : var;
+                if (a.out_est64) a.out_est64[cell] = e64;
+            } else if (kind == 1) {
+                // mean_on_failure (cont_kriging.h:80-94)
+                if (a.out_val) a.out_val[cell] = mf;
+                if (a.out_mask) a.out_mask[cell] = 1;
+                if (a.out_var) a.out_var[cell] = -1.0;
+                if (a.out_est64) a.out_est64[cell] = (double)mf;
+            } else {
+                // undefined_on_failure: the output keeps the input clone
+                if (a.out_val) a.out_val[cell] = a.values[cell];
+                if (a.out_mask) a.out_mask[cell] = a.mask[cell];
+                if (a.out_var) a.out_var[cell] = -1.0;
+                if (a.out_est64) a.out_est64[cell] = __builtin_nan("");
+            }
+            if (a.out_status) a.out_status[cell] = (unsigned char)status;
+        }
+        return;
+    }
+
+    // Median indicator kriging: one weight set, one combine per threshold.
+    constexpr int KMAXIND = 16;
+    float probs[KMAXIND];
+    const double var = a.c0 - wb;
+#pragma unroll
+    for (int k = 0; k < KMAXIND; ++k) {
+        probs[k] = 0.f;
+        if (k < a.nind) {
+            const float pf = (float)a.marg[k];
+            double term[S];
+            double pe = 0.0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                float iv = 0.f;
+                if (G * s + t < n) {
+                    if (a.zfield)
+                        iv = (a.zfield[cell + dnode[s]] <= a.thresholds[k]) ? 1.f : 0.f;
+                    else
+                        iv = (a.ind[(long long)k * g.n + cell + dnode[s]] == 1) ? 1.f : 0.f;
+                }
+                term[s] = w[s] * (double)(iv - pf);
+                pe += w[s] * ((double)iv - (double)pf);
+            }
+            double acc = (double)pf;
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                const double ti = gbc<G>(term[i / G], i % G);
+                if (i < n) acc += ti;
+            }
+            const double e64 = (double)pf + gsum<G>(pe);
+            probs[k] = (float)acc;
+            if (t == 0) {
+                if (a.out_est64) a.out_est64[(long long)k * g.n + cell] = e64;
+                if (a.out_class) a.out_class[(long long)k * g.n + cell] = ((float)acc <= 0.5f) ? 0 : 1;
+            }
+        }
+    }
+    if (t == 0) {
+        if (a.order_relations) {
+            // indicator_kriging.h:34-59
+#pragma unroll
+            for (int k = 0; k < KMAXIND; ++k)
+                if (k < a.nind) probs[k] = probs[k] < 0.0f ? 0.0f : (probs[k] > 1.0f ? 1.0f : probs[k]);
+#pragma unroll
+            for (int k = 0; k + 1 < KMAXIND; ++k)
+                if (k + 1 < a.nind && probs[k] > probs[k + 1]) {
+                    const double avg = ((double)probs[k] + (double)probs[k + 1]) / 2.0;
+                    probs[k] = (float)avg;
+                    probs[k + 1] = (float)avg;
+                }
+        }
+        if (a.out_prob) {
+#pragma unroll
+            for (int k = 0; k < KMAXIND; ++k)
+                if (k < a.nind) a.out_prob[(long long)k * g.n + cell] = probs[k];
+        }
+        if (a.out_mask) a.out_mask[cell] = 1;
+        if (a.out_var) a.out_var[cell] = status == ST_KRIGED ? (var < 0 ? 0.0 : var) : -1.0;
+        if (a.out_status) a.out_status[cell] = (unsigned char)status;
+    }
+}
+
+// One group per cell; with a cell list, groups stride over the list.
+template <int NMAX, int G, bool MIXED>
+__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
+    __shared__ double xsh_all[MIXED ? (256 / G) * 2 * NMAX : 1];
+    double* xsh = xsh_all + (MIXED ? (threadIdx.x / G) * 2 * NMAX : 0);
+    const int t = threadIdx.x & (G - 1);
+    const long long gid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const long long ngroups = ((long long)gridDim.x * blockDim.x) / G;
+    const long long total = a.cell_list ? (long long)*a.n_list : a.g.n;
+    for (long long item = gid; item < total; item += ngroups) {
+        const long long cell = a.cell_list ? (long long)a.cell_list[item] : item;
+        solve_cell<NMAX, G, MIXED>(a, cell, t, xsh);
+    }
+}
+
+}  // namespace hpglx

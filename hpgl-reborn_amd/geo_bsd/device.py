@@ -60,7 +60,7 @@ def _dims(grid):
 
 def krige(kind, grid, radiuses, max_neighbours, cov_model, mask, values=None, indicators=None, zfield=None,
           thresholds=None, marginal=None, mean=None, precision="fp64", variance=True, est64=False, status=False,
-          neighbours=False, order_relations=False, pivot_tol=1e-3, refine_steps=2, stream=None):
+          neighbours=False, order_relations=False, pivot_tol=1e-3, refine_steps=-1, stream=None):
     """Grid kriging on device tensors; returns a dict of result tensors.
 
     kind: KIND_OK, KIND_SK or KIND_MIK.  For MIK give ``indicators`` (uint8,

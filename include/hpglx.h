@@ -57,7 +57,7 @@ typedef struct hpglx_grid_request_t {
     int* out_count;            /* neighbour count per cell (-1 on informed OK/SK cells) */
     int* out_nboff;            /* N x max_neighbours packed offsets, see hpglx_unpack_offset */
     double pivot_tol;          /* MIXED: re-solve in FP64 when min pivot < tol * C(0); <= 0 -> 1e-3 */
-    int refine_steps;          /* MIXED: FP64 residual refinement steps; < 0 -> 2 */
+    int refine_steps;          /* MIXED: FP64 residual refinement steps; < 0 -> 1 */
 } hpglx_grid_request_t;
 
 /* The work of hpgl_ordinary_kriging / hpgl_simple_kriging / hpgl_median_ik
