@@ -123,23 +123,119 @@ __global__ void k_cluster_flags(GridDims g, const int* cl_count, long long limit
     nbhd_exceeded[ci] = ex;
 }
 
-__global__ void k_tdelta(GridDims g, int tsize, const int* tdx, const int* tdy, const int* tdz, int* tdelta,
-                         int* tpack) {
+// Template offsets as deltas in the padded mask and in a tile's LDS box;
+// entries from tsize up to the padded length are zero (never counted).
+__global__ void k_tdelta(GridDims g, int tsize, int tpad, const int* tdx, const int* tdy, const int* tdz,
+                         int* tdelta, int* tdelta_lds, int lx, int ly, int* tpack) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tsize) return;
+    if (t >= tpad) return;
+    if (t >= tsize) {
+        tdelta[t] = 0;
+        tdelta_lds[t] = 0;
+        tpack[t] = 0;
+        return;
+    }
     tdelta[t] = (tdz[t] * g.py + tdy[t]) * g.px + tdx[t];
+    tdelta_lds[t] = (tdz[t] * ly + tdy[t]) * lx + tdx[t];
     tpack[t] = pack_off(tdx[t], tdy[t], tdz[t]);
 }
 
+// --------------------------------------------------------- tiled template walk
+// One block per TILE_X x TILE_Y x 1 tile of cells.  The block first copies the
+// padded-mask box its cells can reach (tile + 2r halo) into LDS; the template
+// then streams through LDS in chunks, so every probe is one broadcast LDS read
+// of the template entry plus one LDS byte read of the mask.  Only the rare
+// hits write to global memory.  Cells whose 27 clusters are all below the
+// limit are handed to the cluster path instead.
+constexpr int TILE_X = 16, TILE_Y = 16;
+constexpr int TMPL_U = 32;  // probes per iteration; template arrays are padded to a multiple
+
+__global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int chunk) {
+    extern __shared__ unsigned int lds_dw[];
+    const int qx = a.lx >> 2;
+    const int ndw = qx * a.ly * a.lz;
+    unsigned char* box = (unsigned char*)lds_dw;
+    int* tdel = (int*)(lds_dw + ((ndw + 3) & ~3));
+    int* tpk = tdel + chunk;
+    const GridDims& g = a.g;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = blockIdx.z;
+    // Box origin is padded (x0, y0, z0) = cell (x0 - rx, y0 - ry, z0 - rz).
+    for (int i = tid; i < ndw; i += blockDim.x) {
+        const int xq = i % qx, rest = i / qx;
+        const int yy = rest % a.ly, zz = rest / a.ly;
+        const int px = x0 + 4 * xq, py = y0 + yy, pz = z0 + zz;
+        unsigned int v = 0;
+        if (px < g.px && py < g.py && pz < g.pz)
+            v = *(const unsigned int*)(a.pmask + ((long long)pz * g.py + py) * g.px + px);
+        lds_dw[i] = v;
+    }
+    const int tx = tid % TILE_X, ty = tid / TILE_X;
+    const int x = x0 + tx, y = y0 + ty, z = z0;
+    const bool inside = x < g.nx && y < g.ny;
+    const long long cell = inside ? ((long long)z * g.ny + y) * g.nx + x : 0;
+    bool active = false;
+    if (inside) {
+        const bool needs = a.kind == 2 || !a.mask[cell];
+        if (!needs) {
+            a.count[cell] = CNT_INFORMED;
+        } else if (a.cl_nbhd_exceeded[cluster_of(g, x, y, z)]) {
+            active = true;
+        } else {
+            a.count[cell] = CNT_PENDING_CLUSTER;
+            const int pos = atomicAdd(a.n_cl_cells, 1);
+            a.cl_cells[pos] = (int)cell;
+        }
+    }
+    // Inactive lanes probe around the box position of tile cell (0, 0), which
+    // keeps every template delta inside the box; their bits are discarded.
+    const int lbase = (g.rz * a.ly + (active ? ty : 0) + g.ry) * a.lx + (active ? tx : 0) + g.rx;
+    int cnt = 0;
+    const int K = a.kmax;
+    int* out = a.nboff + (active ? cell * K : 0);
+    for (int c0 = 0; c0 < a.tsize; c0 += chunk) {
+        if (!__syncthreads_or(active && cnt < K)) break;
+        const int clen = min(chunk, a.tsize - c0);
+        const int cpad = (clen + TMPL_U - 1) / TMPL_U * TMPL_U;
+        for (int i = tid; i < cpad; i += blockDim.x) {
+            tdel[i] = a.tdelta_lds[c0 + i];
+            tpk[i] = a.tpack[c0 + i];
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < clen; t0 += TMPL_U) {
+            const bool want = active && cnt < K;
+            if (!__any(want)) break;
+            // 32 probes into a bit mask (mask bytes are 0 or 1), no branches.
+            unsigned bits = 0;
+#pragma unroll
+            for (int u = 0; u < TMPL_U; ++u) {
+                const unsigned m = box[lbase + tdel[t0 + u]];
+                bits |= m << u;
+            }
+            const int left = clen - t0;
+            if (left < TMPL_U) bits &= (1u << left) - 1u;
+            if (!want) bits = 0;
+            // Hits in template order until max_neighbours.
+            while (bits != 0 && cnt < K) {
+                const int u = __builtin_ctz(bits);
+                out[cnt++] = tpk[t0 + u];
+                bits &= bits - 1u;
+            }
+        }
+    }
+    if (active) a.count[cell] = cnt;
+}
+
 // ------------------------------------------------------------ template walk
-// One lane per cell.  All lanes of a wave walk the template in lock step, so
-// each template delta is a wave-uniform (scalar) load and the 64 mask probes
-// of one step hit neighbouring bytes of the padded mask.
+// Global-memory form for boxes too large for LDS: one lane per cell, the
+// template read with uniform vector loads, the mask from the padded array.
 __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const GridDims& g = a.g;
     bool active = false;
-    long long pbase = 0;
+    // Inactive lanes probe around grid cell (0, 0, 0): every template delta
+    // stays inside the padded mask; their bits are discarded.
+    long long pbase = ((long long)g.rz * g.py + g.ry) * g.px + g.rx;
     if (cell < g.n) {
         const bool needs = a.kind == 2 || !a.mask[cell];
         if (!needs) {
@@ -160,26 +256,24 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
     int cnt = 0;
     const int K = a.kmax;
     int* out = a.nboff + (active ? cell * K : 0);
-    constexpr int U = 8;
-    for (int t0 = 0; t0 < a.tsize; t0 += U) {
+    for (int t0 = 0; t0 < a.tsize; t0 += TMPL_U) {
         const bool want = active && cnt < K;
         if (!__any(want)) break;
-        unsigned char m[U];
+        // The template arrays are padded to a multiple of TMPL_U, so these
+        // loads are unconditional and issue together.
+        unsigned bits = 0;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int t = t0 + u;
-            m[u] = 0;
-            if (t < a.tsize) {
-                const int d = a.tdelta[t];
-                if (want) m[u] = a.pmask[pbase + d];
-            }
+        for (int u = 0; u < TMPL_U; ++u) {
+            const unsigned m = a.pmask[pbase + a.tdelta[t0 + u]];
+            bits |= m << u;
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (m[u] && cnt < K) {
-                out[cnt] = a.tpack[t0 + u];
-                ++cnt;
-            }
+        const int left = a.tsize - t0;
+        if (left < TMPL_U) bits &= (1u << left) - 1u;
+        if (!want) bits = 0;
+        while (bits != 0 && cnt < K) {
+            const int u = __builtin_ctz(bits);
+            out[cnt++] = a.tpack[t0 + u];
+            bits &= bits - 1u;
         }
     }
     if (active) a.count[cell] = cnt;
@@ -331,16 +425,19 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
 
 // -------------------------------------------------------------------- solve
 // (solve.h)
-#define HPGLX_SOLVE(NMAX, G)                                     \
-    template __global__ void k_solve<NMAX, G, false>(SolveArgs); \
-    template __global__ void k_solve<NMAX, G, true>(SolveArgs);
-HPGLX_SOLVE(8, 4)
-HPGLX_SOLVE(16, 4)
-HPGLX_SOLVE(24, 4)
-HPGLX_SOLVE(24, 8)
-HPGLX_SOLVE(32, 4)
-HPGLX_SOLVE(32, 8)
-#undef HPGLX_SOLVE
+#define HPGLX_SOLVE_K(NMAX, G, MIXED)                                \
+    template __global__ void k_solve<NMAX, G, MIXED, 0>(SolveArgs); \
+    template __global__ void k_solve<NMAX, G, MIXED, 1>(SolveArgs); \
+    template __global__ void k_solve<NMAX, G, MIXED, 2>(SolveArgs);
+HPGLX_SOLVE_K(8, 4, false)
+HPGLX_SOLVE_K(8, 4, true)
+HPGLX_SOLVE_K(16, 4, false)
+HPGLX_SOLVE_K(16, 4, true)
+HPGLX_SOLVE_K(24, 4, false)
+HPGLX_SOLVE_K(24, 4, true)
+HPGLX_SOLVE_K(32, 8, false)
+HPGLX_SOLVE_K(32, 4, true)
+#undef HPGLX_SOLVE_K
 
 // ------------------------------------------------------------------ launcher
 #define HPGLX_CHECK(expr)                                              \
@@ -415,40 +512,47 @@ void Workspace::release() {
 
 namespace {
 
-template <int NMAX, int G>
-void launch_solve(const SolveArgs& sa, bool mixed, long long ncells, hipStream_t st) {
+template <int NMAX, int G, bool MIXED, int KIND>
+void launch_solve(const SolveArgs& sa, long long ncells, hipStream_t st) {
     const int block = 256;
     const unsigned grid = (unsigned)((ncells * G + block - 1) / block);
-    if (mixed)
-        hipLaunchKernelGGL((k_solve<NMAX, G, true>), dim3(grid), dim3(block), 0, st, sa);
-    else
-        hipLaunchKernelGGL((k_solve<NMAX, G, false>), dim3(grid), dim3(block), 0, st, sa);
+    hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3(grid), dim3(block), 0, st, sa);
 }
 
 // FP64 form over a device-sized cell list (fallback of the mixed form).
-template <int NMAX, int G>
+template <int NMAX, int G, int KIND>
 void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st) {
     const int block = 256;
     long long blocks = (max_cells * G + block - 1) / block;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((k_solve<NMAX, G, false>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+    hipLaunchKernelGGL((k_solve<NMAX, G, false, KIND>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
 }
 
-// Group width per (NMAX, precision): G lanes share one cell.
-template <int NMAX, int G64, int G32>
-void dispatch_solve(const SolveArgs& sa, bool mixed, long long n, hipStream_t st) {
+// G64 / G32: lanes per cell of the FP64 and of the mixed form.
+template <int NMAX, int G64, int G32, int KIND>
+void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, hipStream_t st) {
     if (mixed) {
-        launch_solve<NMAX, G32>(sa, true, n, st);
+        launch_solve<NMAX, G32, true, KIND>(sa, n, st);
         SolveArgs fb = sa;
         fb.cell_list = sa.fallback_cells;
         fb.n_list = sa.n_fallback;
         fb.fallback_cells = nullptr;
         fb.n_fallback = nullptr;
-        launch_solve_list<NMAX, G64>(fb, n, st);
+        launch_solve_list<NMAX, G64, KIND>(fb, n, st);
     } else {
-        launch_solve<NMAX, G64>(sa, false, n, st);
+        launch_solve<NMAX, G64, false, KIND>(sa, n, st);
     }
+}
+
+template <int NMAX, int G64, int G32>
+void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, hipStream_t st) {
+    if (kind == 0)
+        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, st);
+    else if (kind == 1)
+        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, st);
+    else
+        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, st);
 }
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -484,7 +588,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     g.rx = h.r[0];
     g.ry = h.r[1];
     g.rz = h.r[2];
-    g.px = c.nx + 2 * g.rx;
+    g.px = (c.nx + 2 * g.rx + 15) / 16 * 16;  // row pitch: 16-byte aligned rows
     g.py = c.ny + 2 * g.ry;
     g.pz = c.nz + 2 * g.rz;
     // clusterizer.cpp:87-89: g / r + 2, in double, truncated.
@@ -501,8 +605,17 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     auto* cl_start = (int*)ws.get("cl_start", (size_t)ncl * 4, err);
     auto* cl_cursor = (int*)ws.get("cl_cursor", (size_t)ncl * 4, err);
     auto* nbhd = (unsigned char*)ws.get("cl_nbhd", (size_t)ncl, err);
-    auto* tdelta = (int*)ws.get("tdelta", (size_t)(T > 0 ? T : 1) * 4, err);
-    auto* tpack = (int*)ws.get("tpack", (size_t)(T > 0 ? T : 1) * 4, err);
+    const int tpad = (T + TMPL_U - 1) / TMPL_U * TMPL_U + TMPL_U;
+    auto* tdelta = (int*)ws.get("tdelta", (size_t)tpad * 4, err);
+    auto* tdelta_lds = (int*)ws.get("tdelta_lds", (size_t)tpad * 4, err);
+    const int lx = (TILE_X + 2 * g.rx + 3) / 4 * 4, ly = TILE_Y + 2 * g.ry, lz = 1 + 2 * g.rz;
+    const size_t box_bytes = ((size_t)lx * ly * lz + 15) / 16 * 16;
+    // LDS of a tile block: the mask box, then a chunk of the template (delta + pack).
+    const size_t lds_budget = 65536;
+    int chunk = box_bytes < lds_budget ? (int)((lds_budget - box_bytes) / 8) : 0;
+    chunk = chunk > 4096 ? 4096 : chunk / TMPL_U * TMPL_U;
+    const bool tiled = chunk >= 256 && c.search_variant != 1;
+    auto* tpack = (int*)ws.get("tpack", (size_t)tpad * 4, err);
     auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
     auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
     auto* cl_cells = (int*)ws.get("cl_cells", (size_t)n * 4, err);
@@ -540,9 +653,8 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n, B)), dim3(B), 0, st, g, c.mask, pmask, cl_count);
     hipLaunchKernelGGL(k_cluster_flags, dim3(blocks_for(ncl, B)), dim3(B), 0, st, g, (const int*)cl_count,
                        (long long)h.limit, nbhd);
-    if (T > 0)
-        hipLaunchKernelGGL(k_tdelta, dim3(blocks_for(T, B)), dim3(B), 0, st, g, T, (const int*)plan.tdx,
-                           (const int*)plan.tdy, (const int*)plan.tdz, tdelta, tpack);
+    hipLaunchKernelGGL(k_tdelta, dim3(blocks_for(tpad, B)), dim3(B), 0, st, g, T, tpad, (const int*)plan.tdx,
+                       (const int*)plan.tdy, (const int*)plan.tdz, tdelta, tdelta_lds, lx, ly, tpack);
     if (c.kind == KIND_SK && c.auto_mean) {
         hipLaunchKernelGGL(k_mean_partial, dim3(mean_blocks), dim3(256), 0, st, n, c.values, c.mask, psum, pcnt);
         hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, st, mean_blocks, (const double*)psum,
@@ -561,6 +673,10 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     sa.mask = c.mask;
     sa.pmask = pmask;
     sa.tdelta = tdelta;
+    sa.tdelta_lds = tdelta_lds;
+    sa.lx = lx;
+    sa.ly = ly;
+    sa.lz = lz;
     sa.tpack = tpack;
     sa.tsize = T;
     sa.cl_nbhd_exceeded = nbhd;
@@ -568,7 +684,13 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     sa.nboff = nboff;
     sa.cl_cells = cl_cells;
     sa.n_cl_cells = n_cl_cells;
-    hipLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, sa);
+    if (tiled) {
+        const dim3 grid((c.nx + TILE_X - 1) / TILE_X, (c.ny + TILE_Y - 1) / TILE_Y, c.nz);
+        hipLaunchKernelGGL(k_search_tile, grid, dim3(TILE_X * TILE_Y), box_bytes + (size_t)chunk * 8, st, sa,
+                           chunk);
+    } else {
+        hipLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, sa);
+    }
 
     hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(1024), 0, st, ncl, (const int*)cl_count, (long long)h.limit,
                        cl_start, cl_total, (const int*)n_cl_cells);
@@ -640,26 +762,14 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[2], st));
-    const int gsel = c.solver_g;
     if (K <= 8)
-        dispatch_solve<8, 4, 4>(so, mixed, n, st);
+        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, st);
     else if (K <= 16)
-        dispatch_solve<16, 4, 4>(so, mixed, n, st);
-    else if (K <= 24) {
-        if (gsel == 8)
-            dispatch_solve<24, 8, 8>(so, mixed, n, st);
-        else if (gsel == 4)
-            dispatch_solve<24, 4, 4>(so, mixed, n, st);
-        else
-            dispatch_solve<24, 8, 4>(so, mixed, n, st);
-    } else {
-        if (gsel == 8)
-            dispatch_solve<32, 8, 8>(so, mixed, n, st);
-        else if (gsel == 4)
-            dispatch_solve<32, 4, 4>(so, mixed, n, st);
-        else
-            dispatch_solve<32, 8, 4>(so, mixed, n, st);
-    }
+        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, st);
+    else if (K <= 24)
+        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, st);
+    else
+        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, st);
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[3], st));
     HPGLX_CHECK(hipGetLastError());
     return 0;

@@ -39,8 +39,10 @@ struct SearchArgs {
     const unsigned char* mask;     // input mask (N)
     const unsigned char* pmask;    // padded 0/1 mask
     const int* tdelta;             // template: padded-linear delta
+    const int* tdelta_lds;         // template: delta in the tile's LDS box
     const int* tpack;              // template: packed offset
     int tsize;
+    int lx, ly, lz;                // LDS box of a tile (tile + 2r halo; lx multiple of 4)
     const unsigned char* cl_nbhd_exceeded;  // per cluster: any of its 27 neighbours exceeded
     int* count;                    // per cell neighbour count / marker
     int* nboff;                    // per cell kmax packed offsets

@@ -58,6 +58,7 @@ struct GridCall {
     double pivot_tol = 1e-3;
     int refine_steps = 1;
     int solver_g = 0;                       // lanes per cell (4 or 8); 0 = default
+    int search_variant = 0;                 // 0 = LDS tiles when the box fits, 1 = global walk
     bool profile = false;
 };
 

@@ -220,27 +220,181 @@ __device__ __forceinline__ void write_informed(const SolveArgs& a, long long cel
     if (a.out_status) a.out_status[cell] = ST_INFORMED;
 }
 
-// Kriges one cell with the group of G lanes; t = lane index in the group;
-// xsh = this group's LDS scratch (2 * NMAX doubles) for the mixed form.
-template <int NMAX, int G, bool MIXED>
-__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int t, double* xsh) {
+// Weights of one cell: w[s] for the lane's rows, mu (OK), status.  Returns
+// false when the mixed form hands the cell to the FP64 form.
+template <int NMAX, int G, bool MIXED, int KIND>
+__device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, double* xsh,
+                                             double (&w)[NMAX / G], double (&bq)[NMAX / G], int (&dnode)[NMAX / G],
+                                             double& mu, int& status) {
     using T = typename std::conditional<MIXED, float, double>::type;
     typedef Chol<NMAX, G, T> CH;
     constexpr int S = CH::S;
+    constexpr bool OKK = KIND == 0;
+    const GridDims& g = a.g;
+    const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
+    const int center = (a.wz * SY + a.wy) * SX + a.wx;
+    const int* nb = a.nboff + cell * a.kmax;
+
+    // Table-linear offsets of my rows; column offsets come by broadcast.
+    int tr[S];
+    int pk[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int r = G * s + t;
+        pk[s] = nb[r < n ? r : 0];  // unconditional, so the loads issue together
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int r = G * s + t;
+        int dx, dy, dz;
+        unpack_off(pk[s], dx, dy, dz);
+        if (r >= n) dx = dy = dz = 0;
+        tr[s] = (dz * SY + dy) * SX + dx;
+        dnode[s] = (dz * g.ny + dy) * g.nx + dx;
+    }
+    T L[CH::TOT];
+    T D[S];
+    const T* tab = MIXED ? (const T*)(const void*)a.ctab32 : (const T*)(const void*)a.ctab;
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) {
+        const int tcol = gbc<G>(tr[c / G], c % G);
+#pragma unroll
+        for (int s = c / G; s < S; ++s) {
+            const int r = G * s + t;
+            const bool live = r < n && c < n;
+            const T v = tab[live ? center + tr[s] - tcol : center];
+            L[slot_off<G>(s) + c] = live ? v : ((r == c) ? T(1) : T(0));
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double v = a.ctab[center + tr[s]];
+        bq[s] = (G * s + t < n) ? v : 0.0;
+    }
+    T minpiv;
+    const bool ok = CH::factor(L, D, minpiv, t);
+    if (MIXED && (!ok || !((double)minpiv >= a.pivot_tol * a.c0))) {
+        if (t == 0) {
+            const int pos = atomicAdd(a.n_fallback, 1);
+            a.fallback_cells[pos] = (int)cell;
+        }
+        return false;
+    }
+    if (!ok) {
+        status = ST_SINGULAR;
+        return true;
+    }
+    double sv[S], ov[S];
+    {
+        T z[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) z[s] = (T)bq[s];
+        CH::forward(L, D, z, t);
+        CH::backward(L, D, z, t);
+#pragma unroll
+        for (int s = 0; s < S; ++s) sv[s] = (double)z[s];
+    }
+    if (OKK) {
+        T z[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) z[s] = (G * s + t < n) ? T(1) : T(0);
+        CH::forward(L, D, z, t);
+        CH::backward(L, D, z, t);
+#pragma unroll
+        for (int s = 0; s < S; ++s) ov[s] = (double)z[s];
+    }
+    if (MIXED) {
+        for (int it = 0; it < a.refine_steps; ++it) {
+            // Residuals b - A s (and 1 - A o) in FP64: the current solutions
+            // are shared through LDS, A comes from the FP64 table.
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int r = G * s + t;
+                if (r < n) {
+                    xsh[r] = sv[s];
+                    if (OKK) xsh[NMAX + r] = ov[s];
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double rs_[S], ro_[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                rs_[s] = bq[s];
+                ro_[s] = (G * s + t < n) ? 1.0 : 0.0;
+            }
+            for (int c = 0; c < n; ++c) {
+                int dx, dy, dz;
+                unpack_off(nb[c], dx, dy, dz);
+                const int tcol = (dz * SY + dy) * SX + dx;
+                const double xs = xsh[c];
+                const double xo = OKK ? xsh[NMAX + c] : 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const bool live = G * s + t < n;
+                    const double av = a.ctab[live ? center + tr[s] - tcol : center];
+                    if (live) {
+                        rs_[s] -= av * xs;
+                        if (OKK) ro_[s] -= av * xo;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            {
+                T z[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) z[s] = (T)rs_[s];
+                CH::forward(L, D, z, t);
+                CH::backward(L, D, z, t);
+#pragma unroll
+                for (int s = 0; s < S; ++s) sv[s] += (double)z[s];
+            }
+            if (OKK) {
+                T z[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) z[s] = (T)ro_[s];
+                CH::forward(L, D, z, t);
+                CH::backward(L, D, z, t);
+#pragma unroll
+                for (int s = 0; s < S; ++s) ov[s] += (double)z[s];
+            }
+        }
+    }
+    if (OKK) {
+        double ps = 0.0, po = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            ps += sv[s];
+            po += ov[s];
+        }
+        const double sum_sk = gsum<G>(ps), sum_ones = gsum<G>(po);
+        if (fabs(sum_ones) < 1e-15) {
+            status = ST_SINGULAR;
+            return true;
+        }
+        mu = (sum_sk - 1) / sum_ones;
+#pragma unroll
+        for (int s = 0; s < S; ++s) w[s] = sv[s] - mu * ov[s];
+    } else {
+#pragma unroll
+        for (int s = 0; s < S; ++s) w[s] = sv[s];
+    }
+    return true;
+}
+
+// Kriges one cell with the group of G lanes; t = lane index in the group;
+// xsh = this group's LDS scratch (2 * NMAX doubles) for the mixed form.
+template <int NMAX, int G, bool MIXED, int KIND>
+__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int t, double* xsh) {
+    constexpr int S = NMAX / G;
     const GridDims& g = a.g;
     const int cnt = a.count[cell];
-    const int kind = a.kind;
-    if (cnt == CNT_INFORMED) {
+    if (KIND != 2 && cnt == CNT_INFORMED) {
         write_informed<G>(a, cell, t);
         return;
     }
     const int n = cnt < 0 ? 0 : cnt;
-    const double mean = (kind == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
-    const float mf = (float)mean;
-    const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
-    const long long center = ((long long)a.wz * SY + a.wy) * SX + a.wx;
-    const int* nb = a.nboff + cell * a.kmax;
-
     int status = ST_KRIGED;
     double w[S], bq[S];
     int dnode[S];
@@ -253,159 +407,29 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
     if (n == 0) {
         status = ST_NO_NEIGHBOURS;
-    } else {
-        // Table-linear offsets of my rows; column offsets come by broadcast.
-        int tr[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int r = G * s + t;
-            int dx = 0, dy = 0, dz = 0;
-            if (r < n) unpack_off(nb[r], dx, dy, dz);
-            tr[s] = (dz * SY + dy) * SX + dx;
-            dnode[s] = (dz * g.ny + dy) * g.nx + dx;
-        }
-        T L[CH::TOT];
-        T D[S];
-        const T* tab = MIXED ? (const T*)(const void*)a.ctab32 : (const T*)(const void*)a.ctab;
-#pragma unroll
-        for (int c = 0; c < NMAX; ++c) {
-            const int tcol = gbc<G>(tr[c / G], c % G);
-#pragma unroll
-            for (int s = c / G; s < S; ++s) {
-                const int r = G * s + t;
-                T v = (r == c) ? T(1) : T(0);
-                if (r < n && c < n) v = tab[center + tr[s] - tcol];
-                L[slot_off<G>(s) + c] = v;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < S; ++s) bq[s] = (G * s + t < n) ? a.ctab[center + tr[s]] : 0.0;
-        const bool ok_kind = kind == 0;
-        T minpiv;
-        const bool ok = CH::factor(L, D, minpiv, t);
-        if (MIXED && (!ok || !((double)minpiv >= a.pivot_tol * a.c0))) {
-            // Queue the cell for the FP64 form (the whole group leaves together).
-            if (t == 0) {
-                const int pos = atomicAdd(a.n_fallback, 1);
-                a.fallback_cells[pos] = (int)cell;
-            }
-            return;
-        }
-        if (!ok) {
-            status = ST_SINGULAR;
-        } else {
-            double sv[S], ov[S];
-            T zs[S], zo[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                zs[s] = (T)bq[s];
-                zo[s] = (G * s + t < n) ? T(1) : T(0);
-            }
-            CH::forward(L, D, zs, t);
-            CH::backward(L, D, zs, t);
-            if (ok_kind) {
-                CH::forward(L, D, zo, t);
-                CH::backward(L, D, zo, t);
-            }
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                sv[s] = (double)zs[s];
-                ov[s] = ok_kind ? (double)zo[s] : 0.0;
-            }
-            if (MIXED) {
-                for (int it = 0; it < a.refine_steps; ++it) {
-                    // residuals b - A s and 1 - A o in FP64; the current
-                    // solutions are shared through LDS, A comes from the table.
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const int r = G * s + t;
-                        if (r < n) {
-                            xsh[r] = sv[s];
-                            xsh[NMAX + r] = ov[s];
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    double rs_[S], ro_[S];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        rs_[s] = bq[s];
-                        ro_[s] = (G * s + t < n) ? 1.0 : 0.0;
-                    }
-                    for (int c = 0; c < n; ++c) {
-                        int dx, dy, dz;
-                        unpack_off(nb[c], dx, dy, dz);
-                        const int tcol = (dz * SY + dy) * SX + dx;
-                        const double xs = xsh[c];
-                        const double xo = xsh[NMAX + c];
-#pragma unroll
-                        for (int s = 0; s < S; ++s) {
-                            if (G * s + t < n) {
-                                const double av = a.ctab[center + tr[s] - tcol];
-                                rs_[s] -= av * xs;
-                                ro_[s] -= av * xo;
-                            }
-                        }
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    T cs[S], co[S];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        cs[s] = (T)rs_[s];
-                        co[s] = (T)ro_[s];
-                    }
-                    CH::forward(L, D, cs, t);
-                    CH::backward(L, D, cs, t);
-                    if (ok_kind) {
-                        CH::forward(L, D, co, t);
-                        CH::backward(L, D, co, t);
-                    }
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        sv[s] += (double)cs[s];
-                        ov[s] += ok_kind ? (double)co[s] : 0.0;
-                    }
-                }
-            }
-            if (ok_kind) {
-                double ps = 0.0, po = 0.0;
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    ps += sv[s];
-                    po += ov[s];
-                }
-                const double sum_sk = gsum<G>(ps), sum_ones = gsum<G>(po);
-                if (fabs(sum_ones) < 1e-15) {
-                    status = ST_SINGULAR;
-                } else {
-                    mu = (sum_sk - 1) / sum_ones;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) w[s] = sv[s] - mu * ov[s];
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < S; ++s) w[s] = sv[s];
-            }
-        }
+    } else if (!cell_weights<NMAX, G, MIXED, KIND>(a, cell, t, n, xsh, w, bq, dnode, mu, status)) {
+        return;  // queued for the FP64 form
     }
     if (status != ST_KRIGED) {
 #pragma unroll
         for (int s = 0; s < S; ++s) w[s] = 0.0;
         mu = 0.0;
     }
-
     double pwb = 0.0;
 #pragma unroll
     for (int s = 0; s < S; ++s) pwb += w[s] * bq[s];
     const double wb = gsum<G>(pwb);
 
-    if (kind != 2) {
+    if (KIND != 2) {
+        const double mean = (KIND == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
+        const float mf = (float)mean;
         float term[S];
         double pe = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const float v = (G * s + t < n) ? a.values[cell + dnode[s]] : 0.0f;
+            const bool live = G * s + t < n;
+            const float v0 = a.values[cell + (live ? dnode[s] : 0)];
+            const float v = live ? v0 : 0.0f;
             term[s] = (float)(w[s] * (double)(v - mf));
             pe += w[s] * ((double)v - (double)mf);
         }
@@ -423,7 +447,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
                 if (a.out_mask) a.out_mask[cell] = 1;
                 if (a.out_var) a.out_var[cell] = var < 0 ? 0.0 : var;
                 if (a.out_est64) a.out_est64[cell] = e64;
-            } else if (kind == 1) {
+            } else if (KIND == 1) {
                 // mean_on_failure (cont_kriging.h:80-94)
                 if (a.out_val) a.out_val[cell] = mf;
                 if (a.out_mask) a.out_mask[cell] = 1;
@@ -442,69 +466,68 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
 
     // Median indicator kriging: one weight set, one combine per threshold.
-    constexpr int KMAXIND = 16;
-    float probs[KMAXIND];
-    const double var = a.c0 - wb;
+    for (int k = 0; k < a.nind; ++k) {
+        const float pf = (float)a.marg[k];
+        const float thr = a.zfield ? a.thresholds[k] : 0.f;
+        double term[S];
+        double pe = 0.0;
 #pragma unroll
-    for (int k = 0; k < KMAXIND; ++k) {
-        probs[k] = 0.f;
-        if (k < a.nind) {
-            const float pf = (float)a.marg[k];
-            double term[S];
-            double pe = 0.0;
+        for (int s = 0; s < S; ++s) {
+            const bool live = G * s + t < n;
+            const long long node = cell + (live ? dnode[s] : 0);
+            float iv;
+            if (a.zfield)
+                iv = (a.zfield[node] <= thr) ? 1.f : 0.f;
+            else
+                iv = (a.ind[(long long)k * g.n + node] == 1) ? 1.f : 0.f;
+            iv = live ? iv : 0.f;
+            term[s] = w[s] * (double)(iv - pf);
+            pe += w[s] * ((double)iv - (double)pf);
+        }
+        double acc = (double)pf;
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                float iv = 0.f;
-                if (G * s + t < n) {
-                    if (a.zfield)
-                        iv = (a.zfield[cell + dnode[s]] <= a.thresholds[k]) ? 1.f : 0.f;
-                    else
-                        iv = (a.ind[(long long)k * g.n + cell + dnode[s]] == 1) ? 1.f : 0.f;
-                }
-                term[s] = w[s] * (double)(iv - pf);
-                pe += w[s] * ((double)iv - (double)pf);
-            }
-            double acc = (double)pf;
-#pragma unroll
-            for (int i = 0; i < NMAX; ++i) {
-                const double ti = gbc<G>(term[i / G], i % G);
-                if (i < n) acc += ti;
-            }
-            const double e64 = (double)pf + gsum<G>(pe);
-            probs[k] = (float)acc;
-            if (t == 0) {
-                if (a.out_est64) a.out_est64[(long long)k * g.n + cell] = e64;
-                if (a.out_class) a.out_class[(long long)k * g.n + cell] = ((float)acc <= 0.5f) ? 0 : 1;
-            }
+        for (int i = 0; i < NMAX; ++i) {
+            const double ti = gbc<G>(term[i / G], i % G);
+            if (i < n) acc += ti;
+        }
+        const double e64 = (double)pf + gsum<G>(pe);
+        if (t == 0) {
+            if (a.out_est64) a.out_est64[(long long)k * g.n + cell] = e64;
+            if (a.out_class) a.out_class[(long long)k * g.n + cell] = ((float)acc <= 0.5f) ? 0 : 1;
+            if (a.out_prob) a.out_prob[(long long)k * g.n + cell] = (float)acc;
         }
     }
     if (t == 0) {
-        if (a.order_relations) {
-            // indicator_kriging.h:34-59
+        if (a.order_relations && a.out_prob) {
+            // indicator_kriging.h:34-59 on this cell's CDF (written above by
+            // this lane): clamp to [0,1], then one forward averaging pass.
+            constexpr int KMAXIND = 16;
+            float p[KMAXIND];
 #pragma unroll
-            for (int k = 0; k < KMAXIND; ++k)
-                if (k < a.nind) probs[k] = probs[k] < 0.0f ? 0.0f : (probs[k] > 1.0f ? 1.0f : probs[k]);
+            for (int k = 0; k < KMAXIND; ++k) {
+                const float v = k < a.nind ? a.out_prob[(long long)k * g.n + cell] : 0.f;
+                p[k] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+            }
 #pragma unroll
             for (int k = 0; k + 1 < KMAXIND; ++k)
-                if (k + 1 < a.nind && probs[k] > probs[k + 1]) {
-                    const double avg = ((double)probs[k] + (double)probs[k + 1]) / 2.0;
-                    probs[k] = (float)avg;
-                    probs[k + 1] = (float)avg;
+                if (k + 1 < a.nind && p[k] > p[k + 1]) {
+                    const double avg = ((double)p[k] + (double)p[k + 1]) / 2.0;
+                    p[k] = (float)avg;
+                    p[k + 1] = (float)avg;
                 }
-        }
-        if (a.out_prob) {
 #pragma unroll
             for (int k = 0; k < KMAXIND; ++k)
-                if (k < a.nind) a.out_prob[(long long)k * g.n + cell] = probs[k];
+                if (k < a.nind) a.out_prob[(long long)k * g.n + cell] = p[k];
         }
         if (a.out_mask) a.out_mask[cell] = 1;
+        const double var = a.c0 - wb;
         if (a.out_var) a.out_var[cell] = status == ST_KRIGED ? (var < 0 ? 0.0 : var) : -1.0;
         if (a.out_status) a.out_status[cell] = (unsigned char)status;
     }
 }
 
 // One group per cell; with a cell list, groups stride over the list.
-template <int NMAX, int G, bool MIXED>
+template <int NMAX, int G, bool MIXED, int KIND>
 __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     __shared__ double xsh_all[MIXED ? (256 / G) * 2 * NMAX : 1];
     double* xsh = xsh_all + (MIXED ? (threadIdx.x / G) * 2 * NMAX : 0);
@@ -514,7 +537,7 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     const long long total = a.cell_list ? (long long)*a.n_list : a.g.n;
     for (long long item = gid; item < total; item += ngroups) {
         const long long cell = a.cell_list ? (long long)a.cell_list[item] : item;
-        solve_cell<NMAX, G, MIXED>(a, cell, t, xsh);
+        solve_cell<NMAX, G, MIXED, KIND>(a, cell, t, xsh);
     }
 }
 
