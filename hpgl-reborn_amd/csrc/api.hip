@@ -234,7 +234,7 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     call.out_nboff = q->out_nboff;
     call.pivot_tol = q->pivot_tol > 0 ? q->pivot_tol : 1e-3;
     call.refine_steps = q->refine_steps >= 0 ? q->refine_steps : 1;
-    if (const char* gs = std::getenv("HPGLX_SOLVER_G")) call.solver_g = std::atoi(gs);
+    if (const char* sv = std::getenv("HPGLX_SOLVE")) call.solve_variant = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH")) call.search_variant = std::atoi(sv);
     if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
         err = "values and mask are required";

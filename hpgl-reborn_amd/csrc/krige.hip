@@ -27,6 +27,9 @@
 #include "krige_launch.h"
 #include "solve.h"
 
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 namespace hpglx {
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
         srt.sort(m);
         const int n = m < a.kmax ? m : a.kmax;
         for (int i = 0; i < n; ++i) a.nboff[cell * a.kmax + i] = st.get(i).off;
-        a.count[cell] = n;
+        a.count[cell] = n | CNT_CLUSTER;
     }
 }
 
@@ -462,6 +465,14 @@ bool DevPlan::upload(std::string& err) {
         release();
         return false;
     }
+    if (h.octant && !h.otab.empty()) {
+        if (hipMalloc(&otab, h.otab.size() * sizeof(double)) != hipSuccess ||
+            hipMemcpy(otab, h.otab.data(), h.otab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "hipMalloc/hipMemcpy failed for the octant table";
+            release();
+            return false;
+        }
+    }
     if (hipMemcpy(ctab, h.ctab.data(), nt * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ctab32, t32.data(), nt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
         (tsize > 0 && (hipMemcpy(tdx, h.tdx.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
@@ -480,6 +491,8 @@ void DevPlan::release() {
     if (tdx) (void)hipFree(tdx);
     if (tdy) (void)hipFree(tdy);
     if (tdz) (void)hipFree(tdz);
+    if (otab) (void)hipFree(otab);
+    otab = nullptr;
     ctab = nullptr;
     ctab32 = nullptr;
     tdx = tdy = tdz = nullptr;
@@ -519,40 +532,92 @@ void launch_solve(const SolveArgs& sa, long long ncells, hipStream_t st) {
     hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3(grid), dim3(block), 0, st, sa);
 }
 
-// FP64 form over a device-sized cell list (fallback of the mixed form).
-template <int NMAX, int G, int KIND>
+// Global-table form over a device-sized cell list (cluster-path cells, and
+// the FP64 re-solve of the mixed form's fallbacks).
+template <int NMAX, int G, bool MIXED, int KIND>
 void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st) {
     const int block = 256;
     long long blocks = (max_cells * G + block - 1) / block;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((k_solve<NMAX, G, false, KIND>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+    hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
 }
 
+// Persistent LDS-table form: as many blocks as are resident at once.  Returns
+// false (nothing launched) when the table does not fit a block's LDS.
+template <int NMAX, int G, bool MIXED, int KIND, int BLOCK>
+bool launch_solve_lds(const SolveArgs& sa, int nelem, hipStream_t st) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, int> grids;  // (device, LDS bytes) -> grid
+    const void* fn = (const void*)&k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>;
+    const int lds = nelem * (int)sizeof(double);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    int grid = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = grids.find({dev, lds});
+        if (it != grids.end()) {
+            grid = it->second;
+        } else {
+            int cus = 0, nb = 0;
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, BLOCK, lds) != hipSuccess)
+                nb = 0;
+            (void)hipGetLastError();
+            grid = nb * cus;
+            grids[{dev, lds}] = grid;
+        }
+    }
+    if (grid <= 0) return false;
+    hipLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), (size_t)lds, st,
+                       sa, nelem);
+    return true;
+}
+
+// Where the plan has an octant table: the LDS form over template-path cells,
+// then the global form over the cluster-path list; otherwise the global form
+// over every cell.  The mixed form then re-solves its fallbacks in FP64.
 // G64 / G32: lanes per cell of the FP64 and of the mixed form.
 template <int NMAX, int G64, int G32, int KIND>
-void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, hipStream_t st) {
-    if (mixed) {
+void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipStream_t st) {
+    constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
+    bool lds = false;
+    if (nelem > 0)
+        lds = mixed ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32>(sa, nelem, st)
+                    : launch_solve_lds<NMAX, G64, false, KIND, 256>(sa, nelem, st);
+    if (lds) {
+        SolveArgs cl = sa;
+        cl.cell_list = sa.cl_cells;
+        cl.n_list = sa.n_cl_cells;
+        if (mixed)
+            launch_solve_list<NMAX, G32, true, KIND>(cl, n, st);
+        else
+            launch_solve_list<NMAX, G64, false, KIND>(cl, n, st);
+    } else if (mixed) {
         launch_solve<NMAX, G32, true, KIND>(sa, n, st);
+    } else {
+        launch_solve<NMAX, G64, false, KIND>(sa, n, st);
+    }
+    if (mixed) {
         SolveArgs fb = sa;
         fb.cell_list = sa.fallback_cells;
         fb.n_list = sa.n_fallback;
         fb.fallback_cells = nullptr;
         fb.n_fallback = nullptr;
-        launch_solve_list<NMAX, G64, KIND>(fb, n, st);
-    } else {
-        launch_solve<NMAX, G64, false, KIND>(sa, n, st);
+        launch_solve_list<NMAX, G64, false, KIND>(fb, n, st);
     }
 }
 
 template <int NMAX, int G64, int G32>
-void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, hipStream_t st) {
+void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, hipStream_t st) {
     if (kind == 0)
-        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, st);
+        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, nelem, st);
     else if (kind == 1)
-        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, st);
+        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, nelem, st);
     else
-        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, st);
+        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, nelem, st);
 }
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -745,6 +810,10 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.wx = h.W[0];
     so.wy = h.W[1];
     so.wz = h.W[2];
+    so.otab = plan.otab;
+    so.ex = h.E[0];
+    so.ey = h.E[1];
+    so.ez = h.E[2];
     so.c0 = h.c0();
     so.pivot_tol = c.pivot_tol;
     so.refine_steps = c.refine_steps;
@@ -758,18 +827,21 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.order_relations = c.order_relations;
     so.cell_list = nullptr;
     so.n_list = nullptr;
+    so.cl_cells = cl_cells;
+    so.n_cl_cells = n_cl_cells;
     so.fallback_cells = fb_cells;
     so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[2], st));
+    const int nelem = (plan.otab && c.solve_variant != 1) ? (int)h.otab.size() : 0;
     if (K <= 8)
-        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, st);
+        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, nelem, st);
     else if (K <= 16)
-        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, st);
+        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, nelem, st);
     else if (K <= 24)
-        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, st);
+        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, nelem, st);
     else
-        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, st);
+        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, nelem, st);
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[3], st));
     HPGLX_CHECK(hipGetLastError());
     return 0;

@@ -21,8 +21,10 @@ __host__ __device__ inline void unpack_off(int p, int& dx, int& dy, int& dz) {
 // Cell status written by the solve kernel (same codes as the oracle).
 enum CellStatus { ST_KRIGED = 0, ST_NO_NEIGHBOURS = 1, ST_SINGULAR = 2, ST_INFORMED = 3 };
 
-// count[] markers set by the search kernels.
-enum { CNT_INFORMED = -1, CNT_PENDING_CLUSTER = -2 };
+// count[] markers set by the search kernels.  A non-negative count holds the
+// number of neighbours in bits 0-15; CNT_CLUSTER marks a cell whose neighbours
+// came from the cluster path (offsets up to 4r - 2 instead of r).
+enum { CNT_INFORMED = -1, CNT_PENDING_CLUSTER = -2, CNT_CLUSTER = 1 << 16, CNT_MASK = 0xFFFF };
 
 struct GridDims {
     int nx, ny, nz;
@@ -89,6 +91,8 @@ struct SolveArgs {
     const double* ctab;
     const float* ctab32;
     int wx, wy, wz;
+    const double* otab;            // octant table (or null), staged in LDS by k_solve_lds
+    int ex, ey, ez;                // its extents (|d| <= e)
     double c0;                     // C(0)
     double pivot_tol;              // mixed precision: FP64 fallback threshold
     int refine_steps;
@@ -103,6 +107,8 @@ struct SolveArgs {
     int order_relations;           // MIK CDF: apply the order-relation correction
     const int* cell_list;          // when set: krige only these cells
     const int* n_list;             // its length (device)
+    const int* cl_cells;           // cluster-path cells (solved from the global table)
+    const int* n_cl_cells;         // their number (device)
     int* fallback_cells;           // mixed precision: cells queued for the FP64 form
     int* n_fallback;               // its length (device counter)
 };

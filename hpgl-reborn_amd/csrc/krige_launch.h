@@ -18,6 +18,7 @@ struct DevPlan {
     int* tdx = nullptr;
     int* tdy = nullptr;
     int* tdz = nullptr;
+    double* otab = nullptr;
     int tsize = 0;
     bool upload(std::string& err);
     void release();
@@ -57,8 +58,8 @@ struct GridCall {
     int* out_nboff = nullptr;               // debug: packed offsets (device, N x kmax)
     double pivot_tol = 1e-3;
     int refine_steps = 1;
-    int solver_g = 0;                       // lanes per cell (4 or 8); 0 = default
     int search_variant = 0;                 // 0 = LDS tiles when the box fits, 1 = global walk
+    int solve_variant = 0;                  // 0 = LDS covariance table when it fits, 1 = global table
     bool profile = false;
 };
 

@@ -16,6 +16,8 @@
 //   limit      src/geo_bsd/hpgl/sugarbox_indexed_neighbour_lookup.h:126 (|template| / 250)
 #include "plan.h"
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <thread>
@@ -183,6 +185,33 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
         pool.emplace_back(fill_planes, z0, z1);
     }
     for (auto& th : pool) th.join();
+
+    // Octant table for the LDS solve: only when it fits one workgroup's LDS
+    // and the symmetry holds bit for bit over the whole +-2r box.
+    p.octant = false;
+    p.otab.clear();
+    for (int i = 0; i < 3; ++i) p.E[i] = 2 * p.r[i];
+    const int ox = p.E[0] + 1, oy = p.E[1] + 1, oz = p.E[2] + 1;
+    const size_t on = (size_t)ox * oy * oz;
+    if (cov.angles[0] == 0 && cov.angles[1] == 0 && cov.angles[2] == 0 && on * sizeof(double) <= kOctantMaxBytes) {
+        auto at = [&](int x, int y, int z) {
+            return p.ctab[((size_t)(z + p.W[2]) * sy + (y + p.W[1])) * sx + (x + p.W[0])];
+        };
+        p.otab.resize(on);
+        bool same = true;
+        for (int z = 0; z < oz && same; ++z)
+            for (int y = 0; y < oy && same; ++y)
+                for (int x = 0; x < ox && same; ++x) {
+                    const double v = at(x, y, z);
+                    p.otab[((size_t)z * oy + y) * ox + x] = v;
+                    for (int m = 1; m < 8; ++m) {
+                        const double u = at(m & 1 ? -x : x, m & 2 ? -y : y, m & 4 ? -z : z);
+                        if (std::memcmp(&u, &v, sizeof v) != 0) same = false;
+                    }
+                }
+        p.octant = same;
+        if (!same) p.otab.clear();
+    }
     return true;
 }
 

@@ -24,6 +24,9 @@ struct CovParams {
     double nugget;
 };
 
+// Largest octant table staged in LDS (one workgroup's LDS on gfx950).
+constexpr size_t kOctantMaxBytes = 160 * 1024;
+
 struct PlanHost {
     Kind kind;
     CovParams cov;
@@ -37,6 +40,14 @@ struct PlanHost {
     // axis covers every neighbour pair of either search path.
     int W[3];
     std::vector<double> ctab;
+    // Octant table: with all angles zero the covariance is even in each
+    // offset component separately, so C(dx,dy,dz) = C(|dx|,|dy|,|dz|) bit for
+    // bit (checked entry by entry when the plan is built).  otab covers
+    // |d| <= E = 2r (every pair of template-path neighbours), x fastest; empty
+    // when the angles are not all zero or it exceeds kOctantMaxBytes.
+    bool octant = false;
+    int E[3] = {0, 0, 0};
+    std::vector<double> otab;
     // Clusterizer capacity: |template| / 250.
     long long limit;
     double c0() const;  // C(0) = the sill

@@ -220,10 +220,22 @@ __device__ __forceinline__ void write_informed(const SolveArgs& a, long long cel
     if (a.out_status) a.out_status[cell] = ST_INFORMED;
 }
 
+// |a - b| + acc on non-negative ints (v_sad_u32).
+__device__ __forceinline__ unsigned adiff(unsigned a, unsigned b, unsigned acc) {
+    return (a > b ? a - b : b - a) + acc;
+}
+
+// LDS form of the covariance lookup: the octant table staged in LDS is
+// indexed by |dx|, |dy|, |dz| of the pair's offset difference.  Each
+// neighbour offset is kept as three biased, pre-scaled components
+// ((d + OFF_BIAS) * stride * 8), so a pair's byte address is the sum of three
+// absolute differences.
+constexpr int OFF_BIAS = 1024;
+
 // Weights of one cell: w[s] for the lane's rows, mu (OK), status.  Returns
 // false when the mixed form hands the cell to the FP64 form.
-template <int NMAX, int G, bool MIXED, int KIND>
-__device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, double* xsh,
+template <int NMAX, int G, bool MIXED, int KIND, bool LTAB>
+__device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, const double* ltab,
                                              double (&w)[NMAX / G], double (&bq)[NMAX / G], int (&dnode)[NMAX / G],
                                              double& mu, int& status) {
     using T = typename std::conditional<MIXED, float, double>::type;
@@ -237,12 +249,14 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 
     // Table-linear offsets of my rows; column offsets come by broadcast.
     int tr[S];
+    unsigned ux[S], uy[S], uz[S];
     int pk[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int r = G * s + t;
         pk[s] = nb[r < n ? r : 0];  // unconditional, so the loads issue together
     }
+    const unsigned OX = (unsigned)(a.ex + 1), OXY = OX * (unsigned)(a.ey + 1);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int r = G * s + t;
@@ -251,24 +265,47 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         if (r >= n) dx = dy = dz = 0;
         tr[s] = (dz * SY + dy) * SX + dx;
         dnode[s] = (dz * g.ny + dy) * g.nx + dx;
+        ux[s] = (unsigned)(dx + OFF_BIAS) * 8u;
+        uy[s] = (unsigned)(dy + OFF_BIAS) * (OX * 8u);
+        uz[s] = (unsigned)(dz + OFF_BIAS) * (OXY * 8u);
     }
+    const char* lb = (const char*)ltab;
+    // LDS covariance of my slot s with the neighbour (cx, cy, cz).
+    auto lcov = [&](int s, unsigned cx, unsigned cy, unsigned cz) -> double {
+        return *(const double*)(lb + adiff(ux[s], cx, adiff(uy[s], cy, adiff(uz[s], cz, 0u))));
+    };
     T L[CH::TOT];
     T D[S];
-    const T* tab = MIXED ? (const T*)(const void*)a.ctab32 : (const T*)(const void*)a.ctab;
 #pragma unroll
     for (int c = 0; c < NMAX; ++c) {
-        const int tcol = gbc<G>(tr[c / G], c % G);
+        int tcol = 0;
+        unsigned cx = 0, cy = 0, cz = 0;
+        if (LTAB) {
+            cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
+            cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+            cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
+        } else {
+            tcol = gbc<G>(tr[c / G], c % G);
+        }
 #pragma unroll
         for (int s = c / G; s < S; ++s) {
             const int r = G * s + t;
             const bool live = r < n && c < n;
-            const T v = tab[live ? center + tr[s] - tcol : center];
+            T v;
+            if (LTAB)
+                v = (T)lcov(s, cx, cy, cz);
+            else if (MIXED)
+                v = (T)a.ctab32[live ? center + tr[s] - tcol : center];
+            else
+                v = (T)a.ctab[live ? center + tr[s] - tcol : center];
             L[slot_off<G>(s) + c] = live ? v : ((r == c) ? T(1) : T(0));
         }
     }
+    const unsigned c0x = (unsigned)OFF_BIAS * 8u, c0y = (unsigned)OFF_BIAS * (OX * 8u),
+                   c0z = (unsigned)OFF_BIAS * (OXY * 8u);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        const double v = a.ctab[center + tr[s]];
+        const double v = LTAB ? lcov(s, c0x, c0y, c0z) : a.ctab[center + tr[s]];
         bq[s] = (G * s + t < n) ? v : 0.0;
     }
     T minpiv;
@@ -305,42 +342,37 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
     }
     if (MIXED) {
         for (int it = 0; it < a.refine_steps; ++it) {
-            // Residuals b - A s (and 1 - A o) in FP64: the current solutions
-            // are shared through LDS, A comes from the FP64 table.
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const int r = G * s + t;
-                if (r < n) {
-                    xsh[r] = sv[s];
-                    if (OKK) xsh[NMAX + r] = ov[s];
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Residuals b - A s (and 1 - A o) in FP64 with A from the FP64
+            // table; the current solutions reach every lane by broadcast.
             double rs_[S], ro_[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 rs_[s] = bq[s];
                 ro_[s] = (G * s + t < n) ? 1.0 : 0.0;
             }
-            for (int c = 0; c < n; ++c) {
-                int dx, dy, dz;
-                unpack_off(nb[c], dx, dy, dz);
-                const int tcol = (dz * SY + dy) * SX + dx;
-                const double xs = xsh[c];
-                const double xo = OKK ? xsh[NMAX + c] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NMAX; ++c) {
+                int tcol = 0;
+                unsigned cx = 0, cy = 0, cz = 0;
+                if (LTAB) {
+                    cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
+                    cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+                    cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
+                } else {
+                    tcol = gbc<G>(tr[c / G], c % G);
+                }
+                const double xs = gbc<G>(sv[c / G], c % G);
+                const double xo = OKK ? gbc<G>(ov[c / G], c % G) : 0.0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const bool live = G * s + t < n;
-                    const double av = a.ctab[live ? center + tr[s] - tcol : center];
+                    const bool live = G * s + t < n && c < n;
+                    const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[live ? center + tr[s] - tcol : center];
                     if (live) {
                         rs_[s] -= av * xs;
                         if (OKK) ro_[s] -= av * xo;
                     }
                 }
             }
-            __builtin_amdgcn_wave_barrier();
             {
                 T z[S];
 #pragma unroll
@@ -383,18 +415,17 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
     return true;
 }
 
-// Kriges one cell with the group of G lanes; t = lane index in the group;
-// xsh = this group's LDS scratch (2 * NMAX doubles) for the mixed form.
-template <int NMAX, int G, bool MIXED, int KIND>
-__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int t, double* xsh) {
+// Kriges one cell with the group of G lanes; t = lane index in the group,
+// cnt = the cell's count[] entry, ltab = the LDS octant table (LTAB form).
+template <int NMAX, int G, bool MIXED, int KIND, bool LTAB>
+__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int cnt, int t, const double* ltab) {
     constexpr int S = NMAX / G;
     const GridDims& g = a.g;
-    const int cnt = a.count[cell];
     if (KIND != 2 && cnt == CNT_INFORMED) {
         write_informed<G>(a, cell, t);
         return;
     }
-    const int n = cnt < 0 ? 0 : cnt;
+    const int n = cnt < 0 ? 0 : (cnt & CNT_MASK);
     int status = ST_KRIGED;
     double w[S], bq[S];
     int dnode[S];
@@ -407,7 +438,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
     if (n == 0) {
         status = ST_NO_NEIGHBOURS;
-    } else if (!cell_weights<NMAX, G, MIXED, KIND>(a, cell, t, n, xsh, w, bq, dnode, mu, status)) {
+    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB>(a, cell, t, n, ltab, w, bq, dnode, mu, status)) {
         return;  // queued for the FP64 form
     }
     if (status != ST_KRIGED) {
@@ -526,18 +557,35 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
 }
 
-// One group per cell; with a cell list, groups stride over the list.
+// One group per cell (global covariance table); with a cell list, groups
+// stride over the list.
 template <int NMAX, int G, bool MIXED, int KIND>
 __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-    __shared__ double xsh_all[MIXED ? (256 / G) * 2 * NMAX : 1];
-    double* xsh = xsh_all + (MIXED ? (threadIdx.x / G) * 2 * NMAX : 0);
     const int t = threadIdx.x & (G - 1);
     const long long gid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const long long ngroups = ((long long)gridDim.x * blockDim.x) / G;
     const long long total = a.cell_list ? (long long)*a.n_list : a.g.n;
     for (long long item = gid; item < total; item += ngroups) {
         const long long cell = a.cell_list ? (long long)a.cell_list[item] : item;
-        solve_cell<NMAX, G, MIXED, KIND>(a, cell, t, xsh);
+        solve_cell<NMAX, G, MIXED, KIND, false>(a, cell, a.count[cell], t, nullptr);
+    }
+}
+
+// Persistent form with the octant covariance table staged in LDS: every
+// template-path cell; cluster-path cells (pair offsets beyond the octant
+// table) are left to k_solve over the cluster cell list.
+template <int NMAX, int G, bool MIXED, int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_solve_lds(SolveArgs a, int nelem) {
+    extern __shared__ double otab_lds[];
+    for (int i = threadIdx.x; i < nelem; i += BLOCK) otab_lds[i] = a.otab[i];
+    __syncthreads();
+    const int t = threadIdx.x & (G - 1);
+    const long long gid = ((long long)blockIdx.x * BLOCK + threadIdx.x) / G;
+    const long long ngroups = ((long long)gridDim.x * BLOCK) / G;
+    for (long long cell = gid; cell < a.g.n; cell += ngroups) {
+        const int cnt = a.count[cell];
+        if (cnt >= 0 && (cnt & CNT_CLUSTER)) continue;
+        solve_cell<NMAX, G, MIXED, KIND, true>(a, cell, cnt, t, otab_lds);
     }
 }
 

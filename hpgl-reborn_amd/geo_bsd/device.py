@@ -142,6 +142,11 @@ def krige(kind, grid, radiuses, max_neighbours, cov_model, mask, values=None, in
     if stream is None:
         stream = torch.cuda.current_stream(dev).cuda_stream
     _call(q, stream)
+    if neighbours:
+        # Bit 16 of a non-negative count marks a cluster-path cell.
+        raw = out["count"]
+        out["cluster_path"] = (raw >= 0) & ((raw & 0x10000) != 0)
+        out["count"] = torch.where(raw >= 0, raw & 0xFFFF, raw)
     return out
 
 

@@ -35,7 +35,7 @@ typedef struct hpglx_grid_request_t {
     int precision;             /* HPGLX_FP64 or HPGLX_MIXED (FP32 Cholesky + FP64 refinement) */
     int dims[3];               /* nx, ny, nz */
     int radiuses[3];
-    int max_neighbours;        /* 1..64 */
+    int max_neighbours;        /* 1..32 */
     struct hpgl_cov_params_t cov;
     const float* values;       /* OK/SK: data */
     const unsigned char* mask; /* nonzero = informed */
@@ -54,7 +54,9 @@ typedef struct hpglx_grid_request_t {
     double* out_est64;         /* estimate accumulated in FP64 (nind x N for MIK) */
     unsigned char* out_status; /* HPGLX_KRIGED ... HPGLX_INFORMED */
     int order_relations;       /* MIK: correct the CDF (indicator_kriging.h:34-59) */
-    int* out_count;            /* neighbour count per cell (-1 on informed OK/SK cells) */
+    int* out_count;            /* neighbour count per cell in bits 0-15; bit 16 set when the cell's
+                                  neighbours came from the cluster search (clusterizer.cpp);
+                                  -1 on informed OK/SK cells */
     int* out_nboff;            /* N x max_neighbours packed offsets, see hpglx_unpack_offset */
     double pivot_tol;          /* MIXED: re-solve in FP64 when min pivot < tol * C(0); <= 0 -> 1e-3 */
     int refine_steps;          /* MIXED: FP64 residual refinement steps; < 0 -> 1 */

@@ -83,6 +83,42 @@ def test_neighbour_sets_bit_exact(orc, name, scale):
     assert np.array_equal(g["count"][searched], o["count"][searched])
     nodes = _node_lists(wl, g["count"], g["nboff"])
     assert np.array_equal(nodes[searched], o["nb"][searched].astype(np.int64))
+    # the search path (template walk or cluster gather) agrees cell by cell
+    assert np.array_equal(g["cluster_path"][searched], o["path"][searched].astype(bool))
+
+
+VARIANT_CASES = [("sk", 0.5), ("ok", 0.4), ("mik", 0.25), ("wells", 0.2)]
+
+
+@pytest.mark.parametrize("name,scale", VARIANT_CASES)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_kernel_variants_bitwise(monkeypatch, name, scale, precision):
+    """The LDS-table solve and the LDS-tile search give the same bits as the
+    global-memory forms (same table values, same operation order)."""
+    wl = _wl(name, scale)
+    base = _device(wl, precision)
+    monkeypatch.setenv("HPGLX_SOLVE", "1")
+    monkeypatch.setenv("HPGLX_SEARCH", "1")
+    alt = _device(wl, precision)
+    assert base.keys() == alt.keys()
+    for key in base:
+        a, b = base[key], alt[key]
+        assert a.dtype == b.dtype and a.shape == b.shape, key
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), key
+
+
+def test_rotated_model_global_table(orc):
+    """A rotated model has no octant symmetry: the solve reads the full table."""
+    wl = _wl("ok", 0.25)
+    wl.cov = dict(wl.cov, angles=(30.0, 0.0, 0.0))
+    o = _oracle(orc, wl)
+    g = _device(wl)
+    kr = o["status"] == orc.ST_KRIGED
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    assert np.array_equal(_node_lists(wl, g["count"], g["nboff"])[kr], o["nb"][kr].astype(np.int64))
+    assert np.allclose(g["est64"][kr], o["est64"][kr], rtol=1e-9, atol=1e-12)
+    gm = _device(wl, "mixed")
+    assert np.abs(gm["est64"][kr] - o["est64"][kr]).max() <= 1e-6
 
 
 @pytest.mark.parametrize("name,scale", CASES)
