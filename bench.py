@@ -27,6 +27,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6    # MI355X vector FP64
 PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (packed)
 
+# HBM bytes per launch of the main solve kernel on this workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of
+# known size (profiles/hbm_traffic_ok_100x100x50.md); kernels not measured
+# report null.
+TRAFFIC_BYTES = {}
+
 
 def flops_ok(n):
     """Algorithmic flops of one OK cell with n neighbours: covariance evaluations
@@ -35,12 +41,6 @@ def flops_ok(n):
     n = n.astype(np.float64)
     e = n * (n + 1) / 2 + n + 1
     return 25 * e + n ** 3 / 3 + n ** 2 / 2 + n / 6 + 4 * n ** 2 + 10 * n
-
-
-def run_steps(D, wl, model, v, m, precision, steps, torch):
-    out = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=precision, variance=True)
-    torch.cuda.synchronize()
-    return out
 
 
 def cpu_baseline(wl):
@@ -102,45 +102,44 @@ def main():
     kriged = int((wl.mask == 0).sum())
     stream = torch.cuda.current_stream(dev)
 
-    # Neighbour counts for the algorithmic flop count (untimed).
+    # Neighbour counts and search paths for the algorithmic flop count (untimed).
     nb = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, neighbours=True, variance=False)
     torch.cuda.synchronize()
     counts = nb["count"].cpu().numpy()
-    counts = counts[counts > 0]
-    flops_per_pass = float(flops_ok(counts).sum())
+    cluster = nb["cluster_path"].cpu().numpy()
+    flops_all = float(flops_ok(counts[counts > 0]).sum())
+    # the main solve kernel takes the template-path cells; cluster-path cells
+    # go to a second, list-driven launch
+    main_cells = (counts > 0) & ~cluster
+    flops_main = float(flops_ok(counts[main_cells]).sum())
 
     results = {}
     for prec in ("fp64", "mixed"):
         for _ in range(args.warmup):
             D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
         torch.cuda.synchronize()
+        # Timed region: the library records HIP events on the launch stream
+        # around the search and the solve kernels of every step.
+        D.set_profiling(True)
         t0 = time.perf_counter()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for _ in range(args.steps):
-            out = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
+            D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        ms = ev0.elapsed_time(ev1) / args.steps
-        # Per-kernel durations: HIP events recorded by the library around the
-        # search and the solve kernels on the launch stream.
-        D.set_profiling(True)
-        search, solve = [], []
-        for _ in range(max(5, min(args.steps, 20))):
-            D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
-            s_ms, v_ms, _ = D.last_timing()
-            search.append(s_ms)
-            solve.append(v_ms)
         D.set_profiling(False)
+        prof = D.profile_summary()
+        assert prof["calls"] == args.steps, prof
+        ms = ev0.elapsed_time(ev1) / args.steps
         stats = D.last_call_stats()
         est = D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, est64=True,
                       status=True)
         torch.cuda.synchronize()
-        results[prec] = dict(ms=ms, wall_ms=wall * 1e3 / args.steps, search_ms=statistics.mean(search),
-                             solve_ms=statistics.mean(solve), est64=est["est64"].cpu().numpy(),
-                             status=est["status"].cpu().numpy(), fallbacks=stats["fp64_fallbacks"])
+        results[prec] = dict(ms=ms, wall_ms=wall * 1e3 / args.steps, prof=prof, est64=est["est64"].cpu().numpy(),
+                             fallbacks=stats["fp64_fallbacks"], cluster_cells=stats["cluster_path_cells"])
 
     err = {}
     if not args.no_oracle:
@@ -149,10 +148,21 @@ def main():
         for prec in results:
             err[prec] = float(np.abs(results[prec]["est64"][kr] - o["est64"][kr]).max())
 
+    def phase(r):
+        p = r["prof"]
+        return {"ms_per_step": r["ms"], "host_wall_ms_per_step": r["wall_ms"], "search_ms": p["search"],
+                "solve_main_ms": p["solve_main"], "solve_ms": p["solve"], "call_ms": p["call"]}
+
+    def roof(r, peak, kernel):
+        achieved = flops_main / (r["prof"]["solve_main"] / 1e3) / 1e12
+        return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": TRAFFIC_BYTES.get(kernel), "flops_per_launch": flops_main,
+                "launch_ms": r["prof"]["solve_main"]}
+
     mixed = results["mixed"]
     fp64 = results["fp64"]
-    solve_s = mixed["solve_ms"] / 1e3
-    achieved = flops_per_pass / solve_s / 1e12
+    k_mixed = "k_solve_lds<24,4,mixed,OK>"
+    k_fp64 = "k_solve_lds<24,4,fp64,OK>"
     line = {
         "metric": "kriged cells/s on 1 MI355X: FP32+refine vs FP64, at <=1e-6 estimate error",
         "value": kriged / (mixed["ms"] / 1e3),
@@ -164,22 +174,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "f32+f64",
         "data": "synthetic (seeded moving-average field, 5% hard data)",
         "config": {"workload": "ordinary kriging 100x100x50 (500k cells, 475k kriged), exponential (20,20,4), "
                                "nugget 0.05, radiuses (20,20,4), max_neighbours 24", "cells": 500000,
-                   "kriged_cells": kriged, "solver": "FP32 Cholesky + 2 FP64 refinement steps, FP64 fallback"},
-        "fp64": {"value": kriged / (fp64["ms"] / 1e3), "ms_per_step": fp64["ms"], "solve_ms": fp64["solve_ms"],
-                 "search_ms": fp64["search_ms"], "max_abs_err_vs_oracle": err.get("fp64")},
-        "mixed": {"ms_per_step": mixed["ms"], "solve_ms": mixed["solve_ms"], "search_ms": mixed["search_ms"],
-                  "max_abs_err_vs_oracle": err.get("mixed"), "fp64_fallbacks": mixed["fallbacks"]},
+                   "kriged_cells": kriged,
+                   "solver": "FP32 Cholesky + 1 FP64 residual refinement step, FP64 re-solve on small pivots"},
+        "mixed": dict(phase(mixed), max_abs_err_vs_oracle=err.get("mixed"), fp64_fallbacks=mixed["fallbacks"],
+                      cluster_path_cells=mixed["cluster_cells"]),
+        "fp64": dict(phase(fp64), value=kriged / (fp64["ms"] / 1e3), max_abs_err_vs_oracle=err.get("fp64"),
+                     roofline=roof(fp64, PEAK_FP64_TFLOPS, k_fp64)),
         "speedup_mixed_vs_fp64": fp64["ms"] / mixed["ms"],
-        "roofline": {"bound": "valu", "kernel": "k_solve<24,32,mixed>", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops_per_pass,
-                     "fp64_form": {"achieved": flops_per_pass / (fp64["solve_ms"] / 1e3) / 1e12,
-                                   "peak": PEAK_FP64_TFLOPS,
-                                   "frac": flops_per_pass / (fp64["solve_ms"] / 1e3) / 1e12 / PEAK_FP64_TFLOPS}},
+        "roofline": roof(mixed, PEAK_FP32_TFLOPS, k_mixed),
+        "flops_per_step": flops_all,
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl)

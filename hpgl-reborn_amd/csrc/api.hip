@@ -118,7 +118,10 @@ struct Library {
     int thread_num = 1;
     int default_precision = PREC_FP64;
     bool profiling = false;
-    CallTiming timing;
+    // Event sets of the profiled calls since profiling was switched on; the
+    // newest kTimingPool calls are kept.
+    std::vector<CallTiming> tpool;
+    size_t tcalls = 0;
     hipEvent_t done = nullptr;   // completion of the last call's work
     hipStream_t io_stream = nullptr;
 };
@@ -187,15 +190,35 @@ bool ensure_events(Library& L, std::string& err) {
             return false;
         }
     }
-    if (!L.timing.valid) {
-        for (int i = 0; i < 4; ++i)
-            if (hipEventCreate(&L.timing.ev[i]) != hipSuccess) {
-                err = "hipEventCreate failed";
-                return false;
-            }
-        L.timing.valid = true;
-    }
     return true;
+}
+
+constexpr size_t kTimingPool = 4096;
+
+// The event set for the next profiled call.
+CallTiming* next_timing(Library& L, std::string& err) {
+    const size_t slot = L.tcalls % kTimingPool;
+    if (slot >= L.tpool.size()) {
+        CallTiming t;
+        for (int i = 0; i < CallTiming::N; ++i)
+            if (hipEventCreate(&t.ev[i]) != hipSuccess) {
+                err = "hipEventCreate failed";
+                return nullptr;
+            }
+        t.valid = true;
+        L.tpool.push_back(t);
+    }
+    ++L.tcalls;
+    return &L.tpool[slot];
+}
+
+// Milliseconds {search, main solve kernel, whole solve, whole call} of one call.
+bool read_timing(const CallTiming& t, float ms[4]) {
+    if (hipEventSynchronize(t.ev[CallTiming::N - 1]) != hipSuccess) return false;
+    return hipEventElapsedTime(&ms[0], t.ev[0], t.ev[1]) == hipSuccess &&
+           hipEventElapsedTime(&ms[1], t.ev[2], t.ev[3]) == hipSuccess &&
+           hipEventElapsedTime(&ms[2], t.ev[2], t.ev[4]) == hipSuccess &&
+           hipEventElapsedTime(&ms[3], t.ev[0], t.ev[4]) == hipSuccess;
 }
 
 int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err) {
@@ -252,7 +275,8 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
         err = "hipStreamWaitEvent failed";
         return -1;
     }
-    CallTiming* t = L.profiling ? &L.timing : nullptr;
+    CallTiming* t = nullptr;
+    if (L.profiling && !(t = next_timing(L, err))) return -1;
     int rc = launch_grid(*plan, L.ws, call, st, t, err);
     if (rc == 0 && hipEventRecord(L.done, st) != hipSuccess) {
         err = "hipEventRecord failed";
@@ -658,21 +682,36 @@ void hpglx_set_profiling(int on) {
     Library& L = lib();
     std::lock_guard<std::mutex> g(L.mu);
     L.profiling = on != 0;
+    if (L.profiling) L.tcalls = 0;
 }
 
 int hpglx_last_timing(float* ms3) {
     Library& L = lib();
     std::lock_guard<std::mutex> g(L.mu);
-    if (!L.timing.valid) return fail(L, "no profiled call");
-    if (hipEventSynchronize(L.timing.ev[3]) != hipSuccess) return fail(L, "hipEventSynchronize failed");
-    float a = 0, b = 0, c = 0;
-    if (hipEventElapsedTime(&a, L.timing.ev[0], L.timing.ev[1]) != hipSuccess ||
-        hipEventElapsedTime(&b, L.timing.ev[2], L.timing.ev[3]) != hipSuccess ||
-        hipEventElapsedTime(&c, L.timing.ev[0], L.timing.ev[3]) != hipSuccess)
-        return fail(L, "hipEventElapsedTime failed (was profiling on for the last call?)");
-    ms3[0] = a;
-    ms3[1] = b;
-    ms3[2] = c;
+    L.last_error.clear();
+    if (L.tcalls == 0) return fail(L, "no profiled call");
+    float ms[4];
+    if (!read_timing(L.tpool[(L.tcalls - 1) % kTimingPool], ms)) return fail(L, "reading the call's events failed");
+    ms3[0] = ms[0];
+    ms3[1] = ms[2];
+    ms3[2] = ms[3];
+    return 0;
+}
+
+int hpglx_profile_summary(float* ms4, int* ncalls) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    const size_t n = L.tcalls < kTimingPool ? L.tcalls : kTimingPool;
+    if (n == 0) return fail(L, "no profiled call");
+    double sum[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < n; ++i) {
+        float ms[4];
+        if (!read_timing(L.tpool[i], ms)) return fail(L, "reading the call's events failed");
+        for (int k = 0; k < 4; ++k) sum[k] += ms[k];
+    }
+    for (int k = 0; k < 4; ++k) ms4[k] = (float)(sum[k] / (double)n);
+    if (ncalls) *ncalls = (int)n;
     return 0;
 }
 

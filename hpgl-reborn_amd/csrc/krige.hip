@@ -581,12 +581,19 @@ bool launch_solve_lds(const SolveArgs& sa, int nelem, hipStream_t st) {
 // over every cell.  The mixed form then re-solves its fallbacks in FP64.
 // G64 / G32: lanes per cell of the FP64 and of the mixed form.
 template <int NMAX, int G64, int G32, int KIND>
-void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipStream_t st) {
+void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipEvent_t after_main, hipStream_t st) {
     constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
     bool lds = false;
     if (nelem > 0)
         lds = mixed ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32>(sa, nelem, st)
                     : launch_solve_lds<NMAX, G64, false, KIND, 256>(sa, nelem, st);
+    if (!lds) {
+        if (mixed)
+            launch_solve<NMAX, G32, true, KIND>(sa, n, st);
+        else
+            launch_solve<NMAX, G64, false, KIND>(sa, n, st);
+    }
+    if (after_main) (void)hipEventRecord(after_main, st);
     if (lds) {
         SolveArgs cl = sa;
         cl.cell_list = sa.cl_cells;
@@ -595,10 +602,6 @@ void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipS
             launch_solve_list<NMAX, G32, true, KIND>(cl, n, st);
         else
             launch_solve_list<NMAX, G64, false, KIND>(cl, n, st);
-    } else if (mixed) {
-        launch_solve<NMAX, G32, true, KIND>(sa, n, st);
-    } else {
-        launch_solve<NMAX, G64, false, KIND>(sa, n, st);
     }
     if (mixed) {
         SolveArgs fb = sa;
@@ -611,13 +614,14 @@ void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipS
 }
 
 template <int NMAX, int G64, int G32>
-void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, hipStream_t st) {
+void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, hipEvent_t after_main,
+                    hipStream_t st) {
     if (kind == 0)
-        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, nelem, st);
+        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, nelem, after_main, st);
     else if (kind == 1)
-        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, nelem, st);
+        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, nelem, after_main, st);
     else
-        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, nelem, st);
+        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, nelem, after_main, st);
 }
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -834,15 +838,16 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     const bool mixed = c.precision == PREC_MIXED;
     if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[2], st));
     const int nelem = (plan.otab && c.solve_variant != 1) ? (int)h.otab.size() : 0;
+    hipEvent_t after_main = (timing && timing->valid) ? timing->ev[3] : nullptr;
     if (K <= 8)
-        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, nelem, st);
+        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, nelem, after_main, st);
     else if (K <= 16)
-        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, nelem, st);
+        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, nelem, after_main, st);
     else if (K <= 24)
-        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, nelem, st);
+        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, nelem, after_main, st);
     else
-        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, nelem, st);
-    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[3], st));
+        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, nelem, after_main, st);
+    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[4], st));
     HPGLX_CHECK(hipGetLastError());
     return 0;
 }

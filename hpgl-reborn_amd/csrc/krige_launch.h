@@ -63,8 +63,11 @@ struct GridCall {
     bool profile = false;
 };
 
+// Events of one profiled call: start, after search, before solve, after the
+// main solve kernel, end.
 struct CallTiming {
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // start, after search, before solve, end
+    static constexpr int N = 5;
+    hipEvent_t ev[N] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool valid = false;
 };
 

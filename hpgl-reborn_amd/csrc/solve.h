@@ -287,17 +287,19 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         } else {
             tcol = gbc<G>(tr[c / G], c % G);
         }
+        // Every entry that is read has c <= r, so row liveness is enough;
+        // dead rows and columns have offset 0, so every address is in range.
 #pragma unroll
         for (int s = c / G; s < S; ++s) {
             const int r = G * s + t;
-            const bool live = r < n && c < n;
+            const bool live = r < n;
             T v;
             if (LTAB)
                 v = (T)lcov(s, cx, cy, cz);
             else if (MIXED)
-                v = (T)a.ctab32[live ? center + tr[s] - tcol : center];
+                v = (T)a.ctab32[center + tr[s] - tcol];
             else
-                v = (T)a.ctab[live ? center + tr[s] - tcol : center];
+                v = (T)a.ctab[center + tr[s] - tcol];
             L[slot_off<G>(s) + c] = live ? v : ((r == c) ? T(1) : T(0));
         }
     }
@@ -340,59 +342,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 #pragma unroll
         for (int s = 0; s < S; ++s) ov[s] = (double)z[s];
     }
-    if (MIXED) {
-        for (int it = 0; it < a.refine_steps; ++it) {
-            // Residuals b - A s (and 1 - A o) in FP64 with A from the FP64
-            // table; the current solutions reach every lane by broadcast.
-            double rs_[S], ro_[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                rs_[s] = bq[s];
-                ro_[s] = (G * s + t < n) ? 1.0 : 0.0;
-            }
-#pragma unroll
-            for (int c = 0; c < NMAX; ++c) {
-                int tcol = 0;
-                unsigned cx = 0, cy = 0, cz = 0;
-                if (LTAB) {
-                    cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
-                    cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
-                    cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
-                } else {
-                    tcol = gbc<G>(tr[c / G], c % G);
-                }
-                const double xs = gbc<G>(sv[c / G], c % G);
-                const double xo = OKK ? gbc<G>(ov[c / G], c % G) : 0.0;
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const bool live = G * s + t < n && c < n;
-                    const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[live ? center + tr[s] - tcol : center];
-                    if (live) {
-                        rs_[s] -= av * xs;
-                        if (OKK) ro_[s] -= av * xo;
-                    }
-                }
-            }
-            {
-                T z[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) z[s] = (T)rs_[s];
-                CH::forward(L, D, z, t);
-                CH::backward(L, D, z, t);
-#pragma unroll
-                for (int s = 0; s < S; ++s) sv[s] += (double)z[s];
-            }
-            if (OKK) {
-                T z[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) z[s] = (T)ro_[s];
-                CH::forward(L, D, z, t);
-                CH::backward(L, D, z, t);
-#pragma unroll
-                for (int s = 0; s < S; ++s) ov[s] += (double)z[s];
-            }
-        }
-    }
+    // Weights from the (possibly float) solutions.
     if (OKK) {
         double ps = 0.0, po = 0.0;
 #pragma unroll
@@ -408,9 +358,97 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         mu = (sum_sk - 1) / sum_ones;
 #pragma unroll
         for (int s = 0; s < S; ++s) w[s] = sv[s] - mu * ov[s];
+        if (MIXED) {
+            // Refinement on the bordered system [A 1; 1' 0] [w; mu] = [b; 1]:
+            // r = b - A w - mu, r_mu = 1 - sum w in FP64; A z = r with the
+            // float factor; dmu = (sum z - r_mu) / sum o, dw = z - dmu o.
+            for (int it = 0; it < a.refine_steps; ++it) {
+                double r[S];
+                double pw = 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    r[s] = bq[s] - mu;
+                    pw += w[s];
+                }
+                const double r_mu = 1.0 - gsum<G>(pw);
+#pragma unroll
+                for (int c = 0; c < NMAX; ++c) {
+                    int tcol = 0;
+                    unsigned cx = 0, cy = 0, cz = 0;
+                    if (LTAB) {
+                        cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
+                        cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+                        cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
+                    } else {
+                        tcol = gbc<G>(tr[c / G], c % G);
+                    }
+                    const double xw = gbc<G>(w[c / G], c % G);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        // dead columns carry w = 0; dead rows are cleared below
+                        const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[center + tr[s] - tcol];
+                        r[s] = __builtin_fma(-av, xw, r[s]);
+                    }
+                    // bound the loads the scheduler keeps in flight
+                    if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+                T z[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) z[s] = (T)r[s];
+                CH::forward(L, D, z, t);
+                CH::backward(L, D, z, t);
+                double pz = 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) pz += (double)z[s];
+                const double dmu = (gsum<G>(pz) - r_mu) / sum_ones;
+#pragma unroll
+                for (int s = 0; s < S; ++s) w[s] += (double)z[s] - dmu * ov[s];
+                mu += dmu;
+            }
+        }
     } else {
 #pragma unroll
         for (int s = 0; s < S; ++s) w[s] = sv[s];
+        if (MIXED) {
+            // r = b - A w in FP64, A z = r with the float factor, w += z.
+            for (int it = 0; it < a.refine_steps; ++it) {
+                double r[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) r[s] = bq[s];
+#pragma unroll
+                for (int c = 0; c < NMAX; ++c) {
+                    int tcol = 0;
+                    unsigned cx = 0, cy = 0, cz = 0;
+                    if (LTAB) {
+                        cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
+                        cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+                        cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
+                    } else {
+                        tcol = gbc<G>(tr[c / G], c % G);
+                    }
+                    const double xw = gbc<G>(w[c / G], c % G);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        // dead columns carry w = 0; dead rows are cleared below
+                        const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[center + tr[s] - tcol];
+                        r[s] = __builtin_fma(-av, xw, r[s]);
+                    }
+                    // bound the loads the scheduler keeps in flight
+                    if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+                T z[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) z[s] = (T)r[s];
+                CH::forward(L, D, z, t);
+                CH::backward(L, D, z, t);
+#pragma unroll
+                for (int s = 0; s < S; ++s) w[s] += (double)z[s];
+            }
+        }
     }
     return true;
 }

@@ -190,6 +190,18 @@ def last_timing():
     return tuple(float(v) for v in ms)
 
 
+def profile_summary():
+    """Mean milliseconds over the calls profiled since set_profiling(True):
+    {"search", "solve_main", "solve", "call"} (HIP events on each call's
+    stream) and the number of calls."""
+    ms = (C.c_float * 4)()
+    nc = C.c_int(0)
+    if _hpgl_so.hpglx_profile_summary(ms, C.byref(nc)) != 0:
+        raise RuntimeError(_hpgl_so.hpglx_last_error().decode())
+    return {"search": float(ms[0]), "solve_main": float(ms[1]), "solve": float(ms[2]), "call": float(ms[3]),
+            "calls": nc.value}
+
+
 def last_call_stats():
     fb = C.c_uint(0)
     cl = C.c_int(0)

@@ -126,6 +126,8 @@ _hpgl_so.hpglx_set_profiling.restype = None
 _hpgl_so.hpglx_set_profiling.argtypes = [C.c_int]
 _hpgl_so.hpglx_last_timing.restype = C.c_int
 _hpgl_so.hpglx_last_timing.argtypes = [C.POINTER(C.c_float)]
+_hpgl_so.hpglx_profile_summary.restype = C.c_int
+_hpgl_so.hpglx_profile_summary.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int)]
 _hpgl_so.hpglx_last_call_stats.restype = C.c_int
 _hpgl_so.hpglx_last_call_stats.argtypes = [C.POINTER(C.c_uint), C.POINTER(C.c_int)]
 _hpgl_so.hpglx_set_default_precision.restype = None
