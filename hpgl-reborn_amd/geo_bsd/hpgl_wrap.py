@@ -1,8 +1,8 @@
 """ctypes binding of libhpgl.so (the MI355X kriging library).
 
-Mirrors the reference binding (src/geo_bsd/hpgl_wrap.py:40-455): the same C
-struct layouts, and the library is loaded from this package directory only
-(hpgl_wrap.py:187-290).  The library is the HIP build; there is no CPU
+Mirrors the reference binding (src/geo_bsd/hpgl_wrap.py): the same C struct
+layouts (hpgl_wrap.py:44-135), and the library is loaded from this package
+directory only (hpgl_wrap.py:170-273).  The library is the HIP build; there is no CPU
 fallback, and importing this module fails loudly when it is missing.
 """
 import ctypes as C
