@@ -27,6 +27,7 @@
 #include "krige_launch.h"
 #include "solve.h"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -144,42 +145,83 @@ __global__ void k_tdelta(GridDims g, int tsize, int tpad, const int* tdx, const 
 }
 
 // --------------------------------------------------------- tiled template walk
-// One block per TILE_X x TILE_Y x 1 tile of cells.  The block first copies the
-// padded-mask box its cells can reach (tile + 2r halo) into LDS; the template
-// then streams through LDS in chunks, so every probe is one broadcast LDS read
-// of the template entry plus one LDS byte read of the mask.  Only the rare
-// hits write to global memory.  Cells whose 27 clusters are all below the
-// limit are handed to the cluster path instead.
+// One block per TILE_X x TILE_Y tile of one z slice, one lane per cell.  The
+// block copies the padded-mask box its cells can reach (tile + 2r halo) and
+// the packed offsets of the first SEARCH_HEAD template entries into LDS.
+// Every lane then walks the template in reference order, 32 entries per step:
+// the step's box deltas are wave-uniform and come through scalar loads, each
+// probe is one LDS byte read, and the 32 results form a bit mask whose set
+// bits (ctz order) are the hits until max_neighbours.  Slices are dispatched
+// from both grid faces inwards, because cells near a z face need the most
+// probes and would otherwise finish last.  Cells whose 27 clusters are all
+// below the limit are handed to the cluster path instead.
 constexpr int TILE_X = 16, TILE_Y = 16;
-constexpr int TMPL_U = 32;  // probes per iteration; template arrays are padded to a multiple
+constexpr int TMPL_U = 32;       // probes per step; template arrays are padded to a multiple (plus one more)
+constexpr int LOAD_BATCH = 8;    // loads in flight per thread while filling LDS
+constexpr int SEARCH_HEAD = 1024;  // packed offsets staged in LDS
+constexpr size_t kSearchLdsMax = 160 * 1024;  // one workgroup's LDS
 
-__global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int chunk) {
+typedef const int __attribute__((address_space(4)))* const_int_ptr;
+
+// head = number of staged packed offsets (<= SEARCH_HEAD).
+__global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int head) {
     extern __shared__ unsigned int lds_dw[];
-    const int qx = a.lx >> 2;
-    const int ndw = qx * a.ly * a.lz;
-    unsigned char* box = (unsigned char*)lds_dw;
-    int* tdel = (int*)(lds_dw + ((ndw + 3) & ~3));
-    int* tpk = tdel + chunk;
     const GridDims& g = a.g;
+    const int qx = a.lx >> 2;  // dwords per box row
+    const int nrows = a.ly * a.lz;
+    const unsigned char* box = (const unsigned char*)lds_dw;
+    int* tpk = (int*)(lds_dw + ((qx * nrows + 3) & ~3));
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = blockIdx.z;
+    const int nzb = (int)gridDim.z;
+    const int bz = (blockIdx.z & 1) ? nzb - 1 - (int)(blockIdx.z >> 1) : (int)(blockIdx.z >> 1);
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = bz;
+
     // Box origin is padded (x0, y0, z0) = cell (x0 - rx, y0 - ry, z0 - rz).
-    for (int i = tid; i < ndw; i += blockDim.x) {
-        const int xq = i % qx, rest = i / qx;
-        const int yy = rest % a.ly, zz = rest / a.ly;
-        const int px = x0 + 4 * xq, py = y0 + yy, pz = z0 + zz;
-        unsigned int v = 0;
-        if (px < g.px && py < g.py && pz < g.pz)
-            v = *(const unsigned int*)(a.pmask + ((long long)pz * g.py + py) * g.px + px);
-        lds_dw[i] = v;
+    // Thread -> (dword column xq, row); rows advance by rstep per pass.
+    {
+        const int rstep = blockDim.x / qx;  // qx <= 256 by the launcher
+        const int xq = tid % qx;
+        int row = tid / qx;
+        const bool lane_ok = row < rstep;
+        int yy = row % a.ly, zz = row / a.ly;
+        const int px = x0 + 4 * xq;
+        while (lane_ok && row < nrows) {
+            unsigned v[LOAD_BATCH];
+            int rr[LOAD_BATCH];
+#pragma unroll
+            for (int k = 0; k < LOAD_BATCH; ++k) {
+                const int py = y0 + yy, pz = z0 + zz;
+                const bool ok = row < nrows && px < g.px && py < g.py && pz < g.pz;
+                // unconditional load (from offset 0 when out of range), so the
+                // batch issues without branches
+                const long long off = ok ? ((long long)pz * g.py + py) * g.px + px : 0;
+                const unsigned w = *(const unsigned int*)(a.pmask + off);
+                v[k] = ok ? w : 0u;
+                rr[k] = row < nrows ? row : -1;
+                row += rstep;
+                yy += rstep;
+                while (yy >= a.ly) {
+                    yy -= a.ly;
+                    ++zz;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < LOAD_BATCH; ++k)
+                if (rr[k] >= 0) lds_dw[rr[k] * qx + xq] = v[k];
+        }
     }
+    head = min(head, a.tsize);
+    for (int i = tid; i < head; i += blockDim.x) tpk[i] = a.tpack[i];
+    __syncthreads();
+
     const int tx = tid % TILE_X, ty = tid / TILE_X;
     const int x = x0 + tx, y = y0 + ty, z = z0;
     const bool inside = x < g.nx && y < g.ny;
     const long long cell = inside ? ((long long)z * g.ny + y) * g.nx + x : 0;
+    const int lcell = (g.rz * a.ly + ty + g.ry) * a.lx + tx + g.rx;  // the cell's own box byte
     bool active = false;
     if (inside) {
-        const bool needs = a.kind == 2 || !a.mask[cell];
+        const bool needs = a.kind == 2 || !box[lcell];
         if (!needs) {
             a.count[cell] = CNT_INFORMED;
         } else if (a.cl_nbhd_exceeded[cluster_of(g, x, y, z)]) {
@@ -192,36 +234,33 @@ __global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int chunk) {
     }
     // Inactive lanes probe around the box position of tile cell (0, 0), which
     // keeps every template delta inside the box; their bits are discarded.
-    const int lbase = (g.rz * a.ly + (active ? ty : 0) + g.ry) * a.lx + (active ? tx : 0) + g.rx;
+    const int lbase = active ? lcell : (g.rz * a.ly + g.ry) * a.lx + g.rx;
+    const const_int_ptr tdel = (const_int_ptr)a.tdelta_lds;
     int cnt = 0;
     const int K = a.kmax;
     int* out = a.nboff + (active ? cell * K : 0);
-    for (int c0 = 0; c0 < a.tsize; c0 += chunk) {
-        if (!__syncthreads_or(active && cnt < K)) break;
-        const int clen = min(chunk, a.tsize - c0);
-        const int cpad = (clen + TMPL_U - 1) / TMPL_U * TMPL_U;
-        for (int i = tid; i < cpad; i += blockDim.x) {
-            tdel[i] = a.tdelta_lds[c0 + i];
-            tpk[i] = a.tpack[c0 + i];
-        }
-        __syncthreads();
-        for (int t0 = 0; t0 < clen; t0 += TMPL_U) {
-            const bool want = active && cnt < K;
-            if (!__any(want)) break;
-            // 32 probes into a bit mask (mask bytes are 0 or 1), no branches.
-            unsigned bits = 0;
+    for (int t0 = 0; t0 < a.tsize; t0 += TMPL_U) {
+        const bool want = active && cnt < K;
+        if (!__any(want)) break;
+        const int tu = __builtin_amdgcn_readfirstlane(t0);
+        // 32 probes into a bit mask (mask bytes are 0 or 1), no branches.
+        unsigned bits = 0;
 #pragma unroll
-            for (int u = 0; u < TMPL_U; ++u) {
-                const unsigned m = box[lbase + tdel[t0 + u]];
-                bits |= m << u;
-            }
-            const int left = clen - t0;
-            if (left < TMPL_U) bits &= (1u << left) - 1u;
-            if (!want) bits = 0;
-            // Hits in template order until max_neighbours.
+        for (int u = 0; u < TMPL_U; ++u) bits |= (unsigned)box[lbase + tdel[tu + u]] << u;
+        const int left = a.tsize - t0;
+        if (left < TMPL_U) bits &= (1u << left) - 1u;
+        if (!want) bits = 0;
+        // Hits in template order until max_neighbours.
+        if (t0 + TMPL_U <= head) {
             while (bits != 0 && cnt < K) {
                 const int u = __builtin_ctz(bits);
                 out[cnt++] = tpk[t0 + u];
+                bits &= bits - 1u;
+            }
+        } else {
+            while (bits != 0 && cnt < K) {
+                const int u = __builtin_ctz(bits);
+                out[cnt++] = a.tpack[t0 + u];
                 bits &= bits - 1u;
             }
         }
@@ -626,6 +665,21 @@ void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, int 
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Raises a kernel's dynamic-LDS limit (once per device and size).
+hipError_t allow_lds(const void* fn, int bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> done;  // (kernel, device) -> bytes allowed
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    int& have = done[{fn, dev}];
+    if (have >= bytes) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess) have = bytes;
+    return e;
+}
+
 }  // namespace
 
 int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st, CallTiming* timing,
@@ -677,13 +731,11 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     const int tpad = (T + TMPL_U - 1) / TMPL_U * TMPL_U + TMPL_U;
     auto* tdelta = (int*)ws.get("tdelta", (size_t)tpad * 4, err);
     auto* tdelta_lds = (int*)ws.get("tdelta_lds", (size_t)tpad * 4, err);
+    // Tiled search: LDS = mask box (tile + 2r halo) + the template's first
+    // SEARCH_HEAD packed offsets.
     const int lx = (TILE_X + 2 * g.rx + 3) / 4 * 4, ly = TILE_Y + 2 * g.ry, lz = 1 + 2 * g.rz;
-    const size_t box_bytes = ((size_t)lx * ly * lz + 15) / 16 * 16;
-    // LDS of a tile block: the mask box, then a chunk of the template (delta + pack).
-    const size_t lds_budget = 65536;
-    int chunk = box_bytes < lds_budget ? (int)((lds_budget - box_bytes) / 8) : 0;
-    chunk = chunk > 4096 ? 4096 : chunk / TMPL_U * TMPL_U;
-    const bool tiled = chunk >= 256 && c.search_variant != 1;
+    const size_t tile_lds = ((size_t)lx * ly * lz + 15) / 16 * 16 + (size_t)SEARCH_HEAD * 4;
+    const bool tiled = tile_lds <= kSearchLdsMax && lx / 4 <= 256 && c.search_variant != 1;
     auto* tpack = (int*)ws.get("tpack", (size_t)tpad * 4, err);
     auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
     auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
@@ -754,9 +806,10 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     sa.cl_cells = cl_cells;
     sa.n_cl_cells = n_cl_cells;
     if (tiled) {
+        if (tile_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_tile, (int)tile_lds));
         const dim3 grid((c.nx + TILE_X - 1) / TILE_X, (c.ny + TILE_Y - 1) / TILE_Y, c.nz);
-        hipLaunchKernelGGL(k_search_tile, grid, dim3(TILE_X * TILE_Y), box_bytes + (size_t)chunk * 8, st, sa,
-                           chunk);
+        const int head = c.search_head > 0 && c.search_head < SEARCH_HEAD ? c.search_head : SEARCH_HEAD;
+        hipLaunchKernelGGL(k_search_tile, grid, dim3(TILE_X * TILE_Y), tile_lds, st, sa, head);
     } else {
         hipLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, sa);
     }

@@ -60,6 +60,7 @@ struct GridCall {
     int refine_steps = 1;
     int search_variant = 0;                 // 0 = LDS tiles when the box fits, 1 = global walk
     int solve_variant = 0;                  // 0 = LDS covariance table when it fits, 1 = global table
+    int search_head = 0;                    // packed offsets staged in LDS by the search (0 = default)
     bool profile = false;
 };
 
