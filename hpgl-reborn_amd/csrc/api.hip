@@ -212,13 +212,11 @@ CallTiming* next_timing(Library& L, std::string& err) {
     return &L.tpool[slot];
 }
 
-// Milliseconds {search, main solve kernel, whole solve, whole call} of one call.
-bool read_timing(const CallTiming& t, float ms[4]) {
+// Milliseconds {search kernel, main solve kernel} of one call.
+bool read_timing(const CallTiming& t, float ms[2]) {
     if (hipEventSynchronize(t.ev[CallTiming::N - 1]) != hipSuccess) return false;
     return hipEventElapsedTime(&ms[0], t.ev[0], t.ev[1]) == hipSuccess &&
-           hipEventElapsedTime(&ms[1], t.ev[2], t.ev[3]) == hipSuccess &&
-           hipEventElapsedTime(&ms[2], t.ev[2], t.ev[4]) == hipSuccess &&
-           hipEventElapsedTime(&ms[3], t.ev[0], t.ev[4]) == hipSuccess;
+           hipEventElapsedTime(&ms[1], t.ev[2], t.ev[3]) == hipSuccess;
 }
 
 int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err) {
@@ -686,32 +684,31 @@ void hpglx_set_profiling(int on) {
     if (L.profiling) L.tcalls = 0;
 }
 
-int hpglx_last_timing(float* ms3) {
+int hpglx_last_timing(float* ms2) {
     Library& L = lib();
     std::lock_guard<std::mutex> g(L.mu);
     L.last_error.clear();
     if (L.tcalls == 0) return fail(L, "no profiled call");
-    float ms[4];
+    float ms[2];
     if (!read_timing(L.tpool[(L.tcalls - 1) % kTimingPool], ms)) return fail(L, "reading the call's events failed");
-    ms3[0] = ms[0];
-    ms3[1] = ms[2];
-    ms3[2] = ms[3];
+    ms2[0] = ms[0];
+    ms2[1] = ms[1];
     return 0;
 }
 
-int hpglx_profile_summary(float* ms4, int* ncalls) {
+int hpglx_profile_summary(float* ms2, int* ncalls) {
     Library& L = lib();
     std::lock_guard<std::mutex> g(L.mu);
     L.last_error.clear();
     const size_t n = L.tcalls < kTimingPool ? L.tcalls : kTimingPool;
     if (n == 0) return fail(L, "no profiled call");
-    double sum[4] = {0, 0, 0, 0};
+    double sum[2] = {0, 0};
     for (size_t i = 0; i < n; ++i) {
-        float ms[4];
+        float ms[2];
         if (!read_timing(L.tpool[i], ms)) return fail(L, "reading the call's events failed");
-        for (int k = 0; k < 4; ++k) sum[k] += ms[k];
+        for (int k = 0; k < 2; ++k) sum[k] += ms[k];
     }
-    for (int k = 0; k < 4; ++k) ms4[k] = (float)(sum[k] / (double)n);
+    for (int k = 0; k < 2; ++k) ms2[k] = (float)(sum[k] / (double)n);
     if (ncalls) *ncalls = (int)n;
     return 0;
 }

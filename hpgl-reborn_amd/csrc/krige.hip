@@ -22,6 +22,8 @@
 //                   neighbour order for OK/SK, double for median IK)
 //   failure policy  src/geo_bsd/hpgl/cont_kriging.h:73-105, median_ik.cpp:60-78
 //   order relations src/geo_bsd/hpgl/indicator_kriging.h:34-59
+#include <hip/hip_ext.h>
+
 #include "introsort.h"
 #include "krige_kernels.h"
 #include "krige_launch.h"
@@ -60,16 +62,55 @@ __device__ __forceinline__ int cluster_of(const GridDims& g, int x, int y, int z
     return ((z / g.rz) * g.cy + (y / g.ry)) * g.cx + (x / g.rx);
 }
 
-// ------------------------------------------------------------------ prepare
-__global__ void k_prepare(GridDims g, const unsigned char* mask, unsigned char* pmask, int* cl_count) {
-    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= g.n) return;
-    if (!mask[cell]) return;
-    int x, y, z;
-    cell_xyz(g, cell, x, y, z);
-    const long long p = ((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
-    pmask[p] = 1;
-    atomicAdd(&cl_count[cluster_of(g, x, y, z)], 1);
+// ------------------------------------------------------------ cluster build
+// One block (256 lanes) per cluster (clusterizer.cpp:14-189: cluster = cell
+// coordinate / radius, the grid split into boxes of r cells).  The block
+// writes its cells into the padded mask, counts its informed cells and lists
+// them, in ascending cell order, at cl_nodes[ci * cap] (at most cap = limit
+// entries, which holds every cluster within the limit).  Lists of clusters
+// within the limit are exactly what the cluster path reads; those of exceeded
+// clusters are never read.  Block 0 also clears the call's counters.
+__global__ __launch_bounds__(256) void k_cluster_build(GridDims g, const unsigned char* mask, unsigned char* pmask,
+                                                       int* cl_count, int* cl_nodes, int cap, int* counters,
+                                                       int ncounters) {
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int ci = blockIdx.x;
+    if (ci == 0 && tid < ncounters) counters[tid] = 0;
+    const int kx = ci % g.cx, ky = (ci / g.cx) % g.cy, kz = ci / (g.cx * g.cy);
+    const int xa = kx * g.rx, ya = ky * g.ry, za = kz * g.rz;
+    const int wx = max(0, min(g.nx, xa + g.rx) - xa), wy = max(0, min(g.ny, ya + g.ry) - ya),
+              wz = max(0, min(g.nz, za + g.rz) - za);
+    const int ncell = wx * wy * wz;
+    int total = 0;
+    for (int i0 = 0; i0 < ncell; i0 += 256) {
+        const int i = i0 + tid;
+        bool inf = false;
+        long long cell = 0;
+        if (i < ncell) {
+            const int lx = i % wx, r = i / wx, ly = r % wy, lz = r / wy;
+            const int x = xa + lx, y = ya + ly, z = za + lz;
+            cell = ((long long)z * g.ny + y) * g.nx + x;
+            inf = mask[cell] != 0;
+            pmask[((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx)] = inf ? 1 : 0;
+        }
+        // Box order (z, y, x) is ascending cell order, so ranks by a
+        // block-wide prefix count give the ascending list.
+        const unsigned long long b = __ballot(inf);
+        if (lane == 0) wsum[w] = __popcll(b);
+        __syncthreads();
+        int before = 0, chunk = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            before += k < w ? wsum[k] : 0;
+            chunk += wsum[k];
+        }
+        const int pos = total + before + __popcll(b & ((1ull << lane) - 1ull));
+        if (inf && pos < cap) cl_nodes[(long long)ci * cap + pos] = (int)cell;
+        total += chunk;
+        __syncthreads();
+    }
+    if (tid == 0) cl_count[ci] = total;
 }
 
 // Mean of informed values in double (calc_mean.cpp:6-30); per-block partial
@@ -321,70 +362,6 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
     if (active) a.count[cell] = cnt;
 }
 
-// --------------------------------------------------------------- clusters
-// Exclusive scan of the node counts of non-exceeded clusters (one block).
-__global__ void k_cluster_scan(int ncl, const int* cl_count, long long limit, int* cl_start, int* total,
-                               const int* n_cl_cells) {
-    __shared__ int part[1024];
-    if (*n_cl_cells == 0) return;
-    const int tid = threadIdx.x;
-    const int per = (ncl + 1023) / 1024;
-    const int b = tid * per, e = min(ncl, b + per);
-    int s = 0;
-    for (int i = b; i < e; ++i) s += ((long long)cl_count[i] > limit) ? 0 : cl_count[i];
-    part[tid] = s;
-    __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int i = 0; i < 1024; ++i) {
-            int v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-        *total = acc;
-    }
-    __syncthreads();
-    int acc = part[tid];
-    for (int i = b; i < e; ++i) {
-        cl_start[i] = acc;
-        acc += ((long long)cl_count[i] > limit) ? 0 : cl_count[i];
-    }
-}
-
-__global__ void k_cluster_fill(GridDims g, const unsigned char* mask, const int* cl_count, long long limit,
-                               const int* cl_start, int* cl_cursor, int* cl_nodes, const int* n_cl_cells) {
-    if (*n_cl_cells == 0) return;
-    long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= g.n || !mask[cell]) return;
-    int x, y, z;
-    cell_xyz(g, cell, x, y, z);
-    const int ci = cluster_of(g, x, y, z);
-    if ((long long)cl_count[ci] > limit) return;
-    const int slot = atomicAdd(&cl_cursor[ci], 1);
-    cl_nodes[cl_start[ci] + slot] = (int)cell;
-}
-
-// Nodes were added to the reference clusters in ascending node order
-// (cont_kriging.h:63-69); sort each list back into that order.
-__global__ void k_cluster_sort(int ncl, const int* cl_count, long long limit, const int* cl_start, int* cl_nodes,
-                               const int* n_cl_cells) {
-    if (*n_cl_cells == 0) return;
-    int ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= ncl) return;
-    const int c = cl_count[ci];
-    if ((long long)c > limit || c < 2) return;
-    int* v = cl_nodes + cl_start[ci];
-    for (int i = 1; i < c; ++i) {
-        const int x = v[i];
-        int j = i - 1;
-        while (j >= 0 && v[j] > x) {
-            v[j + 1] = v[j];
-            --j;
-        }
-        v[j + 1] = x;
-    }
-}
-
 // Scratch store for the introsort: element i of lane l lives at i*nlanes + l.
 struct LaneStore {
     struct Item {
@@ -436,7 +413,7 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
                     if (i < 0 || j < 0 || k < 0 || i >= g.cx || j >= g.cy || k >= g.cz) continue;
                     const int ci = (k * g.cy + j) * g.cx + i;
                     const int c = a.cl_count[ci];
-                    const int* nodes = a.cl_nodes + a.cl_start[ci];
+                    const int* nodes = a.cl_nodes + (long long)ci * a.node_cap;
                     for (int s = 0; s < c; ++s) {
                         const int nd = nodes[s];
                         int nx_, ny_, nz_;
@@ -504,6 +481,14 @@ bool DevPlan::upload(std::string& err) {
         release();
         return false;
     }
+    tpad = (tsize + TMPL_U - 1) / TMPL_U * TMPL_U + TMPL_U;
+    if (hipMalloc(&tdelta, (size_t)tpad * 4) != hipSuccess || hipMalloc(&tdelta_lds, (size_t)tpad * 4) != hipSuccess ||
+        hipMalloc(&tpack, (size_t)tpad * 4) != hipSuccess) {
+        err = "hipMalloc failed for the template deltas";
+        release();
+        return false;
+    }
+    tkey[0] = tkey[1] = tkey[2] = tkey[3] = -1;
     if (h.octant && !h.otab.empty()) {
         if (hipMalloc(&otab, h.otab.size() * sizeof(double)) != hipSuccess ||
             hipMemcpy(otab, h.otab.data(), h.otab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
@@ -532,6 +517,10 @@ void DevPlan::release() {
     if (tdz) (void)hipFree(tdz);
     if (otab) (void)hipFree(otab);
     otab = nullptr;
+    if (tdelta) (void)hipFree(tdelta);
+    if (tdelta_lds) (void)hipFree(tdelta_lds);
+    if (tpack) (void)hipFree(tpack);
+    tdelta = tdelta_lds = tpack = nullptr;
     ctab = nullptr;
     ctab32 = nullptr;
     tdx = tdy = tdz = nullptr;
@@ -554,21 +543,23 @@ void* Workspace::get(const std::string& name, size_t bytes, std::string& err) {
         return nullptr;
     }
     bufs[name] = std::make_pair(p, bytes);
+    tags.erase(name);
     return p;
 }
 
 void Workspace::release() {
     for (auto& kv : bufs) (void)hipFree(kv.second.first);
     bufs.clear();
+    tags.clear();
 }
 
 namespace {
 
 template <int NMAX, int G, bool MIXED, int KIND>
-void launch_solve(const SolveArgs& sa, long long ncells, hipStream_t st) {
+void launch_solve(const SolveArgs& sa, long long ncells, const hipEvent_t* ev, hipStream_t st) {
     const int block = 256;
     const unsigned grid = (unsigned)((ncells * G + block - 1) / block);
-    hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3(grid), dim3(block), 0, st, sa);
+    hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3(grid), dim3(block), 0, st, ev[0], ev[1], 0, sa);
 }
 
 // Global-table form over a device-sized cell list (cluster-path cells, and
@@ -585,7 +576,7 @@ void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st)
 // Persistent LDS-table form: as many blocks as are resident at once.  Returns
 // false (nothing launched) when the table does not fit a block's LDS.
 template <int NMAX, int G, bool MIXED, int KIND, int BLOCK>
-bool launch_solve_lds(const SolveArgs& sa, int nelem, hipStream_t st) {
+bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipStream_t st) {
     static std::mutex mu;
     static std::map<std::pair<int, int>, int> grids;  // (device, LDS bytes) -> grid
     const void* fn = (const void*)&k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>;
@@ -610,8 +601,8 @@ bool launch_solve_lds(const SolveArgs& sa, int nelem, hipStream_t st) {
         }
     }
     if (grid <= 0) return false;
-    hipLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK), (size_t)lds, st,
-                       sa, nelem);
+    hipExtLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK),
+                          (uint32_t)lds, st, ev[0], ev[1], 0, sa, nelem);
     return true;
 }
 
@@ -620,19 +611,18 @@ bool launch_solve_lds(const SolveArgs& sa, int nelem, hipStream_t st) {
 // over every cell.  The mixed form then re-solves its fallbacks in FP64.
 // G64 / G32: lanes per cell of the FP64 and of the mixed form.
 template <int NMAX, int G64, int G32, int KIND>
-void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipEvent_t after_main, hipStream_t st) {
+void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev, hipStream_t st) {
     constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
     bool lds = false;
     if (nelem > 0)
-        lds = mixed ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32>(sa, nelem, st)
-                    : launch_solve_lds<NMAX, G64, false, KIND, 256>(sa, nelem, st);
+        lds = mixed ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32>(sa, nelem, ev, st)
+                    : launch_solve_lds<NMAX, G64, false, KIND, 256>(sa, nelem, ev, st);
     if (!lds) {
         if (mixed)
-            launch_solve<NMAX, G32, true, KIND>(sa, n, st);
+            launch_solve<NMAX, G32, true, KIND>(sa, n, ev, st);
         else
-            launch_solve<NMAX, G64, false, KIND>(sa, n, st);
+            launch_solve<NMAX, G64, false, KIND>(sa, n, ev, st);
     }
-    if (after_main) (void)hipEventRecord(after_main, st);
     if (lds) {
         SolveArgs cl = sa;
         cl.cell_list = sa.cl_cells;
@@ -653,14 +643,14 @@ void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, hipE
 }
 
 template <int NMAX, int G64, int G32>
-void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, hipEvent_t after_main,
+void dispatch_solve(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
                     hipStream_t st) {
     if (kind == 0)
-        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, nelem, after_main, st);
+        dispatch_kind<NMAX, G64, G32, 0>(sa, mixed, n, nelem, ev, st);
     else if (kind == 1)
-        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, nelem, after_main, st);
+        dispatch_kind<NMAX, G64, G32, 1>(sa, mixed, n, nelem, ev, st);
     else
-        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, nelem, after_main, st);
+        dispatch_kind<NMAX, G64, G32, 2>(sa, mixed, n, nelem, ev, st);
 }
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -725,22 +715,18 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
 
     auto* pmask = (unsigned char*)ws.get("pmask", (size_t)P, err);
     auto* cl_count = (int*)ws.get("cl_count", (size_t)ncl * 4, err);
-    auto* cl_start = (int*)ws.get("cl_start", (size_t)ncl * 4, err);
-    auto* cl_cursor = (int*)ws.get("cl_cursor", (size_t)ncl * 4, err);
     auto* nbhd = (unsigned char*)ws.get("cl_nbhd", (size_t)ncl, err);
-    const int tpad = (T + TMPL_U - 1) / TMPL_U * TMPL_U + TMPL_U;
-    auto* tdelta = (int*)ws.get("tdelta", (size_t)tpad * 4, err);
-    auto* tdelta_lds = (int*)ws.get("tdelta_lds", (size_t)tpad * 4, err);
+    // Node lists: a fixed slot of `limit` entries per cluster.
+    const int node_cap = (int)std::max<long long>(1, std::min<long long>(h.limit, n));
+    auto* cl_nodes = (int*)ws.get("cl_nodes", (size_t)ncl * node_cap * 4, err);
     // Tiled search: LDS = mask box (tile + 2r halo) + the template's first
     // SEARCH_HEAD packed offsets.
     const int lx = (TILE_X + 2 * g.rx + 3) / 4 * 4, ly = TILE_Y + 2 * g.ry, lz = 1 + 2 * g.rz;
     const size_t tile_lds = ((size_t)lx * ly * lz + 15) / 16 * 16 + (size_t)SEARCH_HEAD * 4;
     const bool tiled = tile_lds <= kSearchLdsMax && lx / 4 <= 256 && c.search_variant != 1;
-    auto* tpack = (int*)ws.get("tpack", (size_t)tpad * 4, err);
     auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
     auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
     auto* cl_cells = (int*)ws.get("cl_cells", (size_t)n * 4, err);
-    auto* cl_nodes = (int*)ws.get("cl_nodes", (size_t)n * 4, err);
     auto* counters = (int*)ws.get("counters", 64, err);
     const int mean_blocks = 1024;
     auto* psum = (double*)ws.get("mean_psum", mean_blocks * 8, err);
@@ -755,27 +741,41 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     auto* soff = (int*)ws.get("soff", (size_t)cap * nlanes * 4, err);
     if (!err.empty()) return -1;
     int* n_cl_cells = counters + 0;
-    int* cl_total = counters + 1;
     int* overflow = counters + 2;
     int* n_fallback = counters + 3;
     auto* fb_cells = (int*)ws.get("fb_cells", (size_t)n * 4, err);
     if (!err.empty()) return -1;
 
-    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[0], st));
-
+    // Profiling: start/stop events of the search and of the main solve
+    // kernel, recorded by the kernel dispatches themselves.
+    hipEvent_t ev_t[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (timing && timing->valid)
+        for (int i = 0; i < 4; ++i) ev_t[i] = timing->ev[i];
     const int B = 256;
-    hipLaunchKernelGGL(k_fill_u8, dim3(std::min<long long>(4096, blocks_for(P, B))), dim3(B), 0, st, pmask, P,
-                       (unsigned char)0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long long>(1024, blocks_for(ncl, B))), dim3(B), 0, st, cl_count,
-                       (long long)ncl, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<long long>(1024, blocks_for(ncl, B))), dim3(B), 0, st, cl_cursor,
-                       (long long)ncl, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, counters, 16LL, 0);
-    hipLaunchKernelGGL(k_prepare, dim3(blocks_for(n, B)), dim3(B), 0, st, g, c.mask, pmask, cl_count);
+    // The padded mask's halo stays zero between calls that share its buffer
+    // and geometry; only then is the whole buffer cleared.
+    const std::string pm_sig = std::to_string((uintptr_t)pmask) + ":" + std::to_string(g.px) + ":" +
+                               std::to_string(g.py) + ":" + std::to_string(g.pz) + ":" + std::to_string(g.rx) + ":" +
+                               std::to_string(g.ry) + ":" + std::to_string(g.rz);
+    if (ws.tags["pmask"] != pm_sig) {
+        hipLaunchKernelGGL(k_fill_u8, dim3(std::min<long long>(4096, blocks_for(P, B))), dim3(B), 0, st, pmask, P,
+                           (unsigned char)0);
+        ws.tags["pmask"] = pm_sig;
+    }
+    hipLaunchKernelGGL(k_cluster_build, dim3(ncl), dim3(256), 0, st, g, c.mask, pmask, cl_count, cl_nodes, node_cap,
+                       counters, 16);
     hipLaunchKernelGGL(k_cluster_flags, dim3(blocks_for(ncl, B)), dim3(B), 0, st, g, (const int*)cl_count,
                        (long long)h.limit, nbhd);
-    hipLaunchKernelGGL(k_tdelta, dim3(blocks_for(tpad, B)), dim3(B), 0, st, g, T, tpad, (const int*)plan.tdx,
-                       (const int*)plan.tdy, (const int*)plan.tdz, tdelta, tdelta_lds, lx, ly, tpack);
+    // Template deltas for this grid's padding and tile box, kept by the plan.
+    if (plan.tkey[0] != g.px || plan.tkey[1] != g.py || plan.tkey[2] != lx || plan.tkey[3] != ly) {
+        hipLaunchKernelGGL(k_tdelta, dim3(blocks_for(plan.tpad, B)), dim3(B), 0, st, g, T, plan.tpad,
+                           (const int*)plan.tdx, (const int*)plan.tdy, (const int*)plan.tdz, plan.tdelta,
+                           plan.tdelta_lds, lx, ly, plan.tpack);
+        plan.tkey[0] = g.px;
+        plan.tkey[1] = g.py;
+        plan.tkey[2] = lx;
+        plan.tkey[3] = ly;
+    }
     if (c.kind == KIND_SK && c.auto_mean) {
         hipLaunchKernelGGL(k_mean_partial, dim3(mean_blocks), dim3(256), 0, st, n, c.values, c.mask, psum, pcnt);
         hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, st, mean_blocks, (const double*)psum,
@@ -793,12 +793,12 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     sa.kmax = K;
     sa.mask = c.mask;
     sa.pmask = pmask;
-    sa.tdelta = tdelta;
-    sa.tdelta_lds = tdelta_lds;
+    sa.tdelta = plan.tdelta;
+    sa.tdelta_lds = plan.tdelta_lds;
     sa.lx = lx;
     sa.ly = ly;
     sa.lz = lz;
-    sa.tpack = tpack;
+    sa.tpack = plan.tpack;
     sa.tsize = T;
     sa.cl_nbhd_exceeded = nbhd;
     sa.count = count;
@@ -809,25 +809,20 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         if (tile_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_tile, (int)tile_lds));
         const dim3 grid((c.nx + TILE_X - 1) / TILE_X, (c.ny + TILE_Y - 1) / TILE_Y, c.nz);
         const int head = c.search_head > 0 && c.search_head < SEARCH_HEAD ? c.search_head : SEARCH_HEAD;
-        hipLaunchKernelGGL(k_search_tile, grid, dim3(TILE_X * TILE_Y), tile_lds, st, sa, head);
+        hipExtLaunchKernelGGL(k_search_tile, grid, dim3(TILE_X * TILE_Y), (uint32_t)tile_lds, st, ev_t[0], ev_t[1],
+                              0, sa, head);
     } else {
-        hipLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, sa);
+        hipExtLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, ev_t[0], ev_t[1], 0, sa);
     }
 
-    hipLaunchKernelGGL(k_cluster_scan, dim3(1), dim3(1024), 0, st, ncl, (const int*)cl_count, (long long)h.limit,
-                       cl_start, cl_total, (const int*)n_cl_cells);
-    hipLaunchKernelGGL(k_cluster_fill, dim3(blocks_for(n, B)), dim3(B), 0, st, g, c.mask, (const int*)cl_count,
-                       (long long)h.limit, (const int*)cl_start, cl_cursor, cl_nodes, (const int*)n_cl_cells);
-    hipLaunchKernelGGL(k_cluster_sort, dim3(blocks_for(ncl, B)), dim3(B), 0, st, ncl, (const int*)cl_count,
-                       (long long)h.limit, (const int*)cl_start, cl_nodes, (const int*)n_cl_cells);
     ClusterSearchArgs ca;
     ca.g = g;
     ca.kmax = K;
     ca.cl_cells = cl_cells;
     ca.n_cl_cells = n_cl_cells;
-    ca.cl_start = cl_start;
     ca.cl_count = cl_count;
     ca.cl_nodes = cl_nodes;
+    ca.node_cap = node_cap;
     ca.limit = h.limit;
     ca.ctab = plan.ctab;
     ca.wx = h.W[0];
@@ -842,10 +837,6 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     ca.nboff = nboff;
     ca.overflow = overflow;
     hipLaunchKernelGGL(k_search_cluster, dim3(nlanes / B), dim3(B), 0, st, ca);
-
-    if (timing && timing->valid) {
-        HPGLX_CHECK(hipEventRecord(timing->ev[1], st));
-    }
 
     SolveArgs so;
     so.g = g;
@@ -889,18 +880,15 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.fallback_cells = fb_cells;
     so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
-    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[2], st));
     const int nelem = (plan.otab && c.solve_variant != 1) ? (int)h.otab.size() : 0;
-    hipEvent_t after_main = (timing && timing->valid) ? timing->ev[3] : nullptr;
     if (K <= 8)
-        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, nelem, after_main, st);
+        dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, nelem, ev_t + 2, st);
     else if (K <= 16)
-        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, nelem, after_main, st);
+        dispatch_solve<16, 4, 4>(so, c.kind, mixed, n, nelem, ev_t + 2, st);
     else if (K <= 24)
-        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, nelem, after_main, st);
+        dispatch_solve<24, 4, 4>(so, c.kind, mixed, n, nelem, ev_t + 2, st);
     else
-        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, nelem, after_main, st);
-    if (timing && timing->valid) HPGLX_CHECK(hipEventRecord(timing->ev[4], st));
+        dispatch_solve<32, 8, 4>(so, c.kind, mixed, n, nelem, ev_t + 2, st);
     HPGLX_CHECK(hipGetLastError());
     return 0;
 }

@@ -57,9 +57,9 @@ struct ClusterSearchArgs {
     int kmax;
     const int* cl_cells;
     const int* n_cl_cells;
-    const int* cl_start;           // per cluster: start in cl_nodes (exclusive scan)
     const int* cl_count;           // per cluster: informed count
-    const int* cl_nodes;           // concatenated sorted node lists (non-exceeded clusters)
+    const int* cl_nodes;           // per cluster: ascending node list at ci * node_cap
+    int node_cap;                  // list capacity per cluster (the limit, at least 1)
     long long limit;
     const double* ctab;            // covariance table over +-W
     int wx, wy, wz;                // table half-widths

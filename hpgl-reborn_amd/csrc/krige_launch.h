@@ -20,6 +20,13 @@ struct DevPlan {
     int* tdz = nullptr;
     double* otab = nullptr;
     int tsize = 0;
+    // Template deltas for one grid geometry (padded pitch and tile box);
+    // tkey = (px, py, lx, ly) they were computed for.
+    int* tdelta = nullptr;
+    int* tdelta_lds = nullptr;
+    int* tpack = nullptr;
+    int tpad = 0;
+    int tkey[4] = {-1, -1, -1, -1};
     bool upload(std::string& err);
     void release();
 };
@@ -27,6 +34,7 @@ struct DevPlan {
 // Grow-only named device buffers reused across calls.
 struct Workspace {
     std::map<std::string, std::pair<void*, size_t>> bufs;
+    std::map<std::string, std::string> tags;  // state kept in a buffer between calls
     void* get(const std::string& name, size_t bytes, std::string& err);
     void release();
 };
@@ -64,11 +72,11 @@ struct GridCall {
     bool profile = false;
 };
 
-// Events of one profiled call: start, after search, before solve, after the
-// main solve kernel, end.
+// Events of one profiled call: start and stop of the search kernel and of
+// the main solve kernel, recorded by the kernel dispatches.
 struct CallTiming {
-    static constexpr int N = 5;
-    hipEvent_t ev[N] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    static constexpr int N = 4;
+    hipEvent_t ev[N] = {nullptr, nullptr, nullptr, nullptr};
     bool valid = false;
 };
 

@@ -183,23 +183,22 @@ def set_profiling(on):
 
 
 def last_timing():
-    """Milliseconds (search, solve, whole call) of the last profiled call."""
-    ms = (C.c_float * 3)()
+    """Milliseconds (search kernel, main solve kernel) of the last profiled call."""
+    ms = (C.c_float * 2)()
     if _hpgl_so.hpglx_last_timing(ms) != 0:
         raise RuntimeError(_hpgl_so.hpglx_last_error().decode())
     return tuple(float(v) for v in ms)
 
 
 def profile_summary():
-    """Mean milliseconds over the calls profiled since set_profiling(True):
-    {"search", "solve_main", "solve", "call"} (HIP events on each call's
-    stream) and the number of calls."""
-    ms = (C.c_float * 4)()
+    """Mean milliseconds of the search kernel and of the main solve kernel
+    over the calls profiled since set_profiling(True) (start/stop events
+    recorded by the kernel dispatches), and the number of calls."""
+    ms = (C.c_float * 2)()
     nc = C.c_int(0)
     if _hpgl_so.hpglx_profile_summary(ms, C.byref(nc)) != 0:
         raise RuntimeError(_hpgl_so.hpglx_last_error().decode())
-    return {"search": float(ms[0]), "solve_main": float(ms[1]), "solve": float(ms[2]), "call": float(ms[3]),
-            "calls": nc.value}
+    return {"search": float(ms[0]), "solve_main": float(ms[1]), "calls": nc.value}
 
 
 def last_call_stats():

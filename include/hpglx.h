@@ -89,15 +89,15 @@ void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz);
  * returns its length and writes up to `cap` dx,dy,dz triples. */
 int hpglx_template(const struct hpgl_cov_params_t* cov, const int radiuses[3], int* out_xyz, int cap);
 
-/* Profiling: when on, each call records HIP events on its stream around the
- * search and the solve kernels.  hpglx_last_timing waits for the last call
- * and returns milliseconds {search, solve, whole call};
+/* Profiling: when on, the dispatches of each call's search kernel and main
+ * solve kernel record start and stop events on the call's stream (no extra
+ * packets between kernels).  hpglx_last_timing waits for the last call and
+ * returns milliseconds {search kernel, main solve kernel};
  * hpglx_profile_summary waits for every call profiled since profiling was
- * switched on (the newest 4096) and returns their mean milliseconds
- * {search, main solve kernel, solve, whole call} and their number. */
+ * switched on (the newest 4096) and returns their means and their number. */
 void hpglx_set_profiling(int on);
-int hpglx_last_timing(float* ms3);
-int hpglx_profile_summary(float* ms4, int* ncalls);
+int hpglx_last_timing(float* ms2);
+int hpglx_profile_summary(float* ms2, int* ncalls);
 
 /* Waits for the last call and reports how many systems the mixed-precision
  * solver re-solved in FP64 and how many cells took the cluster path. */
