@@ -258,6 +258,7 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     if (const char* sv = std::getenv("HPGLX_SOLVE")) call.solve_variant = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH")) call.search_variant = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH_HEAD")) call.search_head = std::atoi(sv);
+    if (const char* sv = std::getenv("HPGLX_CLUSTER_CAP")) call.cluster_cap = std::atoi(sv);
     if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
         err = "values and mask are required";
         return -1;

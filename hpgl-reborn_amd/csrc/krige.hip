@@ -362,7 +362,9 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
     if (active) a.count[cell] = cnt;
 }
 
-// Scratch store for the introsort: element i of lane l lives at i*nlanes + l.
+// Scratch store for the introsort: lane l's elements are contiguous
+// (element i at l * capacity + i), so one lane's sort stays in a few cache
+// lines.
 struct LaneStore {
     struct Item {
         double key;
@@ -389,7 +391,9 @@ struct LaneStore {
     }
 };
 
-// Cluster path, one lane per pending cell, persistent over the cell list.
+// Cluster path with the full covariance table, one lane per pending cell of
+// a cell list (all cluster-path cells when the plan has no octant tables,
+// else the cells k_search_cluster_oct could not hold), persistent.
 __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     const GridDims& g = a.g;
@@ -397,11 +401,12 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
     const int SX = 2 * a.wx + 1, SY = 2 * a.wy + 1;
     const long long center = ((long long)a.wz * SY + a.wy) * SX + a.wx;
     LaneStore st;
-    st.keys = a.skey + lane;
-    st.offs = a.soff + lane;
-    st.stride = a.nlanes;
+    st.keys = a.skey + (long long)lane * a.capacity;
+    st.offs = a.soff + (long long)lane * a.capacity;
+    st.stride = 1;
     for (int item = lane; item < ncells; item += a.nlanes) {
         const long long cell = a.cl_cells[item];
+        if (a.count[cell] != CNT_PENDING_CLUSTER) continue;
         int x, y, z;
         cell_xyz(g, cell, x, y, z);
         const int kx = x / g.rx, ky = y / g.ry, kz = z / g.rz;
@@ -442,21 +447,153 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
     }
 }
 
-// -------------------------------------------------------------------- solve
-// (solve.h)
-#define HPGLX_SOLVE_K(NMAX, G, MIXED)                                \
-    template __global__ void k_solve<NMAX, G, MIXED, 0>(SolveArgs); \
-    template __global__ void k_solve<NMAX, G, MIXED, 1>(SolveArgs); \
-    template __global__ void k_solve<NMAX, G, MIXED, 2>(SolveArgs);
-HPGLX_SOLVE_K(8, 4, false)
-HPGLX_SOLVE_K(8, 4, true)
-HPGLX_SOLVE_K(16, 4, false)
-HPGLX_SOLVE_K(16, 4, true)
-HPGLX_SOLVE_K(24, 4, false)
-HPGLX_SOLVE_K(24, 4, true)
-HPGLX_SOLVE_K(32, 8, false)
-HPGLX_SOLVE_K(32, 4, true)
-#undef HPGLX_SOLVE_K
+// ----------------------------------------------- cluster path, octant ranks
+// Cells of one cluster share the candidate list of its 27 neighbouring
+// clusters.  A block takes a cluster, stages that list in LDS (packed cell
+// coordinates, in the reference's gather order: neighbour clusters k, j, i,
+// nodes ascending), and each lane takes one of the cluster's pending cells:
+// for every candidate the key is the dense rank of its covariance (octant
+// rank table; 0 = not above C(0)/100), and the kept candidates go to a
+// per-lane LDS buffer as one word, rank << qbits | candidate index, which the
+// introsort orders exactly as std::sort orders the keys.  Cells with more than
+// `cap1` kept candidates are appended to an overflow list for k_search_cluster.
+constexpr int CL_BLOCK = 256;
+constexpr int kClusterSortCap = 64;  // kept candidates per lane in LDS
+constexpr int CL_GATHER = 8;         // rank gathers in flight per lane
+
+struct WordStore {
+    struct Item {
+        unsigned key;
+        unsigned w;
+    };
+    unsigned* buf;  // element i at buf[i * CL_BLOCK]
+    int qbits;
+    __device__ unsigned key(int i) const { return buf[i * CL_BLOCK] >> qbits; }
+    __device__ Item get(int i) const {
+        Item it;
+        it.w = buf[i * CL_BLOCK];
+        it.key = it.w >> qbits;
+        return it;
+    }
+    __device__ void put(int i, const Item& it) { buf[i * CL_BLOCK] = it.w; }
+    __device__ void swap(int i, int j) {
+        const unsigned a = buf[i * CL_BLOCK], b = buf[j * CL_BLOCK];
+        buf[i * CL_BLOCK] = b;
+        buf[j * CL_BLOCK] = a;
+    }
+};
+
+struct ClusterOctArgs {
+    GridDims g;
+    int kmax;
+    const int* cl_count;
+    const int* cl_nodes;
+    int node_cap;
+    const unsigned char* nbhd_exceeded;
+    const unsigned* rank_w;        // octant rank table over |d| <= W
+    int wx1, wy1;                  // its x and y extents (W + 1)
+    int qbits;                     // bits of the candidate index in a sort word
+    int cap1;                      // sort buffer entries per lane
+    int* count;
+    int* nboff;
+    int* ovf_cells;                // cells left for k_search_cluster
+    int* n_ovf;
+};
+
+__device__ __forceinline__ unsigned pack_xyz(int x, int y, int z) {
+    return (unsigned)x | ((unsigned)y << 11) | ((unsigned)z << 22);
+}
+
+__global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs a) {
+    extern __shared__ unsigned cl_lds[];
+    __shared__ int off27[28];
+    const GridDims& g = a.g;
+    const int tid = threadIdx.x;
+    unsigned* sbuf = cl_lds;                                  // cap1 x CL_BLOCK words
+    unsigned* cand = cl_lds + (size_t)a.cap1 * CL_BLOCK;      // 27 x node_cap packed coordinates
+    WordStore st;
+    st.buf = sbuf + tid;
+    st.qbits = a.qbits;
+    const int ncl = g.cx * g.cy * g.cz;
+    const int K = a.kmax;
+    for (int ci = blockIdx.x; ci < ncl; ci += gridDim.x) {
+        if (a.nbhd_exceeded[ci]) continue;  // its cells all took the template path
+        const int kx = ci % g.cx, ky = (ci / g.cx) % g.cy, kz = ci / (g.cx * g.cy);
+        const int xa = kx * g.rx, ya = ky * g.ry, za = kz * g.rz;
+        const int wx = max(0, min(g.nx, xa + g.rx) - xa), wy = max(0, min(g.ny, ya + g.ry) - ya),
+                  wz = max(0, min(g.nz, za + g.rz) - za);
+        const int ncell = wx * wy * wz;
+        if (ncell == 0) continue;
+        if (tid == 0) {
+            int acc = 0, q = 0;
+            for (int k = kz - 1; k <= kz + 1; ++k)
+                for (int j = ky - 1; j <= ky + 1; ++j)
+                    for (int i = kx - 1; i <= kx + 1; ++i, ++q) {
+                        off27[q] = acc;
+                        if (i < 0 || j < 0 || k < 0 || i >= g.cx || j >= g.cy || k >= g.cz) continue;
+                        acc += a.cl_count[(k * g.cy + j) * g.cx + i];
+                    }
+            off27[27] = acc;
+        }
+        __syncthreads();
+        const int total = off27[27];
+        for (int idx = tid; idx < total; idx += CL_BLOCK) {
+            int q = 0;
+            while (off27[q + 1] <= idx) ++q;
+            const int k = kz - 1 + q / 9, j = ky - 1 + (q / 3) % 3, i = kx - 1 + q % 3;
+            const int nd = a.cl_nodes[(long long)((k * g.cy + j) * g.cx + i) * a.node_cap + (idx - off27[q])];
+            int x, y, z;
+            cell_xyz(g, nd, x, y, z);
+            cand[idx] = pack_xyz(x, y, z);
+        }
+        __syncthreads();
+        for (int c = tid; c < ncell; c += CL_BLOCK) {
+            const int lx = c % wx, r = c / wx, ly = r % wy, lz = r / wy;
+            const int x = xa + lx, y = ya + ly, z = za + lz;
+            const long long cell = ((long long)z * g.ny + y) * g.nx + x;
+            if (a.count[cell] != CNT_PENDING_CLUSTER) continue;
+            int m = 0;
+            bool over = false;
+            for (int q0 = 0; q0 < total; q0 += CL_GATHER) {
+                unsigned rk[CL_GATHER];
+#pragma unroll
+                for (int u = 0; u < CL_GATHER; ++u) {
+                    const int q = q0 + u < total ? q0 + u : total - 1;
+                    const unsigned p = cand[q];
+                    const int dx = (int)(p & 2047u) - x, dy = (int)((p >> 11) & 2047u) - y, dz = (int)(p >> 22) - z;
+                    const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy, az = dz < 0 ? -dz : dz;
+                    rk[u] = a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax];
+                }
+#pragma unroll
+                for (int u = 0; u < CL_GATHER; ++u) {
+                    if (q0 + u < total && rk[u] != 0u) {
+                        if (m < a.cap1)
+                            sbuf[m * CL_BLOCK + tid] = (rk[u] << a.qbits) | (unsigned)(q0 + u);
+                        else
+                            over = true;
+                        ++m;
+                    }
+                }
+            }
+            if (over) {
+                const int pos = atomicAdd(a.n_ovf, 1);
+                a.ovf_cells[pos] = (int)cell;
+                continue;
+            }
+            Introsort<WordStore> srt(st);
+            srt.sort(m);
+            const int n = m < K ? m : K;
+            const unsigned qmask = (1u << a.qbits) - 1u;
+            for (int i = 0; i < n; ++i) {
+                const unsigned p = cand[sbuf[i * CL_BLOCK + tid] & qmask];
+                a.nboff[cell * K + i] =
+                    pack_off((int)(p & 2047u) - x, (int)((p >> 11) & 2047u) - y, (int)(p >> 22) - z);
+            }
+            a.count[cell] = n | CNT_CLUSTER;
+        }
+        __syncthreads();
+    }
+}
 
 // ------------------------------------------------------------------ launcher
 #define HPGLX_CHECK(expr)                                              \
@@ -489,6 +626,16 @@ bool DevPlan::upload(std::string& err) {
         return false;
     }
     tkey[0] = tkey[1] = tkey[2] = tkey[3] = -1;
+    if (h.octant) {
+        const size_t nw = h.otab_w.size();
+        if (hipMalloc(&otab_w, nw * sizeof(double)) != hipSuccess || hipMalloc(&rank_w, nw * sizeof(unsigned)) != hipSuccess ||
+            hipMemcpy(otab_w, h.otab_w.data(), nw * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(rank_w, h.rank_w.data(), nw * sizeof(unsigned), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "hipMalloc/hipMemcpy failed for the octant tables";
+            release();
+            return false;
+        }
+    }
     if (h.octant && !h.otab.empty()) {
         if (hipMalloc(&otab, h.otab.size() * sizeof(double)) != hipSuccess ||
             hipMemcpy(otab, h.otab.data(), h.otab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
@@ -517,6 +664,10 @@ void DevPlan::release() {
     if (tdz) (void)hipFree(tdz);
     if (otab) (void)hipFree(otab);
     otab = nullptr;
+    if (otab_w) (void)hipFree(otab_w);
+    if (rank_w) (void)hipFree(rank_w);
+    otab_w = nullptr;
+    rank_w = nullptr;
     if (tdelta) (void)hipFree(tdelta);
     if (tdelta_lds) (void)hipFree(tdelta_lds);
     if (tpack) (void)hipFree(tpack);
@@ -559,7 +710,12 @@ template <int NMAX, int G, bool MIXED, int KIND>
 void launch_solve(const SolveArgs& sa, long long ncells, const hipEvent_t* ev, hipStream_t st) {
     const int block = 256;
     const unsigned grid = (unsigned)((ncells * G + block - 1) / block);
-    hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3(grid), dim3(block), 0, st, ev[0], ev[1], 0, sa);
+    if (sa.otab_w)
+        hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, true>), dim3(grid), dim3(block), 0, st, ev[0], ev[1], 0,
+                              sa);
+    else
+        hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, false>), dim3(grid), dim3(block), 0, st, ev[0], ev[1],
+                              0, sa);
 }
 
 // Global-table form over a device-sized cell list (cluster-path cells, and
@@ -570,7 +726,10 @@ void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st)
     long long blocks = (max_cells * G + block - 1) / block;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+    if (sa.otab_w)
+        hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, true>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+    else
+        hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, false>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
 }
 
 // Persistent LDS-table form: as many blocks as are resident at once.  Returns
@@ -736,13 +895,26 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     // Cluster-path scratch: every candidate comes from one of 27 clusters
     // holding at most `limit` nodes each.
     const int cap = (int)(27 * (h.limit > 0 ? h.limit : 1));
-    const int nlanes = 32768;
+    // Octant-rank cluster search: packed coordinates (11/11/10 bits), the
+    // sort word rank << qbits | candidate index, and LDS for the buffers.
+    int qbits = 1;
+    while ((1LL << qbits) < 27LL * node_cap) ++qbits;
+    const int cap1 = c.cluster_cap > 0 && c.cluster_cap < kClusterSortCap ? c.cluster_cap : kClusterSortCap;
+    const size_t cl_lds = ((size_t)cap1 * CL_BLOCK + (size_t)27 * node_cap) * 4;
+    const bool oct_cluster = plan.rank_w && c.solve_variant != 1 && c.nx <= 2048 && c.ny <= 2048 && c.nz <= 1024 &&
+                             qbits < 32 && (unsigned long long)h.rank_max < (1ULL << (32 - qbits)) &&
+                             cl_lds <= kSearchLdsMax;
+    // The full-table search takes what the octant form leaves (its overflow
+    // list), or every cluster-path cell without octant tables.
+    const int nlanes = oct_cluster ? 8192 : 32768;
     auto* skey = (double*)ws.get("skey", (size_t)cap * nlanes * 8, err);
     auto* soff = (int*)ws.get("soff", (size_t)cap * nlanes * 4, err);
     if (!err.empty()) return -1;
     int* n_cl_cells = counters + 0;
     int* overflow = counters + 2;
     int* n_fallback = counters + 3;
+    int* n_ovf = counters + 4;
+    auto* ovf_cells = (int*)ws.get("ovf_cells", (size_t)n * 4, err);
     auto* fb_cells = (int*)ws.get("fb_cells", (size_t)n * 4, err);
     if (!err.empty()) return -1;
 
@@ -836,6 +1008,28 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     ca.count = count;
     ca.nboff = nboff;
     ca.overflow = overflow;
+    if (oct_cluster) {
+        ClusterOctArgs oa;
+        oa.g = g;
+        oa.kmax = K;
+        oa.cl_count = cl_count;
+        oa.cl_nodes = cl_nodes;
+        oa.node_cap = node_cap;
+        oa.nbhd_exceeded = nbhd;
+        oa.rank_w = plan.rank_w;
+        oa.wx1 = h.W[0] + 1;
+        oa.wy1 = h.W[1] + 1;
+        oa.qbits = qbits;
+        oa.cap1 = cap1;
+        oa.count = count;
+        oa.nboff = nboff;
+        oa.ovf_cells = ovf_cells;
+        oa.n_ovf = n_ovf;
+        if (cl_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_cluster_oct, (int)cl_lds));
+        hipLaunchKernelGGL(k_search_cluster_oct, dim3((unsigned)std::min(ncl, 2048)), dim3(CL_BLOCK), cl_lds, st, oa);
+        ca.cl_cells = ovf_cells;
+        ca.n_cl_cells = n_ovf;
+    }
     hipLaunchKernelGGL(k_search_cluster, dim3(nlanes / B), dim3(B), 0, st, ca);
 
     SolveArgs so;
@@ -859,6 +1053,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.wy = h.W[1];
     so.wz = h.W[2];
     so.otab = plan.otab;
+    so.otab_w = c.solve_variant == 1 ? nullptr : plan.otab_w;
     so.ex = h.E[0];
     so.ey = h.E[1];
     so.ez = h.E[2];
@@ -880,7 +1075,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.fallback_cells = fb_cells;
     so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
-    const int nelem = (plan.otab && c.solve_variant != 1) ? (int)h.otab.size() : 0;
+    const int nelem = (plan.otab && c.solve_variant == 0) ? (int)h.otab.size() : 0;
     if (K <= 8)
         dispatch_solve<8, 4, 4>(so, c.kind, mixed, n, nelem, ev_t + 2, st);
     else if (K <= 16)

@@ -64,8 +64,8 @@ struct ClusterSearchArgs {
     const double* ctab;            // covariance table over +-W
     int wx, wy, wz;                // table half-widths
     double threshold;              // C(0) / 100
-    double* skey;                  // scratch keys   [capacity][nlanes]
-    int* soff;                     // scratch offsets
+    double* skey;                  // scratch keys   [nlanes][capacity]
+    int* soff;                     // scratch offsets [nlanes][capacity]
     int capacity;                  // scratch entries per lane
     int nlanes;                    // total lanes of the launch
     int* count;
@@ -93,6 +93,7 @@ struct SolveArgs {
     int wx, wy, wz;
     const double* otab;            // octant table (or null), staged in LDS by k_solve_lds
     int ex, ey, ez;                // its extents (|d| <= e)
+    const double* otab_w;          // octant table over |d| <= (wx, wy, wz) (or null)
     double c0;                     // C(0)
     double pivot_tol;              // mixed precision: FP64 fallback threshold
     int refine_steps;

@@ -19,6 +19,8 @@ struct DevPlan {
     int* tdy = nullptr;
     int* tdz = nullptr;
     double* otab = nullptr;
+    double* otab_w = nullptr;
+    unsigned* rank_w = nullptr;
     int tsize = 0;
     // Template deltas for one grid geometry (padded pitch and tile box);
     // tkey = (px, py, lx, ly) they were computed for.
@@ -67,8 +69,9 @@ struct GridCall {
     double pivot_tol = 1e-3;
     int refine_steps = 1;
     int search_variant = 0;                 // 0 = LDS tiles when the box fits, 1 = global walk
-    int solve_variant = 0;                  // 0 = LDS covariance table when it fits, 1 = global table
+    int solve_variant = 0;                  // 0 = LDS / octant tables, 1 = full table only, 2 = no LDS table
     int search_head = 0;                    // packed offsets staged in LDS by the search (0 = default)
+    int cluster_cap = 0;                    // kept candidates per lane of the LDS cluster search (0 = default)
     bool profile = false;
 };
 

@@ -186,31 +186,64 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
     }
     for (auto& th : pool) th.join();
 
-    // Octant table for the LDS solve: only when it fits one workgroup's LDS
-    // and the symmetry holds bit for bit over the whole +-2r box.
+    // Octant tables.  With all angles zero the covariance is even in each
+    // offset component; the symmetry is checked bit for bit over the whole
+    // +-W box before any octant table is used.
     p.octant = false;
     p.otab.clear();
+    p.otab_w.clear();
+    p.rank_w.clear();
     for (int i = 0; i < 3; ++i) p.E[i] = 2 * p.r[i];
-    const int ox = p.E[0] + 1, oy = p.E[1] + 1, oz = p.E[2] + 1;
-    const size_t on = (size_t)ox * oy * oz;
-    if (cov.angles[0] == 0 && cov.angles[1] == 0 && cov.angles[2] == 0 && on * sizeof(double) <= kOctantMaxBytes) {
+    const int wx1 = p.W[0] + 1, wy1 = p.W[1] + 1, wz1 = p.W[2] + 1;
+    const size_t wn = (size_t)wx1 * wy1 * wz1;
+    if (cov.angles[0] == 0 && cov.angles[1] == 0 && cov.angles[2] == 0 && wn <= kOctantWideMaxEntries) {
         auto at = [&](int x, int y, int z) {
             return p.ctab[((size_t)(z + p.W[2]) * sy + (y + p.W[1])) * sx + (x + p.W[0])];
         };
-        p.otab.resize(on);
+        p.otab_w.resize(wn);
         bool same = true;
-        for (int z = 0; z < oz && same; ++z)
-            for (int y = 0; y < oy && same; ++y)
-                for (int x = 0; x < ox && same; ++x) {
+        for (int z = 0; z < wz1 && same; ++z)
+            for (int y = 0; y < wy1 && same; ++y)
+                for (int x = 0; x < wx1 && same; ++x) {
                     const double v = at(x, y, z);
-                    p.otab[((size_t)z * oy + y) * ox + x] = v;
+                    p.otab_w[((size_t)z * wy1 + y) * wx1 + x] = v;
                     for (int m = 1; m < 8; ++m) {
                         const double u = at(m & 1 ? -x : x, m & 2 ? -y : y, m & 4 ? -z : z);
                         if (std::memcmp(&u, &v, sizeof v) != 0) same = false;
                     }
                 }
-        p.octant = same;
-        if (!same) p.otab.clear();
+        if (!same) {
+            p.otab_w.clear();
+        } else {
+            p.octant = true;
+            // Dense ranks of the keys above C(0)/100 (0 = filtered out): the
+            // cluster path's std::sort comparator (key a > key b) and its
+            // filter (key > C(0)/100) give the same answers on ranks.
+            const double thr = p.c0() / 100;
+            std::vector<double> vals;
+            vals.reserve(wn);
+            for (double v : p.otab_w)
+                if (v > thr) vals.push_back(v);
+            std::sort(vals.begin(), vals.end());
+            vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+            p.rank_w.resize(wn);
+            p.rank_max = (unsigned)vals.size();
+            for (size_t i = 0; i < wn; ++i) {
+                const double v = p.otab_w[i];
+                p.rank_w[i] = v > thr ? (unsigned)(std::lower_bound(vals.begin(), vals.end(), v) - vals.begin()) + 1u
+                                      : 0u;
+            }
+            // The LDS table of the solve: |d| <= 2r, when it fits.
+            const int ox = p.E[0] + 1, oy = p.E[1] + 1, oz = p.E[2] + 1;
+            const size_t on = (size_t)ox * oy * oz;
+            if (on * sizeof(double) <= kOctantMaxBytes) {
+                p.otab.resize(on);
+                for (int z = 0; z < oz; ++z)
+                    for (int y = 0; y < oy; ++y)
+                        for (int x = 0; x < ox; ++x)
+                            p.otab[((size_t)z * oy + y) * ox + x] = p.otab_w[((size_t)z * wy1 + y) * wx1 + x];
+            }
+        }
     }
     return true;
 }

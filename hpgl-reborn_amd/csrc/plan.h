@@ -26,6 +26,8 @@ struct CovParams {
 
 // Largest octant table staged in LDS (one workgroup's LDS on gfx950).
 constexpr size_t kOctantMaxBytes = 160 * 1024;
+// Largest +-W octant table built (entries).
+constexpr size_t kOctantWideMaxEntries = size_t(32) << 20;
 
 struct PlanHost {
     Kind kind;
@@ -40,13 +42,20 @@ struct PlanHost {
     // axis covers every neighbour pair of either search path.
     int W[3];
     std::vector<double> ctab;
-    // Octant table: with all angles zero the covariance is even in each
+    // Octant tables: with all angles zero the covariance is even in each
     // offset component separately, so C(dx,dy,dz) = C(|dx|,|dy|,|dz|) bit for
-    // bit (checked entry by entry when the plan is built).  otab covers
-    // |d| <= E = 2r (every pair of template-path neighbours), x fastest; empty
-    // when the angles are not all zero or it exceeds kOctantMaxBytes.
+    // bit (checked entry by entry over +-W when the plan is built; octant is
+    // false and the tables empty otherwise).  All are x fastest.
+    //   otab_w  |d| <= W: every pair either search path can produce
+    //   rank_w  |d| <= W: 0 where C <= C(0)/100, else 1 + the dense rank of C
+    //           among the table's values above C(0)/100
+    //   otab    |d| <= E = 2r: every pair of template-path neighbours; empty
+    //           when it exceeds kOctantMaxBytes (it is staged in LDS)
     bool octant = false;
     int E[3] = {0, 0, 0};
+    std::vector<double> otab_w;
+    std::vector<unsigned> rank_w;
+    unsigned rank_max = 0;  // largest entry of rank_w
     std::vector<double> otab;
     // Clusterizer capacity: |template| / 250.
     long long limit;

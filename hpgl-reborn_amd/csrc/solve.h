@@ -225,9 +225,10 @@ __device__ __forceinline__ unsigned adiff(unsigned a, unsigned b, unsigned acc) 
     return (a > b ? a - b : b - a) + acc;
 }
 
-// LDS form of the covariance lookup: the octant table staged in LDS is
-// indexed by |dx|, |dy|, |dz| of the pair's offset difference.  Each
-// neighbour offset is kept as three biased, pre-scaled components
+// Octant form of the covariance lookup (LTAB): an octant table, staged in
+// LDS or read from global memory, indexed by |dx|, |dy|, |dz| of the pair's
+// offset difference with row strides ox (x fastest) and oxy.  Each neighbour
+// offset is kept as three biased, pre-scaled components
 // ((d + OFF_BIAS) * stride * 8), so a pair's byte address is the sum of three
 // absolute differences.
 constexpr int OFF_BIAS = 1024;
@@ -236,8 +237,8 @@ constexpr int OFF_BIAS = 1024;
 // false when the mixed form hands the cell to the FP64 form.
 template <int NMAX, int G, bool MIXED, int KIND, bool LTAB>
 __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, const double* ltab,
-                                             double (&w)[NMAX / G], double (&bq)[NMAX / G], int (&dnode)[NMAX / G],
-                                             double& mu, int& status) {
+                                             unsigned ox, unsigned oxy, double (&w)[NMAX / G], double (&bq)[NMAX / G],
+                                             int (&dnode)[NMAX / G], double& mu, int& status) {
     using T = typename std::conditional<MIXED, float, double>::type;
     typedef Chol<NMAX, G, T> CH;
     constexpr int S = CH::S;
@@ -256,7 +257,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         const int r = G * s + t;
         pk[s] = nb[r < n ? r : 0];  // unconditional, so the loads issue together
     }
-    const unsigned OX = (unsigned)(a.ex + 1), OXY = OX * (unsigned)(a.ey + 1);
+    const unsigned OX = ox, OXY = oxy;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int r = G * s + t;
@@ -454,9 +455,11 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 }
 
 // Kriges one cell with the group of G lanes; t = lane index in the group,
-// cnt = the cell's count[] entry, ltab = the LDS octant table (LTAB form).
+// cnt = the cell's count[] entry, ltab/ox/oxy = the octant table and its
+// strides (LTAB form).
 template <int NMAX, int G, bool MIXED, int KIND, bool LTAB>
-__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int cnt, int t, const double* ltab) {
+__device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int cnt, int t, const double* ltab,
+                                           unsigned ox, unsigned oxy) {
     constexpr int S = NMAX / G;
     const GridDims& g = a.g;
     if (KIND != 2 && cnt == CNT_INFORMED) {
@@ -476,7 +479,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
     if (n == 0) {
         status = ST_NO_NEIGHBOURS;
-    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB>(a, cell, t, n, ltab, w, bq, dnode, mu, status)) {
+    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB>(a, cell, t, n, ltab, ox, oxy, w, bq, dnode, mu, status)) {
         return;  // queued for the FP64 form
     }
     if (status != ST_KRIGED) {
@@ -595,17 +598,19 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
 }
 
-// One group per cell (global covariance table); with a cell list, groups
+// One group per cell, covariances from global memory: the +-W octant table
+// when OCT (angles all zero), else the full table.  With a cell list, groups
 // stride over the list.
-template <int NMAX, int G, bool MIXED, int KIND>
+template <int NMAX, int G, bool MIXED, int KIND, bool OCT>
 __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     const int t = threadIdx.x & (G - 1);
     const long long gid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const long long ngroups = ((long long)gridDim.x * blockDim.x) / G;
     const long long total = a.cell_list ? (long long)*a.n_list : a.g.n;
+    const unsigned ox = (unsigned)(a.wx + 1), oxy = ox * (unsigned)(a.wy + 1);
     for (long long item = gid; item < total; item += ngroups) {
         const long long cell = a.cell_list ? (long long)a.cell_list[item] : item;
-        solve_cell<NMAX, G, MIXED, KIND, false>(a, cell, a.count[cell], t, nullptr);
+        solve_cell<NMAX, G, MIXED, KIND, OCT>(a, cell, a.count[cell], t, a.otab_w, ox, oxy);
     }
 }
 
@@ -620,10 +625,11 @@ __global__ __launch_bounds__(BLOCK) void k_solve_lds(SolveArgs a, int nelem) {
     const int t = threadIdx.x & (G - 1);
     const long long gid = ((long long)blockIdx.x * BLOCK + threadIdx.x) / G;
     const long long ngroups = ((long long)gridDim.x * BLOCK) / G;
+    const unsigned ox = (unsigned)(a.ex + 1), oxy = ox * (unsigned)(a.ey + 1);
     for (long long cell = gid; cell < a.g.n; cell += ngroups) {
         const int cnt = a.count[cell];
         if (cnt >= 0 && (cnt & CNT_CLUSTER)) continue;
-        solve_cell<NMAX, G, MIXED, KIND, true>(a, cell, cnt, t, otab_lds);
+        solve_cell<NMAX, G, MIXED, KIND, true>(a, cell, cnt, t, otab_lds, ox, oxy);
     }
 }
 

@@ -94,15 +94,24 @@ def test_neighbour_sets_bit_exact(orc, name, scale):
 VARIANT_CASES = [("sk", 0.5), ("ok", 0.4), ("mik", 0.25), ("wells", 0.2)]
 
 
+VARIANT_ENVS = [
+    {"HPGLX_SOLVE": "1", "HPGLX_SEARCH": "1"},   # full covariance table everywhere, global template walk
+    {"HPGLX_SOLVE": "2"},                        # octant table from global memory, no LDS table
+    {"HPGLX_CLUSTER_CAP": "4"},                  # cluster path mostly through the overflow list
+]
+
+
 @pytest.mark.parametrize("name,scale", VARIANT_CASES)
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_kernel_variants_bitwise(monkeypatch, name, scale, precision):
-    """The LDS-table solve and the LDS-tile search give the same bits as the
-    global-memory forms (same table values, same operation order)."""
+@pytest.mark.parametrize("env", range(len(VARIANT_ENVS)))
+def test_kernel_variants_bitwise(monkeypatch, name, scale, precision, env):
+    """The LDS-table solve, the octant tables, the LDS-tile search and the
+    LDS cluster search give the same bits as the full-table global-memory
+    forms (same table values, same operation order, same sort)."""
     wl = _wl(name, scale)
     base = _device(wl, precision)
-    monkeypatch.setenv("HPGLX_SOLVE", "1")
-    monkeypatch.setenv("HPGLX_SEARCH", "1")
+    for k, v in VARIANT_ENVS[env].items():
+        monkeypatch.setenv(k, v)
     alt = _device(wl, precision)
     assert base.keys() == alt.keys()
     for key in base:
