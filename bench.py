@@ -29,9 +29,16 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (packed)
 
 # HBM bytes per launch of the main solve kernel on this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of
-# known size (profiles/hbm_traffic_ok_100x100x50.md); kernels not measured
-# report null.
-TRAFFIC_BYTES = {}
+# known size (--calibrate-copy; profiles/hbm_traffic_ok_100x100x50.md):
+# FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, KiB x 1024.
+TRAFFIC_BYTES = {
+    "k_solve_lds<24,4,mixed,OK>": 71.96e6,
+    "k_solve_lds<24,4,fp64,OK>": 70.92e6,
+}
+# Algorithmic bytes of one solve launch: per kriged cell 4 B count + 96 B
+# offsets in, 13 B estimate/mask/variance out; counts of the informed cells,
+# the value array and the octant table once.
+ALGO_BYTES = 475000 * 113 + 25000 * 4 + 500000 * 4 + 121 * 1024
 
 
 def flops_ok(n):
@@ -84,6 +91,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "
+                         "scaling memory counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the kriging path runs on a "
@@ -156,8 +166,9 @@ def main():
     def roof(r, peak, kernel):
         achieved = flops_main / (r["prof"]["solve_main"] / 1e3) / 1e12
         return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": TRAFFIC_BYTES.get(kernel), "flops_per_launch": flops_main,
-                "launch_ms": r["prof"]["solve_main"]}
+                "frac": achieved / peak, "traffic": TRAFFIC_BYTES.get(kernel), "algorithmic_bytes": ALGO_BYTES,
+                "flops_per_launch": flops_main, "launch_ms": r["prof"]["solve_main"],
+                "hbm_gbs": (TRAFFIC_BYTES.get(kernel) or 0) / (r["prof"]["solve_main"] / 1e3) / 1e9}
 
     mixed = results["mixed"]
     fp64 = results["fp64"]
@@ -188,6 +199,13 @@ def main():
         "roofline": roof(mixed, PEAK_FP32_TFLOPS, k_mixed),
         "flops_per_step": flops_all,
     }
+    if args.calibrate_copy > 0:
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["calibration_copy_bytes"] = src.numel() * 4
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(line), flush=True)
