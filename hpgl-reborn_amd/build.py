@@ -42,10 +42,19 @@ def build(verbose=True):
     plan_o = os.path.join(BUILD, "plan.o")
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "plan.cpp"), "-o", plan_o])
     objs.append(plan_o)
-    for src in ("krige.hip", "api.hip"):
+    # the HIP translation units compile in parallel
+    procs = []
+    units = ["krige.hip", "loo.hip", "api.hip", "loo_m0.hip", "loo_m1.hip", "loo_m2.hip"]
+    units += [f"solve_n{n}_k{k}.hip" for n in (32, 24, 16, 8) for k in (0, 1, 2)]
+    for src in units:
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(o)
+    failed = [src for src, pr in procs if pr.wait() != 0]
+    if failed:
+        raise RuntimeError("hipcc failed for " + ", ".join(failed))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     shutil.move(tmp, LIB)

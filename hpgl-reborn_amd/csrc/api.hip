@@ -18,6 +18,7 @@
 #include "introsort.h"
 #include "krige_kernels.h"
 #include "krige_launch.h"
+#include "loo.h"
 #include "plan.h"
 
 namespace hpglx {
@@ -753,6 +754,79 @@ void hpglx_host_introsort_perm(const double* keys, int n, int* perm) {
 
 const char* hpglx_last_error(void) {
     return lib().last_error.c_str();
+}
+
+// ------------------------------------------------ leave-one-out kriging (loo.hip)
+int hpglx_knn(const float* xyz, int n, int k, int* nbr, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!xyz || !nbr) return fail(L, "knn: coordinates and output are required");
+    if (knn(xyz, n, k, nbr, (hipStream_t)stream, err) != 0) return fail(L, err);
+    return 0;
+}
+
+size_t hpglx_loo_state_size(int n, int k, int nsets) {
+    if (n < 2 || k < 1 || k > 32 || nsets < 1) return 0;
+    return loo_state_doubles(n, k, nsets) * sizeof(double);
+}
+
+int hpglx_loo_forward(const float* xyz, const float* values, int n, const int* nbr, int k, const double* params,
+                      int nsets, int model, double* est, int* status, void* state, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    if (!xyz || !values || !nbr || !params || !est) return fail(L, "loo: inputs and est are required");
+    LooArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.n = n;
+    a.k = k;
+    a.np = nsets;
+    a.model = model;
+    a.xyz = xyz;
+    a.val = values;
+    a.nbr = nbr;
+    a.prm = params;
+    a.est = est;
+    a.status = status;
+    a.state = (double*)state;
+    std::string err;
+    if (loo_forward(a, (hipStream_t)stream, err) != 0) return fail(L, err);
+    return 0;
+}
+
+int hpglx_loo_backward(const float* xyz, const float* values, int n, const int* nbr, int k, const double* params,
+                       int nsets, int model, const void* state, const double* grad_est, double* grad_params,
+                       double* jac, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!xyz || !values || !nbr || !params || !state) return fail(L, "loo backward: inputs and state are required");
+    if (!ensure_events(L, err)) return fail(L, err);
+    hipStream_t st = (hipStream_t)stream;
+    // the partial-sum buffer is shared by calls: order after the previous one
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) return fail(L, "hipStreamWaitEvent failed");
+    double* part = (double*)L.ws.get("loo_part", loo_part_doubles(n, k, nsets) * sizeof(double), err);
+    if (!part) return fail(L, err);
+    LooArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.n = n;
+    a.k = k;
+    a.np = nsets;
+    a.model = model;
+    a.xyz = xyz;
+    a.val = values;
+    a.nbr = nbr;
+    a.prm = params;
+    a.state = (double*)state;
+    a.gest = grad_est;
+    a.jac = jac;
+    a.part = part;
+    if (loo_backward(a, grad_params, st, err) != 0) return fail(L, err);
+    if (hipEventRecord(L.done, st) != hipSuccess) return fail(L, "hipEventRecord failed");
+    return 0;
 }
 
 const char* hpglx_version(void) {

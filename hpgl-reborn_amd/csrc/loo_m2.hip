@@ -1,0 +1,15 @@
+// loo_m2.hip -- leave-one-out kernels for covariance model 2 (0 spherical, 1 exponential, 2 gaussian).
+#include "loo_kernels.h"
+
+namespace hpglx {
+
+void launch_loo_model_2(const LooArgs& a, bool backward, hipStream_t st) {
+    switch (nmax_for(a.k)) {
+        case 8: launch_loo_m<8, 2>(a, backward, st); break;
+        case 16: launch_loo_m<16, 2>(a, backward, st); break;
+        case 24: launch_loo_m<24, 2>(a, backward, st); break;
+        default: launch_loo_m<32, 2>(a, backward, st); break;
+    }
+}
+
+}  // namespace hpglx

@@ -1,0 +1,43 @@
+// solve_dispatch.h -- entry points of the solve kernels, one per
+// neighbour-count bucket (max_neighbours <= 8, 16, 24, 32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "krige_kernels.h"
+
+namespace hpglx {
+
+// Launches the solve of one call: the LDS-table kernel over template-path
+// cells when nelem > 0 (octant table entries), the global-table kernel over
+// the cluster-path list (or over every cell), and for the mixed form the
+// FP64 re-solve of its fallbacks.  ev = start/stop events for the main
+// kernel (entries may be null).
+void dispatch_solve_8(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
+                      hipStream_t st);
+void dispatch_solve_16(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
+                       hipStream_t st);
+void dispatch_solve_24(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
+                       hipStream_t st);
+void dispatch_solve_32(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
+                       hipStream_t st);
+
+// One per (bucket, kind): solve_n<NMAX>_k<KIND>.hip.
+#define HPGLX_DECL_SOLVE(NMAX, KIND)                                                                      \
+    void dispatch_solve_##NMAX##_##KIND(const SolveArgs& sa, bool mixed, long long n, int nelem,          \
+                                        const hipEvent_t* ev, hipStream_t st);
+HPGLX_DECL_SOLVE(8, 0)
+HPGLX_DECL_SOLVE(8, 1)
+HPGLX_DECL_SOLVE(8, 2)
+HPGLX_DECL_SOLVE(16, 0)
+HPGLX_DECL_SOLVE(16, 1)
+HPGLX_DECL_SOLVE(16, 2)
+HPGLX_DECL_SOLVE(24, 0)
+HPGLX_DECL_SOLVE(24, 1)
+HPGLX_DECL_SOLVE(24, 2)
+HPGLX_DECL_SOLVE(32, 0)
+HPGLX_DECL_SOLVE(32, 1)
+HPGLX_DECL_SOLVE(32, 2)
+#undef HPGLX_DECL_SOLVE
+
+}  // namespace hpglx

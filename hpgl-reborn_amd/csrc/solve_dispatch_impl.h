@@ -1,0 +1,115 @@
+// solve_dispatch_impl.h -- launch logic of the solve kernels (solve.h),
+// included by one translation unit per neighbour-count bucket
+// (solve_n8.hip ... solve_n32.hip) so the buckets compile in parallel.
+#pragma once
+
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "solve.h"
+#include "solve_dispatch.h"
+
+namespace hpglx {
+namespace {
+
+template <int NMAX, int G, bool MIXED, int KIND>
+void launch_solve(const SolveArgs& sa, long long ncells, const hipEvent_t* ev, hipStream_t st) {
+    const int block = 256;
+    const unsigned grid = (unsigned)((ncells * G + block - 1) / block);
+    if (sa.otab_w)
+        hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, true>), dim3(grid), dim3(block), 0, st, ev[0], ev[1], 0,
+                              sa);
+    else
+        hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, false>), dim3(grid), dim3(block), 0, st, ev[0], ev[1],
+                              0, sa);
+}
+
+// Global-table form over a device-sized cell list (cluster-path cells, and
+// the FP64 re-solve of the mixed form's fallbacks).
+template <int NMAX, int G, bool MIXED, int KIND>
+void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    const int block = 256;
+    long long blocks = (max_cells * G + block - 1) / block;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    if (sa.otab_w)
+        hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, true>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+    else
+        hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, false>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
+}
+
+// Persistent LDS-table form: as many blocks as are resident at once.  Returns
+// false (nothing launched) when the table does not fit a block's LDS.
+template <int NMAX, int G, bool MIXED, int KIND, int BLOCK>
+bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipStream_t st) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, int> grids;  // (device, LDS bytes) -> grid
+    const void* fn = (const void*)&k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>;
+    const int lds = nelem * (int)sizeof(double);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    int grid = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = grids.find({dev, lds});
+        if (it != grids.end()) {
+            grid = it->second;
+        } else {
+            int cus = 0, nb = 0;
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, BLOCK, lds) != hipSuccess)
+                nb = 0;
+            (void)hipGetLastError();
+            grid = nb * cus;
+            grids[{dev, lds}] = grid;
+        }
+    }
+    if (grid <= 0) return false;
+    hipExtLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK),
+                          (uint32_t)lds, st, ev[0], ev[1], 0, sa, nelem);
+    return true;
+}
+
+// Where the plan has an octant table: the LDS form over template-path cells,
+// then the global form over the cluster-path list; otherwise the global form
+// over every cell.  The mixed form then re-solves its fallbacks in FP64.
+// G64 / G32: lanes per cell of the FP64 and of the mixed form.
+template <int NMAX, int G64, int G32, int KIND>
+void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev, hipStream_t st) {
+    constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
+    bool lds = false;
+    if (nelem > 0)
+        lds = mixed ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32>(sa, nelem, ev, st)
+                    : launch_solve_lds<NMAX, G64, false, KIND, 256>(sa, nelem, ev, st);
+    if (!lds) {
+        if (mixed)
+            launch_solve<NMAX, G32, true, KIND>(sa, n, ev, st);
+        else
+            launch_solve<NMAX, G64, false, KIND>(sa, n, ev, st);
+    }
+    if (lds) {
+        SolveArgs cl = sa;
+        cl.cell_list = sa.cl_cells;
+        cl.n_list = sa.n_cl_cells;
+        if (mixed)
+            launch_solve_list<NMAX, G32, true, KIND>(cl, n, st);
+        else
+            launch_solve_list<NMAX, G64, false, KIND>(cl, n, st);
+    }
+    if (mixed) {
+        SolveArgs fb = sa;
+        fb.cell_list = sa.fallback_cells;
+        fb.n_list = sa.n_fallback;
+        fb.fallback_cells = nullptr;
+        fb.n_fallback = nullptr;
+        launch_solve_list<NMAX, G64, false, KIND>(fb, n, st);
+    }
+}
+
+}  // namespace
+}  // namespace hpglx

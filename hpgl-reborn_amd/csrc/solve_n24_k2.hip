@@ -1,0 +1,11 @@
+// solve_n24_k2.hip -- solve kernels for max_neighbours <= 24, kind 2 (0 OK, 1 SK, 2 median IK).
+#include "solve_dispatch_impl.h"
+
+namespace hpglx {
+
+void dispatch_solve_24_2(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev,
+                           hipStream_t st) {
+    dispatch_kind<24, 4, 4, 2>(sa, mixed, n, nelem, ev, st);
+}
+
+}  // namespace hpglx

@@ -1,0 +1,11 @@
+// solve_n32_k0.hip -- solve kernels for max_neighbours <= 32, kind 0 (0 OK, 1 SK, 2 median IK).
+#include "solve_dispatch_impl.h"
+
+namespace hpglx {
+
+void dispatch_solve_32_0(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev,
+                           hipStream_t st) {
+    dispatch_kind<32, 8, 4, 0>(sa, mixed, n, nelem, ev, st);
+}
+
+}  // namespace hpglx

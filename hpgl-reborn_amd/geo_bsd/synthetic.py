@@ -103,3 +103,22 @@ def ok_wells_c4(scale=1.0, seed=4):
     return Workload(name="ok_wells_256x256x64", kind="ok", shape=shape, values=with_undefined(f, m), mask=m,
                     cov=dict(type=2, ranges=(30.0, 30.0, 6.0), angles=(0.0, 0.0, 0.0), sill=1.0, nugget=0.01),
                     radiuses=(30, 30, 6), k=32)
+
+
+def loo_c5(n=20000, nsets=64, seed=5):
+    """Leave-one-out kriging of scattered points: n float32 points uniform in
+    [0,200) x [0,200) x [0,50), values of the 7x7x3 moving-average field
+    (mean 0.20, sd 0.05) on a 200x200x50 grid sampled at floor(coordinate);
+    spherical model, k = 24; nsets parameter sets with ranges U[5,40] (each
+    axis), nugget U[0,0.3], sill 1."""
+    rng = numpy.random.default_rng(seed)
+    shape = (200, 200, 50)
+    field = moving_average_field(rng, shape, (7, 7, 3), 0.20, 0.05)
+    xyz = (rng.random((n, 3)) * numpy.array(shape, dtype=numpy.float64)).astype(numpy.float32)
+    ix = numpy.minimum(numpy.floor(xyz).astype(numpy.int64), numpy.array(shape) - 1)
+    vals = field[ix[:, 0] + shape[0] * (ix[:, 1] + shape[1] * ix[:, 2])]
+    ranges = rng.uniform(5.0, 40.0, size=(nsets, 3))
+    nugget = rng.uniform(0.0, 0.3, size=nsets)
+    params = numpy.column_stack([numpy.ones(nsets), nugget, ranges]).astype(numpy.float64)
+    return Workload(name=f"loo_{n}pts_{nsets}sets", kind="loo", xyz=xyz, values=vals.astype(numpy.float32),
+                    params=params, k=24, model="spherical")

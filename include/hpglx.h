@@ -17,6 +17,8 @@
 #ifndef HPGL_MI355X_HPGLX_H
 #define HPGL_MI355X_HPGLX_H
 
+#include <stddef.h>
+
 #include "hpgl.h"
 
 #ifdef __cplusplus
@@ -109,6 +111,33 @@ void hpglx_set_default_precision(int precision);
 /* Host run of the device introsort (for tests): perm[i] = input position of
  * the element sorted to slot i, ordering keys descending. */
 void hpglx_host_introsort_perm(const double* keys, int n, int* perm);
+
+/* Leave-one-out ordinary kriging of scattered points (no counterpart in the
+ * reference; built on its covariance model, cov_model.h:146-183, and its OK
+ * system, my_kriging_weights.h:198-361).  Point i is estimated from its k
+ * nearest other points for each of `nsets` parameter sets
+ * params[5 p .. 5 p + 4] = (sill, nugget, range_x, range_y, range_z), model
+ * 0 spherical, 1 exponential, 2 gaussian.  All arrays are device arrays:
+ * xyz n x 3, values n, nbr n x k, est nsets x n (status 0 kriged, 2 singular:
+ * estimate = neighbour mean, zero derivative).
+ *
+ * hpglx_knn: the k nearest other points by Euclidean distance, ties by the
+ * lower index, ascending.
+ * hpglx_loo_forward: estimates; when `state` is non-null (a device buffer of
+ * hpglx_loo_state_size bytes) it keeps the Cholesky factors and weights for
+ * the backward pass.
+ * hpglx_loo_backward: with grad_est (nsets x n, may be null) the gradient of
+ * sum_p,i grad_est[p, i] * est[p, i] w.r.t. each parameter set into
+ * grad_params (nsets x 5, may be null), and optionally the Jacobian
+ * d est[p, i] / d params[p] into jac (nsets x n x 5).  The adjoint reuses the
+ * forward factor and recomputes covariance derivatives. */
+int hpglx_knn(const float* xyz, int n, int k, int* nbr, void* stream);
+size_t hpglx_loo_state_size(int n, int k, int nsets);
+int hpglx_loo_forward(const float* xyz, const float* values, int n, const int* nbr, int k, const double* params,
+                      int nsets, int model, double* est, int* status, void* state, void* stream);
+int hpglx_loo_backward(const float* xyz, const float* values, int n, const int* nbr, int k, const double* params,
+                       int nsets, int model, const void* state, const double* grad_est, double* grad_params,
+                       double* jac, void* stream);
 
 const char* hpglx_last_error(void);
 const char* hpglx_version(void);
