@@ -78,6 +78,71 @@ def cpu_baseline(wl):
             "sample": f"full 100x100x50 OK grid ({kriged} kriged cells), median of 3 runs, {threads} OpenMP threads"}
 
 
+def loo_bench(torch, check_oracle):
+    """Leave-one-out kriging of 20,000 scattered points for 64 spherical
+    parameter sets (k = 24): k-NN, forward, adjoint backward, 200 Adam steps."""
+    import numpy as np
+    from geo_bsd import loo as L, synthetic as S
+    wl = S.loo_c5()
+    dev = torch.device("cuda:0")
+    xyz = torch.from_numpy(wl.xyz).to(dev)
+    v = torch.from_numpy(wl.values).to(dev)
+    prm = torch.from_numpy(wl.params).to(dev)
+    nbr = L.knn(xyz, wl.k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nbr = L.knn(xyz, wl.k)
+    torch.cuda.synchronize()
+    knn_ms = (time.perf_counter() - t0) * 1e3
+    fwd, bwd = [], []
+    for it in range(6):
+        p = prm.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = L.loo_loss(p, xyz, v, nbr)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it >= 2:
+            fwd.append(t1 - t0)
+            bwd.append(t2 - t1)
+    t0 = time.perf_counter()
+    _, hist = L.fit(prm, xyz, v, nbr, steps=200)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    h = hist.cpu().numpy()
+    nsys = wl.params.shape[0] * wl.xyz.shape[0]
+    out = {"workload": f"{wl.xyz.shape[0]} scattered points x {wl.params.shape[0]} spherical parameter sets, k=24",
+           "knn_ms": knn_ms, "forward_ms": 1e3 * statistics.median(fwd), "backward_ms": 1e3 * statistics.median(bwd),
+           "systems_per_s_forward": nsys / statistics.median(fwd),
+           "adam_200_steps_s": fit_s, "loss_best_start": [float(h[0].min()), float(h[-1].min())],
+           "loss_mean": [float(h[0].mean()), float(h[-1].mean())], "dtype": "f64"}
+    if check_oracle:
+        from oracle import loo_oracle as O
+        sub = np.arange(0, wl.xyz.shape[0], 100)
+        with torch.no_grad():
+            est = L.loo_estimates(prm[:2], xyz, v, nbr).cpu().numpy()[:, sub]
+        # the oracle on a subsample of points, with their neighbour lists
+        nb = nbr.cpu().numpy()
+        xyz_np, v_np = wl.xyz, wl.values
+        ref = np.empty((2, len(sub)))
+        for j, i in enumerate(sub):
+            pts = xyz_np[nb[i]].astype(np.float64)
+            for p in range(2):
+                k = nb.shape[1]
+                A = O.cov("spherical", wl.params[p], pts[:, None, :] - pts[None, :, :])
+                b = O.cov("spherical", wl.params[p], pts - xyz_np[i].astype(np.float64))
+                K = np.ones((k + 1, k + 1))
+                K[:k, :k] = A
+                K[k, k] = 0
+                y = np.linalg.solve(K, np.concatenate([b, [1.0]]))
+                ref[p, j] = y[:k] @ v_np[nb[i]].astype(np.float64)
+        out["max_abs_err_vs_oracle_sample"] = float(np.abs(est - ref).max())
+    return out
+
+
 def oracle_est64(wl):
     from oracle import oracle as O
     cov = O.make_cov(wl.cov["type"], wl.cov["ranges"], wl.cov["angles"], wl.cov["sill"], wl.cov["nugget"])
@@ -91,6 +156,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "
                          "scaling memory counters)")
@@ -199,6 +265,8 @@ def main():
         "roofline": roof(mixed, PEAK_FP32_TFLOPS, k_mixed),
         "flops_per_step": flops_all,
     }
+    if not args.no_loo:
+        line["loo"] = loo_bench(torch, not args.no_oracle)
     if args.calibrate_copy > 0:
         src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
