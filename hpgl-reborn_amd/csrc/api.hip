@@ -111,6 +111,8 @@ struct Library {
     std::mutex mu;
     std::string last_error;
     std::map<std::string, std::unique_ptr<DevPlan>> plans;
+    std::map<std::string, unsigned long long> plan_use;  // last use, for eviction
+    unsigned long long plan_clock = 0;
     Workspace ws;
     int (*out_handler)(char*, void*) = nullptr;
     void* out_param = nullptr;
@@ -172,15 +174,37 @@ CovParams to_cov(const hpgl_cov_params_t& p) {
     return c;
 }
 
+// Plans (host tables + their device copies) are cached per parameter set;
+// the least recently used one is dropped beyond kMaxPlans, after the last
+// call's work has finished with it.
+constexpr size_t kMaxPlans = 8;
+
 DevPlan* get_plan(Library& L, int kind, const CovParams& c, const int r[3], int kmax, std::string& err) {
     const std::string key = plan_key(kind, c, r, kmax);
+    ++L.plan_clock;
     auto it = L.plans.find(key);
-    if (it != L.plans.end()) return it->second.get();
+    if (it != L.plans.end()) {
+        L.plan_use[key] = L.plan_clock;
+        return it->second.get();
+    }
     std::unique_ptr<DevPlan> p(new DevPlan());
     if (!build_plan(p->h, (Kind)kind, c, r, kmax, err)) return nullptr;
+    if (L.plans.size() >= kMaxPlans) {
+        auto lru = L.plan_use.begin();
+        for (auto u = L.plan_use.begin(); u != L.plan_use.end(); ++u)
+            if (u->second < lru->second) lru = u;
+        if (L.done && hipEventSynchronize(L.done) != hipSuccess) {
+            err = "hipEventSynchronize failed";
+            return nullptr;
+        }
+        L.plans[lru->first]->release();
+        L.plans.erase(lru->first);
+        L.plan_use.erase(lru);
+    }
     if (!p->upload(err)) return nullptr;
     DevPlan* raw = p.get();
     L.plans[key] = std::move(p);
+    L.plan_use[key] = L.plan_clock;
     return raw;
 }
 

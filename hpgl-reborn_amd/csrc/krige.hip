@@ -466,20 +466,21 @@ struct WordStore {
         unsigned key;
         unsigned w;
     };
-    unsigned* buf;  // element i at buf[i * CL_BLOCK]
+    unsigned* buf;  // element i at buf[i * stride]
     int qbits;
-    __device__ unsigned key(int i) const { return buf[i * CL_BLOCK] >> qbits; }
+    int stride = CL_BLOCK;
+    __device__ unsigned key(int i) const { return buf[i * stride] >> qbits; }
     __device__ Item get(int i) const {
         Item it;
-        it.w = buf[i * CL_BLOCK];
+        it.w = buf[i * stride];
         it.key = it.w >> qbits;
         return it;
     }
-    __device__ void put(int i, const Item& it) { buf[i * CL_BLOCK] = it.w; }
+    __device__ void put(int i, const Item& it) { buf[i * stride] = it.w; }
     __device__ void swap(int i, int j) {
-        const unsigned a = buf[i * CL_BLOCK], b = buf[j * CL_BLOCK];
-        buf[i * CL_BLOCK] = b;
-        buf[j * CL_BLOCK] = a;
+        const unsigned a = buf[i * stride], b = buf[j * stride];
+        buf[i * stride] = b;
+        buf[j * stride] = a;
     }
 };
 
@@ -592,6 +593,97 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
             a.count[cell] = n | CNT_CLUSTER;
         }
         __syncthreads();
+    }
+}
+
+// Cluster path for a cell list (the overflow of k_search_cluster_oct): one
+// lane per cell, the candidates gathered from the 27 clusters' node lists in
+// global memory, rank keys as above, and a deeper per-lane LDS sort buffer
+// (cap2 words; CL_LIST_BLOCK lanes per block).  Cells with more kept
+// candidates stay pending for k_search_cluster.
+constexpr int CL_LIST_BLOCK = 64;
+constexpr int kClusterListCap = 448;
+
+__global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(ClusterOctArgs a, const int* cells,
+                                                                           const int* ncells_p, int cap2,
+                                                                           int* ovf2, int* n_ovf2) {
+    extern __shared__ unsigned cll_lds[];
+    const GridDims& g = a.g;
+    const int tid = threadIdx.x;
+    unsigned* sbuf = cll_lds;  // cap2 x CL_LIST_BLOCK words, element i of lane t at i * CL_LIST_BLOCK + t
+    const int ncells = *ncells_p;
+    const int K = a.kmax;
+    const unsigned qmask = (1u << a.qbits) - 1u;
+    for (int item = blockIdx.x * CL_LIST_BLOCK + tid; item < ncells; item += gridDim.x * CL_LIST_BLOCK) {
+        const long long cell = cells[item];
+        if (a.count[cell] != CNT_PENDING_CLUSTER) continue;
+        int x, y, z;
+        cell_xyz(g, cell, x, y, z);
+        const int kx = x / g.rx, ky = y / g.ry, kz = z / g.rz;
+        int m = 0, q = 0;
+        bool over = false;
+        for (int k = kz - 1; k <= kz + 1; ++k)
+            for (int j = ky - 1; j <= ky + 1; ++j)
+                for (int i = kx - 1; i <= kx + 1; ++i) {
+                    if (i < 0 || j < 0 || k < 0 || i >= g.cx || j >= g.cy || k >= g.cz) continue;
+                    const int ci = (k * g.cy + j) * g.cx + i;
+                    const int c = a.cl_count[ci];
+                    const int* nodes = a.cl_nodes + (long long)ci * a.node_cap;
+                    for (int s0 = 0; s0 < c; s0 += CL_GATHER) {
+                        int nd[CL_GATHER];
+#pragma unroll
+                        for (int u = 0; u < CL_GATHER; ++u) nd[u] = nodes[s0 + u < c ? s0 + u : c - 1];
+                        unsigned rk[CL_GATHER];
+#pragma unroll
+                        for (int u = 0; u < CL_GATHER; ++u) {
+                            int nx_, ny_, nz_;
+                            cell_xyz(g, nd[u], nx_, ny_, nz_);
+                            const int ax = abs(nx_ - x), ay = abs(ny_ - y), az = abs(nz_ - z);
+                            rk[u] = a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax];
+                        }
+#pragma unroll
+                        for (int u = 0; u < CL_GATHER; ++u) {
+                            if (s0 + u < c && rk[u] != 0u) {
+                                // the word keeps the candidate's node id position: q = running index
+                                if (m < cap2)
+                                    sbuf[m * CL_LIST_BLOCK + tid] = (rk[u] << a.qbits) | (unsigned)(q + s0 + u);
+                                else
+                                    over = true;
+                                ++m;
+                            }
+                        }
+                    }
+                    q += c;
+                }
+        if (over) {
+            const int pos = atomicAdd(n_ovf2, 1);
+            ovf2[pos] = (int)cell;
+            continue;
+        }
+        WordStore st;
+        st.buf = sbuf + tid;
+        st.qbits = a.qbits;
+        st.stride = CL_LIST_BLOCK;
+        Introsort<WordStore> srt(st);
+        srt.sort(m);
+        const int n = m < K ? m : K;
+        for (int r = 0; r < n; ++r) {
+            // recover the node from its running index over the 27 clusters
+            int idx = (int)(sbuf[r * CL_LIST_BLOCK + tid] & qmask), nd = 0;
+            for (int k = kz - 1; k <= kz + 1; ++k)
+                for (int j = ky - 1; j <= ky + 1; ++j)
+                    for (int i = kx - 1; i <= kx + 1; ++i) {
+                        if (i < 0 || j < 0 || k < 0 || i >= g.cx || j >= g.cy || k >= g.cz) continue;
+                        const int ci = (k * g.cy + j) * g.cx + i;
+                        const int c = a.cl_count[ci];
+                        if (idx >= 0 && idx < c) nd = a.cl_nodes[(long long)ci * a.node_cap + idx];
+                        idx -= c;
+                    }
+            int nx_, ny_, nz_;
+            cell_xyz(g, nd, nx_, ny_, nz_);
+            a.nboff[cell * K + r] = pack_off(nx_ - x, ny_ - y, nz_ - z);
+        }
+        a.count[cell] = n | CNT_CLUSTER;
     }
 }
 
@@ -814,6 +906,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     while ((1LL << qbits) < 27LL * node_cap) ++qbits;
     const int cap1 = c.cluster_cap > 0 && c.cluster_cap < kClusterSortCap ? c.cluster_cap : kClusterSortCap;
     const size_t cl_lds = ((size_t)cap1 * CL_BLOCK + (size_t)27 * node_cap) * 4;
+    const int cap2 = c.cluster_cap > 0 && c.cluster_cap < kClusterListCap ? c.cluster_cap : kClusterListCap;
     const bool oct_cluster = plan.rank_w && c.solve_variant != 1 && c.nx <= 2048 && c.ny <= 2048 && c.nz <= 1024 &&
                              qbits < 32 && (unsigned long long)h.rank_max < (1ULL << (32 - qbits)) &&
                              cl_lds <= kSearchLdsMax;
@@ -827,7 +920,9 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     int* overflow = counters + 2;
     int* n_fallback = counters + 3;
     int* n_ovf = counters + 4;
+    int* n_ovf2 = counters + 5;
     auto* ovf_cells = (int*)ws.get("ovf_cells", (size_t)n * 4, err);
+    auto* ovf2_cells = (int*)ws.get("ovf2_cells", (size_t)n * 4, err);
     auto* fb_cells = (int*)ws.get("fb_cells", (size_t)n * 4, err);
     if (!err.empty()) return -1;
 
@@ -940,8 +1035,13 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         oa.n_ovf = n_ovf;
         if (cl_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_cluster_oct, (int)cl_lds));
         hipLaunchKernelGGL(k_search_cluster_oct, dim3((unsigned)std::min(ncl, 2048)), dim3(CL_BLOCK), cl_lds, st, oa);
-        ca.cl_cells = ovf_cells;
-        ca.n_cl_cells = n_ovf;
+        // its overflow: one lane per cell with a deeper LDS buffer
+        const size_t cll_lds = (size_t)cap2 * CL_LIST_BLOCK * 4;
+        if (cll_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_cluster_list_oct, (int)cll_lds));
+        hipLaunchKernelGGL(k_search_cluster_list_oct, dim3(1024), dim3(CL_LIST_BLOCK), cll_lds, st, oa,
+                           (const int*)ovf_cells, (const int*)n_ovf, cap2, ovf2_cells, n_ovf2);
+        ca.cl_cells = ovf2_cells;
+        ca.n_cl_cells = n_ovf2;
     }
     hipLaunchKernelGGL(k_search_cluster, dim3(nlanes / B), dim3(B), 0, st, ca);
 
