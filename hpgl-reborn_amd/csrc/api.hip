@@ -244,7 +244,8 @@ bool read_timing(const CallTiming& t, float ms[2]) {
            hipEventElapsedTime(&ms[1], t.ev[2], t.ev[3]) == hipSuccess;
 }
 
-int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err) {
+int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err,
+                const int* cats_host = nullptr) {
     if (!q) {
         err = "null request";
         return -1;
@@ -280,6 +281,8 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     call.out_nboff = q->out_nboff;
     call.pivot_tol = q->pivot_tol > 0 ? q->pivot_tol : 1e-3;
     call.refine_steps = q->refine_steps >= 0 ? q->refine_steps : 1;
+    call.lvm_mean = q->lvm_mean;
+    call.cats_host = cats_host;
     if (const char* sv = std::getenv("HPGLX_SOLVE")) call.solve_variant = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH")) call.search_variant = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH_HEAD")) call.search_head = std::atoi(sv);
@@ -308,6 +311,81 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
         return -1;
     }
     return rc;
+}
+
+bool same_ik_model(const hpgl_ik_params_t& a, const hpgl_ik_params_t& b) {
+    bool same = a.m_covariance_type == b.m_covariance_type && a.m_sill == b.m_sill && a.m_nugget == b.m_nugget &&
+                a.m_max_neighbours == b.m_max_neighbours;
+    for (int i = 0; i < 3; ++i)
+        same = same && a.m_ranges[i] == b.m_ranges[i] && a.m_angles[i] == b.m_angles[i] &&
+               a.m_radiuses[i] == b.m_radiuses[i];
+    return same;
+}
+
+// Indicator kriging over categories (indicator_kriging.h:88-161): one median-IK
+// style pass per group of categories sharing model and neighbourhood, each
+// writing its categories' probability rows, then the per-cell correction and
+// most probable category.
+int run_indicator(Library& L, const hpgl_ik_params_t* P, int ncat, const int dims[3], const unsigned char* data,
+                  const unsigned char* mask, unsigned char* out_cat, unsigned char* out_mask, float* out_prob,
+                  int precision, hipStream_t st, std::string& err) {
+    if (!P || !dims || !data || !mask) {
+        err = "indicator kriging: parameters, dims, data and mask are required";
+        return -1;
+    }
+    if (ncat < 1 || ncat > 256) {
+        err = "indicator kriging: the category count must be 1..256";
+        return -1;
+    }
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || n > 2147483647LL) {
+        err = "indicator kriging: grid dimensions must be positive with a volume within INT_MAX";
+        return -1;
+    }
+    float* prob = out_prob;
+    if (!prob) {
+        prob = (float*)L.ws.get("ik_prob", (size_t)ncat * (size_t)n * 4, err);
+        if (!prob) return -1;
+    }
+    std::vector<char> taken(ncat, 0);
+    for (int c = 0; c < ncat; ++c) {
+        if (taken[c]) continue;
+        std::vector<int> cats;
+        std::vector<double> marg;
+        for (int d = c; d < ncat && cats.size() < 16; ++d)
+            if (!taken[d] && same_ik_model(P[c], P[d])) {
+                taken[d] = 1;
+                cats.push_back(d);
+                marg.push_back(P[d].m_marginal_prob);
+            }
+        hpglx_grid_request_t q;
+        std::memset(&q, 0, sizeof q);
+        q.kind = KIND_MIK;
+        q.precision = precision;
+        for (int i = 0; i < 3; ++i) {
+            q.dims[i] = dims[i];
+            q.radiuses[i] = P[c].m_radiuses[i];
+            q.cov.m_ranges[i] = P[c].m_ranges[i];
+            q.cov.m_angles[i] = P[c].m_angles[i];
+        }
+        q.max_neighbours = P[c].m_max_neighbours;
+        q.cov.m_covariance_type = P[c].m_covariance_type;
+        q.cov.m_sill = P[c].m_sill;
+        q.cov.m_nugget = P[c].m_nugget;
+        q.ind = data;
+        q.nind = (int)cats.size();
+        q.marginal = marg.data();
+        q.mask = mask;
+        q.out_prob = prob;
+        q.refine_steps = -1;
+        if (run_request(L, &q, st, err, cats.data()) != 0) return -1;
+    }
+    if (ik_finalize(n, ncat, prob, out_cat, out_mask, st, err) != 0) return -1;
+    if (hipEventRecord(L.done, st) != hipSuccess) {
+        err = "hipEventRecord failed";
+        return -1;
+    }
+    return 0;
 }
 
 int fail(Library& L, const std::string& msg) {
@@ -561,6 +639,85 @@ void hpgl_median_ik(hpgl_ind_masked_array_t* in, hpgl_median_ik_params_t* p, hpg
     });
 }
 
+void hpgl_lvm_kriging(float* input_data, unsigned char* input_mask, hpgl_shape_t* input_data_shape, float* mean_data,
+                      hpgl_shape_t* mean_data_shape, hpgl_ok_params_t* p, float* output_data, unsigned char* output_mask,
+                      hpgl_shape_t* output_data_shape) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(input_data_shape, err);
+    if (n < 0 || shape_volume(output_data_shape, err) != n || shape_volume(mean_data_shape, err) != n) {
+        L.last_error = err.empty() ? "input, mean and output sizes differ" : err;
+        return;
+    }
+    timed_call(L, "LVM Kriging", n, [&](std::string& e) -> int {
+        HostIO io;
+        if (!stage_in(L, io, n, input_data, input_mask, nullptr, e)) return -1;
+        float* d_mean = (float*)L.ws.get("io_lvm_mean", (size_t)n * 4, e);
+        if (!d_mean) return -1;
+        if (hipMemcpyAsync(d_mean, mean_data, (size_t)n * 4, hipMemcpyHostToDevice, L.io_stream) != hipSuccess) {
+            e = "host to device copy failed";
+            return -1;
+        }
+        hpglx_grid_request_t q;
+        std::memset(&q, 0, sizeof q);
+        q.kind = KIND_SK;
+        q.precision = L.default_precision;
+        for (int i = 0; i < 3; ++i) {
+            q.dims[i] = input_data_shape->m_data[i];
+            q.radiuses[i] = p->m_radiuses[i];
+            q.cov.m_ranges[i] = p->m_ranges[i];
+            q.cov.m_angles[i] = p->m_angles[i];
+        }
+        q.max_neighbours = p->m_max_neighbours;
+        q.cov.m_covariance_type = p->m_covariance_type;
+        q.cov.m_sill = p->m_sill;
+        q.cov.m_nugget = p->m_nugget;
+        q.values = io.d_val;
+        q.mask = io.d_mask;
+        q.lvm_mean = d_mean;
+        q.out_val = io.d_out;
+        q.out_mask = io.d_out_mask;
+        q.refine_steps = -1;
+        if (run_request(L, &q, L.io_stream, e) != 0) return -1;
+        if (hipMemcpyAsync(output_data, io.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipMemcpyAsync(output_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipStreamSynchronize(L.io_stream) != hipSuccess) {
+            e = "device to host copy failed";
+            return -1;
+        }
+        return 0;
+    });
+}
+
+void hpgl_indicator_kriging(hpgl_ind_masked_array_t* in, hpgl_ind_masked_array_t* out, hpgl_ik_params_t* params,
+                            int indicator_count) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(&in->m_shape, err);
+    if (n < 0 || shape_volume(&out->m_shape, err) != n) {
+        L.last_error = err.empty() ? "hpgl_indicator_kriging: input and output size mismatch" : err;
+        return;
+    }
+    timed_call(L, "Indicator Kriging", n, [&](std::string& e) -> int {
+        HostIO io;
+        if (!stage_in(L, io, n, nullptr, in->m_mask, in->m_data, e)) return -1;
+        if (run_indicator(L, params, indicator_count, in->m_shape.m_data, io.d_ind, io.d_mask, io.d_out_class,
+                          io.d_out_mask, nullptr, L.default_precision, L.io_stream, e) != 0)
+            return -1;
+        if (hipMemcpyAsync(out->m_data, io.d_out_class, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipMemcpyAsync(out->m_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
+            hipStreamSynchronize(L.io_stream) != hipSuccess) {
+            e = "device to host copy failed";
+            return -1;
+        }
+        return 0;
+    });
+}
+
 int hpgl_simple_kriging_weights(float* center, float* xs, float* ys, float* zs, int n, hpgl_cov_params_t* params,
                                 float* weights) {
     Library& L = lib();
@@ -682,6 +839,46 @@ int hpglx_indicator_cdf(const hpgl_cov_params_t* cov, const int radiuses[3], int
     q.order_relations = 1;
     q.refine_steps = -1;
     return hpglx_krige(&q, stream);
+}
+
+int hpglx_lvm_kriging(const hpgl_cov_params_t* cov, const int radiuses[3], int max_nb, const int dims[3],
+                      const float* values, const unsigned char* mask, const float* mean_field, float* out_val,
+                      unsigned char* out_mask, double* out_var, int precision, void* stream) {
+    if (!mean_field) {
+        lib().last_error = "hpglx_lvm_kriging: the mean field is required";
+        return -1;
+    }
+    hpglx_grid_request_t q;
+    std::memset(&q, 0, sizeof q);
+    q.kind = KIND_SK;
+    q.precision = precision;
+    for (int i = 0; i < 3; ++i) {
+        q.dims[i] = dims[i];
+        q.radiuses[i] = radiuses[i];
+    }
+    q.max_neighbours = max_nb;
+    q.cov = *cov;
+    q.values = values;
+    q.mask = mask;
+    q.lvm_mean = mean_field;
+    q.out_val = out_val;
+    q.out_mask = out_mask;
+    q.out_var = out_var;
+    q.refine_steps = -1;
+    return hpglx_krige(&q, stream);
+}
+
+int hpglx_indicator_kriging(const hpgl_ik_params_t* params, int ncat, const int dims[3], const unsigned char* data,
+                            const unsigned char* mask, unsigned char* out_cat, unsigned char* out_mask,
+                            float* out_prob, int precision, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (run_indicator(L, params, ncat, dims, data, mask, out_cat, out_mask, out_prob, precision, (hipStream_t)stream,
+                      err) != 0)
+        return fail(L, err);
+    return 0;
 }
 
 void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz) {

@@ -50,6 +50,53 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// Indicator kriging over categories, last step per cell
+// (indicator_kriging.h:34-59, 141-147; cdf_utils.cpp most_probable_category):
+// clamp the category probabilities to [0,1], one forward pass replacing each
+// decreasing adjacent pair by its average (float sum halved in double), then
+// the first category of largest probability.  Probabilities are rewritten
+// corrected; the rows are categories, N cells each.
+__global__ void k_ik_finalize(long long n, int ncat, float* prob, unsigned char* out_cat, unsigned char* out_mask) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto clamp01 = [](float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); };
+    float prev = clamp01(prob[i]);
+    double best = 0.0;
+    int best_k = 0;
+    for (int k = 1; k <= ncat; ++k) {
+        float cur = 0.0f;
+        if (k < ncat) {
+            cur = clamp01(prob[(long long)k * n + i]);
+            if (prev > cur) {
+                const float avg = (float)((double)(prev + cur) / 2.0);
+                prev = avg;
+                cur = avg;
+            }
+        }
+        // entry k - 1 is final once the pair (k - 1, k) has been seen
+        prob[(long long)(k - 1) * n + i] = prev;
+        if (k == 1 || (double)prev > best) {
+            best = prev;
+            best_k = k - 1;
+        }
+        prev = cur;
+    }
+    if (out_cat) out_cat[i] = (unsigned char)best_k;
+    if (out_mask) out_mask[i] = 1;
+}
+
+int ik_finalize(long long n, int ncat, float* prob, unsigned char* out_cat, unsigned char* out_mask, hipStream_t st,
+                std::string& err) {
+    const long long blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_ik_finalize, dim3((unsigned)blocks), dim3(256), 0, st, n, ncat, prob, out_cat, out_mask);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        err = std::string("indicator kriging finalize launch: ") + hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
+}
+
 __device__ __forceinline__ void cell_xyz(const GridDims& g, long long cell, int& x, int& y, int& z) {
     const long long nxy = (long long)g.nx * g.ny;
     z = (int)(cell / nxy);
@@ -857,6 +904,14 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         err = "median IK needs between 1 and 16 thresholds";
         return -1;
     }
+    if (c.cats_host && (c.kind != KIND_MIK || !c.ind || c.zfield)) {
+        err = "indicator categories need indicator data";
+        return -1;
+    }
+    if (c.lvm_mean && c.kind != KIND_SK) {
+        err = "a locally varying mean applies to simple kriging only";
+        return -1;
+    }
     GridDims g;
     g.nx = c.nx;
     g.ny = c.ny;
@@ -897,6 +952,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     auto* pcnt = (int*)ws.get("mean_pcnt", mean_blocks * 4, err);
     auto* dmean = (double*)ws.get("mean", 8, err);
     auto* dmarg = (double*)ws.get("marg", 16 * 8, err);
+    auto* dcats = (int*)ws.get("cats", 16 * 4, err);
     // Cluster-path scratch: every candidate comes from one of 27 clusters
     // holding at most `limit` nodes each.
     const int cap = (int)(27 * (h.limit > 0 ? h.limit : 1));
@@ -965,6 +1021,11 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         double hm[16] = {0};
         for (int k = 0; k < c.nind; ++k) hm[k] = c.marg_host[k];
         HPGLX_CHECK(hipMemcpyAsync(dmarg, hm, sizeof(hm), hipMemcpyHostToDevice, st));
+        if (c.cats_host) {
+            int hc[16] = {0};
+            for (int k = 0; k < c.nind; ++k) hc[k] = c.cats_host[k];
+            HPGLX_CHECK(hipMemcpyAsync(dcats, hc, sizeof(hc), hipMemcpyHostToDevice, st));
+        }
     }
 
     SearchArgs sa;
@@ -1058,6 +1119,8 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.mean_dev = (c.kind == KIND_SK && c.auto_mean) ? dmean : nullptr;
     so.mean = c.mean;
     so.marg = dmarg;
+    so.cats = c.cats_host ? dcats : nullptr;
+    so.lvm_mean = c.lvm_mean;
     so.count = count;
     so.nboff = nboff;
     so.ctab = plan.ctab;

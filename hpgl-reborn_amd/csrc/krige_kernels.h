@@ -86,6 +86,8 @@ struct SolveArgs {
     const double* mean_dev;        // SK automatic mean (device), or null
     double mean;                   // SK explicit mean
     const double* marg;            // MIK: nind marginal probabilities (device)
+    const int* cats;               // IK categories: indicator k is (ind == cats[k]) (device), else null
+    const float* lvm_mean;         // SK with a locally varying mean (N), else null
     const int* count;
     const int* nboff;
     const double* ctab;

@@ -54,6 +54,9 @@ struct GridCall {
     const float* thresholds = nullptr;      // median IK CDF mode: thresholds (device, nind)
     int nind = 0;
     const double* marg_host = nullptr;      // median IK marginal P(I=1) per threshold (host, nind)
+    const int* cats_host = nullptr;         // IK categories (host, nind): indicator k = (ind == cats[k]),
+                                            // float accumulation, probability row cats[k] of out_prob
+    const float* lvm_mean = nullptr;        // SK with a locally varying mean (device, N)
     int auto_mean = 0;                      // SK
     double mean = 0.0;                      // SK explicit mean
     float* out_val = nullptr;
@@ -85,5 +88,10 @@ struct CallTiming {
 
 int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& call, hipStream_t stream, CallTiming* timing,
                 std::string& err);
+
+// Order relations and most probable category over `ncat` probability rows of
+// n cells (indicator kriging over categories); writes category and mask = 1.
+int ik_finalize(long long n, int ncat, float* prob, unsigned char* out_cat, unsigned char* out_mask,
+                hipStream_t stream, std::string& err);
 
 }  // namespace hpglx

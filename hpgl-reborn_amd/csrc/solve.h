@@ -494,16 +494,21 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
 
     if (KIND != 2) {
         const double mean = (KIND == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
-        const float mf = (float)mean;
+        // A locally varying mean (lvm_kriging.cpp:16-39) gives the centre
+        // and each neighbour its own mean; combiner.h:10-25 either way.
+        const bool lvm = KIND == 1 && a.lvm_mean != nullptr;
+        const float mf = lvm ? a.lvm_mean[cell] : (float)mean;
         float term[S];
         double pe = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const bool live = G * s + t < n;
-            const float v0 = a.values[cell + (live ? dnode[s] : 0)];
+            const long long node = cell + (live ? dnode[s] : 0);
+            const float v0 = a.values[node];
             const float v = live ? v0 : 0.0f;
-            term[s] = (float)(w[s] * (double)(v - mf));
-            pe += w[s] * ((double)v - (double)mf);
+            const float ms = lvm ? (live ? a.lvm_mean[node] : 0.0f) : mf;
+            term[s] = (float)(w[s] * (double)(v - ms));
+            pe += w[s] * ((double)v - (double)ms);
         }
         float acc = mf;
 #pragma unroll
@@ -537,10 +542,14 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
         return;
     }
 
-    // Median indicator kriging: one weight set, one combine per threshold.
+    // Median indicator kriging: one weight set, one combine per threshold
+    // (median_ik.cpp:60-78, accumulated in double).  With categories
+    // (indicator_kriging.h:88-161) indicator k is (data == cats[k]), the sum
+    // is accumulated in float and lands in probability row cats[k].
     for (int k = 0; k < a.nind; ++k) {
         const float pf = (float)a.marg[k];
         const float thr = a.zfield ? a.thresholds[k] : 0.f;
+        const int cat = a.cats ? a.cats[k] : -1;
         double term[S];
         double pe = 0.0;
 #pragma unroll
@@ -550,6 +559,8 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
             float iv;
             if (a.zfield)
                 iv = (a.zfield[node] <= thr) ? 1.f : 0.f;
+            else if (cat >= 0)
+                iv = ((int)a.ind[node] == cat) ? 1.f : 0.f;
             else
                 iv = (a.ind[(long long)k * g.n + node] == 1) ? 1.f : 0.f;
             iv = live ? iv : 0.f;
@@ -557,16 +568,30 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
             pe += w[s] * ((double)iv - (double)pf);
         }
         double acc = (double)pf;
+        float accf = pf;
+        if (cat >= 0) {
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i) {
-            const double ti = gbc<G>(term[i / G], i % G);
-            if (i < n) acc += ti;
+            for (int i = 0; i < NMAX; ++i) {
+                const double ti = gbc<G>(term[i / G], i % G);
+                if (i < n) accf += (float)ti;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i) {
+                const double ti = gbc<G>(term[i / G], i % G);
+                if (i < n) acc += ti;
+            }
         }
         const double e64 = (double)pf + gsum<G>(pe);
+        const long long row = (long long)(cat >= 0 ? cat : k) * g.n + cell;
         if (t == 0) {
-            if (a.out_est64) a.out_est64[(long long)k * g.n + cell] = e64;
-            if (a.out_class) a.out_class[(long long)k * g.n + cell] = ((float)acc <= 0.5f) ? 0 : 1;
-            if (a.out_prob) a.out_prob[(long long)k * g.n + cell] = (float)acc;
+            if (a.out_est64) a.out_est64[row] = e64;
+            if (cat >= 0) {
+                if (a.out_prob) a.out_prob[row] = accf;
+            } else {
+                if (a.out_class) a.out_class[row] = ((float)acc <= 0.5f) ? 0 : 1;
+                if (a.out_prob) a.out_prob[row] = (float)acc;
+            }
         }
     }
     if (t == 0) {
@@ -583,7 +608,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
 #pragma unroll
             for (int k = 0; k + 1 < KMAXIND; ++k)
                 if (k + 1 < a.nind && p[k] > p[k + 1]) {
-                    const double avg = ((double)p[k] + (double)p[k + 1]) / 2.0;
+                    const double avg = (double)(p[k] + p[k + 1]) / 2.0;  // float sum, as the reference
                     p[k] = (float)avg;
                     p[k + 1] = (float)avg;
                 }

@@ -14,7 +14,7 @@ import ctypes as C
 
 import numpy
 
-from .hpgl_wrap import _HPGLX_GRID_REQUEST, _hpgl_so
+from .hpgl_wrap import _HPGL_IK_PARAMS, _HPGLX_GRID_REQUEST, _hpgl_so
 
 FP64 = 0
 MIXED = 1
@@ -60,12 +60,13 @@ def _dims(grid):
 
 def krige(kind, grid, radiuses, max_neighbours, cov_model, mask, values=None, indicators=None, zfield=None,
           thresholds=None, marginal=None, mean=None, precision="fp64", variance=True, est64=False, status=False,
-          neighbours=False, order_relations=False, pivot_tol=1e-3, refine_steps=-1, stream=None):
+          neighbours=False, order_relations=False, pivot_tol=1e-3, refine_steps=-1, stream=None, lvm_mean=None):
     """Grid kriging on device tensors; returns a dict of result tensors.
 
     kind: KIND_OK, KIND_SK or KIND_MIK.  For MIK give ``indicators`` (uint8,
     nind x N, value 1 = indicator set) or ``zfield`` + ``thresholds``
     (indicator_k = z <= thresholds[k]), and ``marginal`` (nind floats).
+    For SK, ``lvm_mean`` (float32, N) gives every cell its own mean.
     """
     torch = _torch()
     nx, ny, nz = _dims(grid)
@@ -121,6 +122,10 @@ def krige(kind, grid, radiuses, max_neighbours, cov_model, mask, values=None, in
     if kind == KIND_SK:
         q.auto_mean = 1 if mean is None else 0
         q.mean = 0.0 if mean is None else float(mean)
+    if lvm_mean is not None:
+        _require(kind == KIND_SK, "a locally varying mean applies to simple kriging only")
+        _check_dev(lvm_mean, torch.float32, n, "lvm_mean")
+        q.lvm_mean = _ptr(lvm_mean)
     out["mask"] = torch.empty(n, dtype=torch.uint8, device=dev)
     q.out_mask = _ptr(out["mask"])
     if variance:
@@ -159,6 +164,53 @@ def simple_kriging(values, mask, grid, radiuses, max_neighbours, cov_model, mean
     """SK on device tensors (work of geo.simple_kriging, geo.py:609-649)."""
     return krige(KIND_SK, grid, radiuses, max_neighbours, cov_model, mask, values=values, mean=mean,
                  precision=precision, **kw)
+
+
+def lvm_kriging(values, mask, mean_data, grid, radiuses, max_neighbours, cov_model, precision="fp64", **kw):
+    """SK with a locally varying mean on device tensors (work of
+    geo.lvm_kriging, geo.py:651-696; lvm_kriging.cpp:16-39)."""
+    return krige(KIND_SK, grid, radiuses, max_neighbours, cov_model, mask, values=values, lvm_mean=mean_data,
+                 precision=precision, **kw)
+
+
+def indicator_kriging(data, mask, grid, categories, marginal_probs, precision="fp64", stream=None):
+    """Indicator kriging over categories on device tensors (work of
+    hpgl_indicator_kriging, api.cpp:457-482; indicator_kriging.h:88-161).
+
+    ``data``: uint8 category per cell; ``categories``: one dict per category
+    with cov_model, radiuses and max_neighbours (as geo.indicator_kriging).
+    Returns dict(category, mask, prob) where prob (ncat x N) holds the
+    probabilities after the order-relation correction.  Unlike
+    geo.indicator_kriging, two categories are kriged the same way as more
+    (the reference sends them to median_ik).
+    """
+    torch = _torch()
+    nx, ny, nz = _dims(grid)
+    n = nx * ny * nz
+    ncat = len(categories)
+    _require(1 <= ncat <= 256, "1..256 categories are supported")
+    _require(len(marginal_probs) == ncat, "one marginal probability per category is required")
+    _check_dev(mask, torch.uint8, n, "mask")
+    _check_dev(data, torch.uint8, n, "data")
+    dev = mask.device
+    params = (_HPGL_IK_PARAMS * ncat)()
+    for i, c in enumerate(categories):
+        cm = c["cov_model"]
+        params[i] = _HPGL_IK_PARAMS(covariance_type=int(cm.type), ranges=tuple(float(r) for r in cm.ranges),
+                                    angles=tuple(float(a) for a in cm.angles), sill=float(cm.sill),
+                                    nugget=float(cm.nugget), radiuses=tuple(int(r) for r in c["radiuses"]),
+                                    max_neighbours=int(c["max_neighbours"]), marginal_prob=float(marginal_probs[i]))
+    out = {"category": torch.empty(n, dtype=torch.uint8, device=dev),
+           "mask": torch.empty(n, dtype=torch.uint8, device=dev),
+           "prob": torch.empty((ncat, n), dtype=torch.float32, device=dev)}
+    if stream is None:
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = _hpgl_so.hpglx_indicator_kriging(params, ncat, (C.c_int * 3)(nx, ny, nz), _ptr(data), _ptr(mask),
+                                          _ptr(out["category"]), _ptr(out["mask"]), _ptr(out["prob"]),
+                                          _PREC[precision], C.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("hpglx_indicator_kriging: " + _hpgl_so.hpglx_last_error().decode())
+    return out
 
 
 def indicator_cdf(zfield, mask, grid, thresholds, marginal, radiuses, max_neighbours, cov_model, precision="fp64",

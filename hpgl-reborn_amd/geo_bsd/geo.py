@@ -18,14 +18,14 @@ import ctypes as C
 import numpy
 
 from . import validation
-from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_MEDIAN_IK_PARAMS, _HPGL_OK_PARAMS,
-                        _HPGL_SHAPE, _HPGL_SK_PARAMS, _HPGL_COV_PARAMS, _hpgl_so, hpgl_output_handler,
-                        hpgl_progress_handler, last_error)
+from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_IK_PARAMS, _HPGL_MEDIAN_IK_PARAMS,
+                        _HPGL_OK_PARAMS, _HPGL_SHAPE, _HPGL_SK_PARAMS, _HPGL_COV_PARAMS, _hpgl_so,
+                        hpgl_output_handler, hpgl_progress_handler, last_error)
 from .validation import (CriticalValidationError, GridValidator, ParameterValidator, ValidationError,
                          ValidationWarning)
 
 __all__ = ["ContProperty", "IndProperty", "SugarboxGrid", "CovarianceModel", "covariance", "ordinary_kriging",
-           "simple_kriging", "median_ik", "simple_kriging_weights", "calc_mean", "set_output_handler",
+           "simple_kriging", "lvm_kriging", "median_ik", "indicator_kriging", "simple_kriging_weights", "calc_mean", "set_output_handler",
            "set_progress_handler", "set_thread_num", "get_thread_num", "append_mask", "ValidationError",
            "CriticalValidationError", "ValidationWarning"]
 
@@ -260,6 +260,74 @@ def median_ik(prop, grid, marginal_probs, radiuses, max_neighbours, cov_model):
                                   indicator_count=out_prop.indicator_count)
     _hpgl_so.hpgl_median_ik(C.byref(inp), C.byref(mp), C.byref(outp))
     _check_native("median_ik")
+    return out_prop
+
+
+@accepts_tuple("prop", 0)
+def lvm_kriging(prop, grid, mean_data, radiuses, max_neighbours, cov_model):
+    """Simple kriging with a locally varying mean (geo.py:651-696):
+    mean_data holds one mean per cell, x fastest."""
+    valid_radiuses = _validate_common(grid, radiuses, max_neighbours, cov_model)
+    if not isinstance(mean_data, numpy.ndarray):
+        raise ValueError("lvm_kriging: mean_data must be a numpy array")
+    mean_data = numpy.require(mean_data, "float32", ["F", "W", "A"])
+    if mean_data.size != grid.x * grid.y * grid.z:
+        raise ValueError("lvm_kriging: mean_data must hold one value per grid cell")
+    out_prop = _clone_prop(prop)
+    okp = _HPGL_OK_PARAMS(covariance_type=cov_model.type, ranges=tuple(cov_model.ranges),
+                          angles=tuple(cov_model.angles), sill=cov_model.sill, nugget=cov_model.nugget,
+                          radiuses=valid_radiuses, max_neighbours=max_neighbours)
+    sh = _grid_arrays(prop, grid)
+    _hpgl_so.hpgl_lvm_kriging(prop.data, prop.mask, C.byref(sh), mean_data, C.byref(sh), C.byref(okp),
+                              out_prop.data, out_prop.mask, C.byref(sh))
+    _check_native("lvm_kriging")
+    return out_prop
+
+
+def _ik_params(data, marginal_probs):
+    """geo.py:740-757: one _HPGL_IK_PARAMS per category."""
+    n = len(data)
+    arr = (_HPGL_IK_PARAMS * n)()
+    for i, ikd in enumerate(data):
+        cm = ikd["cov_model"]
+        arr[i] = _HPGL_IK_PARAMS(covariance_type=cm.type, ranges=tuple(cm.ranges), angles=tuple(cm.angles),
+                                 sill=cm.sill, nugget=cm.nugget, radiuses=tuple(int(r) for r in ikd["radiuses"]),
+                                 max_neighbours=ikd["max_neighbours"], marginal_prob=marginal_probs[i])
+    return arr
+
+
+@accepts_tuple("prop", 0)
+def indicator_kriging(prop, grid, data, marginal_probs):
+    """Indicator kriging over categories (geo.py:759-800).  `data` holds one
+    dict per category: cov_model, radiuses, max_neighbours.  Two categories
+    go to median_ik with marginal_probs (p0, 1 - p0), as in the reference."""
+    GridValidator.validate_grid_dimensions(grid.x, grid.y, grid.z)
+    ParameterValidator.validate_indicator_count(len(data))
+    if len(marginal_probs) != len(data):
+        raise ValueError(f"indicator_kriging: marginal_probs length ({len(marginal_probs)}) must match data "
+                         f"length ({len(data)})")
+    for i, p in enumerate(marginal_probs):
+        ParameterValidator.validate_probability(p, f"marginal_probs[{i}]")
+    for i, ikd in enumerate(data):
+        ParameterValidator.validate_radius(ikd["radiuses"], f"data[{i}].radiuses")
+        ParameterValidator.validate_max_neighbors(ikd["max_neighbours"])
+        cm = ikd["cov_model"]
+        ParameterValidator.validate_covariance_parameters(cm.sill, cm.nugget, cm.ranges, cm.angles)
+    for i in range(len(data)):
+        data[i]["marginal_prob"] = marginal_probs[i]
+    if len(data) == 2:
+        return median_ik(prop, grid, (data[0]["marginal_prob"], 1 - data[0]["marginal_prob"]), data[0]["radiuses"],
+                         data[0]["max_neighbours"], data[0]["cov_model"])
+    out_prop = _clone_prop(prop)
+    sh = _shape((grid.x, grid.y, grid.z))
+    inp = _HPGL_IND_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                 mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                 indicator_count=prop.indicator_count)
+    outp = _HPGL_IND_MASKED_ARRAY(data=out_prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                  mask=out_prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                  indicator_count=out_prop.indicator_count)
+    _hpgl_so.hpgl_indicator_kriging(C.byref(inp), C.byref(outp), _ik_params(data, marginal_probs), len(data))
+    _check_native("indicator_kriging")
     return out_prop
 
 

@@ -46,6 +46,12 @@ class _HPGL_MEDIAN_IK_PARAMS(C.Structure):
                 ("max_neighbours", C.c_int), ("marginal_probs", C.c_double * 2)]
 
 
+class _HPGL_IK_PARAMS(C.Structure):
+    _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double), ("radiuses", C.c_int * 3),
+                ("max_neighbours", C.c_int), ("marginal_prob", C.c_double)]
+
+
 class __hpgl_cov_params_t(C.Structure):
     _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
                 ("sill", C.c_double), ("nugget", C.c_double)]
@@ -65,7 +71,7 @@ class _HPGLX_GRID_REQUEST(C.Structure):
         ("out_val", C.c_void_p), ("out_mask", C.c_void_p), ("out_class", C.c_void_p), ("out_prob", C.c_void_p),
         ("out_var", C.c_void_p), ("out_est64", C.c_void_p), ("out_status", C.c_void_p),
         ("order_relations", C.c_int), ("out_count", C.c_void_p), ("out_nboff", C.c_void_p),
-        ("pivot_tol", C.c_double), ("refine_steps", C.c_int)]
+        ("pivot_tol", C.c_double), ("refine_steps", C.c_int), ("lvm_mean", C.c_void_p)]
 
 
 LIB_NAME = "libhpgl.so"
@@ -120,6 +126,19 @@ _hpgl_so.hpgl_median_ik.argtypes = [C.POINTER(_HPGL_IND_MASKED_ARRAY), C.POINTER
 
 _hpgl_so.hpglx_krige.restype = C.c_int
 _hpgl_so.hpglx_krige.argtypes = [C.POINTER(_HPGLX_GRID_REQUEST), C.c_void_p]
+_hpgl_so.hpgl_lvm_kriging.restype = None
+_hpgl_so.hpgl_lvm_kriging.argtypes = [
+    NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]), NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
+    C.POINTER(_HPGL_SHAPE), NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]), C.POINTER(_HPGL_SHAPE),
+    C.POINTER(_HPGL_OK_PARAMS),
+    NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]), NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
+    C.POINTER(_HPGL_SHAPE)]
+_hpgl_so.hpgl_indicator_kriging.restype = None
+_hpgl_so.hpgl_indicator_kriging.argtypes = [C.POINTER(_HPGL_IND_MASKED_ARRAY), C.POINTER(_HPGL_IND_MASKED_ARRAY),
+                                            C.POINTER(_HPGL_IK_PARAMS), C.c_int]
+_hpgl_so.hpglx_indicator_kriging.restype = C.c_int
+_hpgl_so.hpglx_indicator_kriging.argtypes = [C.POINTER(_HPGL_IK_PARAMS), C.c_int, C.c_int * 3, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
 _hpgl_so.hpglx_template.restype = C.c_int
 _hpgl_so.hpglx_template.argtypes = [C.POINTER(__hpgl_cov_params_t), C.c_int * 3, C.c_void_p, C.c_int]
 _hpgl_so.hpglx_set_profiling.restype = None

@@ -105,6 +105,49 @@ def ok_wells_c4(scale=1.0, seed=4):
                     radiuses=(30, 30, 6), k=32)
 
 
+def lvm_c1(scale=1.0, seed=6):
+    """Kriging with a locally varying mean, 50x50x10, 2% hard data,
+    spherical (10,10,5), k=12: the mean trend
+    0.20 + 0.05 sin(x/8) cos(y/11) + 0.004 z plus a 7x7x3 moving-average
+    residual of sd 0.03."""
+    shape = (max(4, int(50 * scale)), max(4, int(50 * scale)), max(2, int(10 * scale)))
+    nx, ny, nz = shape
+    rng = numpy.random.default_rng(seed)
+    n = nx * ny * nz
+    x, y, z = numpy.meshgrid(numpy.arange(nx), numpy.arange(ny), numpy.arange(nz), indexing="ij")
+    trend = (0.20 + 0.05 * numpy.sin(x / 8.0) * numpy.cos(y / 11.0) + 0.004 * z).astype(numpy.float32)
+    trend = trend.reshape(-1, order="F")
+    f = trend + moving_average_field(rng, shape, (7, 7, 3), 0.0, 0.03)
+    m = random_mask(rng, n, 0.02)
+    return Workload(name="lvm_50x50x10", kind="sk", shape=shape, values=with_undefined(f, m), mask=m,
+                    lvm_mean=trend, cov=dict(type=0, ranges=(10.0, 10.0, 5.0), angles=(0.0, 0.0, 0.0), sill=1.0,
+                                             nugget=0.1),
+                    radiuses=(10, 10, 5), k=12)
+
+
+def ik_c3(scale=1.0, seed=7):
+    """Indicator kriging over 4 categories, 200x200x100, 1% hard data: the
+    categories are the quartile classes of a 9x9x3 moving-average field;
+    spherical models of ranges (8,8,2), (12,12,3), (6,6,2) and again
+    (12,12,3) (categories 1 and 3 share a model), sill 0.25, nugget 0.02,
+    radiuses equal to the ranges, k = 16; marginals are the hard-data
+    proportions."""
+    shape = (max(4, int(200 * scale)), max(4, int(200 * scale)), max(2, int(100 * scale)))
+    rng = numpy.random.default_rng(seed)
+    n = shape[0] * shape[1] * shape[2]
+    g = moving_average_field(rng, shape, (9, 9, 3), 0.0, 1.0)
+    m = random_mask(rng, n, 0.01)
+    cuts = numpy.quantile(g[m == 1], [0.25, 0.5, 0.75])
+    data = numpy.searchsorted(cuts, g, side="right").astype(numpy.uint8)
+    data[m == 0] = 0
+    ranges = [(8.0, 8.0, 2.0), (12.0, 12.0, 3.0), (6.0, 6.0, 2.0), (12.0, 12.0, 3.0)]
+    cats = [dict(cov=dict(type=0, ranges=r, angles=(0.0, 0.0, 0.0), sill=0.25, nugget=0.02),
+                 radiuses=tuple(int(v) for v in r), k=16) for r in ranges]
+    marg = [float((data[m == 1] == c).mean()) for c in range(4)]
+    return Workload(name="ik4_200x200x100", kind="ik", shape=shape, values=data, mask=m, categories=cats,
+                    marginal=marg)
+
+
 def loo_c5(n=20000, nsets=64, seed=5):
     """Leave-one-out kriging of scattered points: n float32 points uniform in
     [0,200) x [0,200) x [0,50), values of the 7x7x3 moving-average field

@@ -32,6 +32,7 @@ class ValidationConstants:
     MAX_ANGLE = 360.0
     MIN_PROBABILITY = 0.0
     MAX_PROBABILITY = 1.0
+    MAX_INDICATORS = 256
 
 
 class ValidationError(Exception):
@@ -146,6 +147,15 @@ class ParameterValidator:
                 if a < ValidationConstants.MIN_ANGLE or a > ValidationConstants.MAX_ANGLE:
                     validation_logger.warning(f"Angle[{i}] = {a} is outside typical range "
                                               f"[{ValidationConstants.MIN_ANGLE}, {ValidationConstants.MAX_ANGLE}]")
+
+    @staticmethod
+    def validate_indicator_count(count: int) -> None:
+        """validation.py:560-580: 1..MAX_INDICATORS categories."""
+        if count <= 0:
+            raise CriticalValidationError(f"Indicator count must be positive, got {count}", "indicator_count")
+        if count > ValidationConstants.MAX_INDICATORS:
+            raise CriticalValidationError(
+                f"Indicator count {count} exceeds maximum {ValidationConstants.MAX_INDICATORS}", "indicator_count")
 
     @staticmethod
     def validate_probability(prob: float, name: str = "probability") -> None:

@@ -81,6 +81,18 @@ struct hpgl_median_ik_params_t {
     double m_marginal_probs[2];
 };
 
+/* src/geo_bsd/hpgl/api.h:110-120 (one per category of indicator kriging) */
+struct hpgl_ik_params_t {
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+    int m_radiuses[3];
+    int m_max_neighbours;
+    double m_marginal_prob;
+};
+
 /* Replaces api.cpp:38-41.  Message of the last failed call ("" if none). */
 char* hpgl_get_last_exception_message(void);
 
@@ -116,6 +128,21 @@ int hpgl_simple_kriging_weights(float* center_coords, float* neighbours_x, float
  * (1 when the kriged P(data == 1) exceeds 0.5) and mask 1. */
 void hpgl_median_ik(struct hpgl_ind_masked_array_t* in_data, struct hpgl_median_ik_params_t* params,
                     struct hpgl_ind_masked_array_t* out_data);
+
+/* Replaces api.cpp:412-455 (geo.lvm_kriging, lvm_kriging.cpp:16-39): simple
+ * kriging where every cell has its own mean, mean_data (N floats, the
+ * input's shape).  Cells that cannot be kriged get their mean. */
+void hpgl_lvm_kriging(float* input_data, unsigned char* input_mask, struct hpgl_shape_t* input_data_shape,
+                      float* mean_data, struct hpgl_shape_t* mean_data_shape, struct hpgl_ok_params_t* params,
+                      float* output_data, unsigned char* output_mask, struct hpgl_shape_t* output_data_shape);
+
+/* Replaces api.cpp:457-482 (geo.indicator_kriging with more than two
+ * categories, indicator_kriging.h:88-161): per category c, simple kriging of
+ * (data == c) with mean params[c].m_marginal_prob and that category's model,
+ * radiuses and neighbour limit; the probabilities get the order-relation
+ * correction and every cell gets the first most probable category, mask 1. */
+void hpgl_indicator_kriging(struct hpgl_ind_masked_array_t* in_data, struct hpgl_ind_masked_array_t* out_data,
+                            struct hpgl_ik_params_t* params, int indicator_count);
 
 #ifdef __cplusplus
 }

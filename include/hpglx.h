@@ -62,6 +62,7 @@ typedef struct hpglx_grid_request_t {
     int* out_nboff;            /* N x max_neighbours packed offsets, see hpglx_unpack_offset */
     double pivot_tol;          /* MIXED: re-solve in FP64 when min pivot < tol * C(0); <= 0 -> 1e-3 */
     int refine_steps;          /* MIXED: FP64 residual refinement steps; < 0 -> 1 */
+    const float* lvm_mean;     /* SK: locally varying mean, one per cell (replaces auto_mean / mean) */
 } hpglx_grid_request_t;
 
 /* The work of hpgl_ordinary_kriging / hpgl_simple_kriging / hpgl_median_ik
@@ -83,6 +84,23 @@ int hpglx_indicator_cdf(const struct hpgl_cov_params_t* cov, const int radiuses[
                         const int dims[3], const float* zfield, const unsigned char* mask, int nthr,
                         const float* thresholds, const double* marginal, float* out_cdf, int precision,
                         void* stream);
+
+/* The work of hpgl_lvm_kriging (api.cpp:412-455, lvm_kriging.cpp:16-39):
+ * simple kriging with mean_field[cell] as each cell's mean. */
+int hpglx_lvm_kriging(const struct hpgl_cov_params_t* cov, const int radiuses[3], int max_neighbours,
+                      const int dims[3], const float* values, const unsigned char* mask, const float* mean_field,
+                      float* out_val, unsigned char* out_mask, double* out_var, int precision, void* stream);
+
+/* The work of hpgl_indicator_kriging (api.cpp:457-482,
+ * indicator_kriging.h:88-161) on device buffers.  `data` holds category
+ * indices; `params` is a host array of `ncat` (1..256) per-category models,
+ * neighbourhoods and marginal probabilities.  Categories with the same model,
+ * radiuses and neighbour limit share one search and one weight solve.
+ * out_prob (ncat x N, may be null) receives the probabilities after the
+ * order-relation correction; out_cat the first most probable category. */
+int hpglx_indicator_kriging(const struct hpgl_ik_params_t* params, int ncat, const int dims[3],
+                            const unsigned char* data, const unsigned char* mask, unsigned char* out_cat,
+                            unsigned char* out_mask, float* out_prob, int precision, void* stream);
 
 /* Decodes an entry of out_nboff into the neighbour's offset from the cell. */
 void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz);

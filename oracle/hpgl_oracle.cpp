@@ -18,7 +18,9 @@
 //   combine              src/geo_bsd/hpgl/combiner.h:10-25
 //   drivers              src/geo_bsd/hpgl/cont_kriging.h:39-129,
 //                        ordinary_kriging.cpp:14-48, simple_kriging.cpp:40-75,
-//                        median_ik.cpp:24-90, calc_mean.cpp:6-30
+//                        median_ik.cpp:24-90, calc_mean.cpp:6-30,
+//                        lvm_kriging.cpp:16-39, indicator_kriging.h:88-161
+//   most probable class  src/geo_bsd/hpgl/cdf_utils.cpp:11-27
 //   order relations      src/geo_bsd/hpgl/indicator_kriging.h:34-59
 //
 // The reference delegates three pieces of arithmetic to third-party code; the
@@ -454,11 +456,13 @@ double orc_cov(const orc_cov_t* cp, double dx, double dy, double dz) {
 //   out_path  N bytes, 1 where the cluster path was taken
 //   out_status N ints, see Status
 // Returns 0, or -1 on invalid parameters.
+//   lvm_mean  SK only, may be null: one mean per cell (lvm_kriging.cpp:16-39);
+//             the centre and every neighbour then use their own mean
 int orc_krige(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax, int nx, int ny, int nz,
               const float* values, const unsigned char* ind, int nind, const unsigned char* mask,
               const double* means, int auto_mean, double* mean_used, float* out_val,
               unsigned char* out_mask, double* out_prob, double* out_est64, double* out_var,
-              int* out_count, int* out_nb, unsigned char* out_path, int* out_status) {
+              int* out_count, int* out_nb, unsigned char* out_path, int* out_status, const float* lvm_mean) {
     Setup s;
     if (kmax < 1) return -1;
     if (!setup(s, kind, (const CovParams*)cp, radiuses, kmax, nx, ny, nz, mask)) return -1;
@@ -536,15 +540,17 @@ int orc_krige(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax, in
             if (out_mask) out_mask[idx] = 1;
             continue;
         }
-        float mf = kind == SK ? (float)mean : 0.0f;
+        const bool lvm = kind == SK && lvm_mean != nullptr;
+        float mf = kind == SK ? (lvm ? lvm_mean[idx] : (float)mean) : 0.0f;
         if (cr.status == ST_KRIGED) {
             // combiner.h:10-25 with result_t = float.
             float res = mf;
             double e64 = (double)mf;
             for (int i = 0; i < n; ++i) {
                 float v = values[nb[i]];
-                res += (float)(cr.w[i] * (v - mf));
-                e64 += cr.w[i] * ((double)v - (double)mf);
+                float mi = lvm ? lvm_mean[nb[i]] : mf;
+                res += (float)(cr.w[i] * (v - mi));
+                e64 += cr.w[i] * ((double)v - (double)mi);
             }
             if (out_val) out_val[idx] = res;
             if (out_mask) out_mask[idx] = 1;
@@ -583,6 +589,72 @@ void orc_order_relations(float* probs, int k, long long cells) {
                 p[i + 1] = (float)avg;
             }
     }
+}
+
+// Indicator kriging over `ncat` categories (indicator_kriging.h:88-161).
+// Category c: simple kriging of (data == c) with mean (float)marg[c], the
+// clipped table of covs[c] over radiuses[3c..3c+2], kmax[c] neighbours, every
+// cell searched (informed ones too); the sum is accumulated in float and a
+// cell that cannot be kriged gets the marginal.  Then, per cell, the order
+// relations (orc_order_relations) and the first category of largest
+// probability (cdf_utils.cpp:11-27).
+//   out_cat   N bytes, out_mask N bytes (set to 1)
+//   out_raw   ncat*N floats, probabilities before the correction (may be null)
+//   out_prob  ncat*N floats, after it (may be null)
+//   out_status ncat*N ints (may be null)
+// Returns 0, or -1 on invalid parameters.
+int orc_indicator_kriging(const unsigned char* data, const unsigned char* mask, int nx, int ny, int nz, int ncat,
+                          const orc_cov_t* covs, const int* radiuses, const int* kmax, const double* marg,
+                          unsigned char* out_cat, unsigned char* out_mask, float* out_raw, float* out_prob,
+                          int* out_status) {
+    if (ncat < 1) return -1;
+    const long long N = (long long)nx * ny * nz;
+    std::vector<float> prob((size_t)ncat * N);
+    for (int c = 0; c < ncat; ++c) {
+        Setup s;
+        if (kmax[c] < 1) return -1;
+        if (!setup(s, MIK, (const CovParams*)&covs[c], radiuses + 3 * c, kmax[c], nx, ny, nz, mask)) return -1;
+        const float mf = (float)marg[c];
+#pragma omp parallel for schedule(dynamic, 256)
+        for (long long idx = 0; idx < N; ++idx) {
+            std::vector<int> nb;
+            bool cpath = false;
+            find(s, (int)idx, nb, cpath);
+            CellResult cr;
+            if (nb.empty()) {
+                cr.status = ST_NO_NEIGHBOURS;
+            } else {
+                cr = weights(s, (int)idx, nb);
+            }
+            float p = mf;
+            if (cr.status == ST_KRIGED)
+                for (size_t i = 0; i < nb.size(); ++i) {
+                    float iv = data[nb[i]] == c ? 1.0f : 0.0f;
+                    p += (float)(cr.w[i] * (iv - mf));
+                }
+            prob[(size_t)c * N + idx] = p;
+            if (out_status) out_status[(size_t)c * N + idx] = cr.status;
+        }
+    }
+    if (out_raw) std::memcpy(out_raw, prob.data(), prob.size() * sizeof(float));
+#pragma omp parallel for schedule(static)
+    for (long long idx = 0; idx < N; ++idx) {
+        std::vector<float> p(ncat);
+        for (int c = 0; c < ncat; ++c) p[c] = prob[(size_t)c * N + idx];
+        orc_order_relations(p.data(), ncat, 1);
+        double best = p[0];
+        int best_c = 0;
+        for (int c = 1; c < ncat; ++c)
+            if ((double)p[c] > best) {
+                best = p[c];
+                best_c = c;
+            }
+        if (out_prob)
+            for (int c = 0; c < ncat; ++c) out_prob[(size_t)c * N + idx] = p[c];
+        if (out_cat) out_cat[idx] = (unsigned char)best_c;
+        if (out_mask) out_mask[idx] = 1;
+    }
+    return 0;
 }
 
 // Simple kriging weights at arbitrary float coordinates

@@ -89,8 +89,9 @@ def cov_value(cov, dx, dy, dz):
 
 
 def krige(kind, cov, radiuses, max_nb, shape, mask, values=None, indicators=None, means=None,
-          auto_mean=False, want_neighbours=True):
-    """Run the oracle over a whole grid.  Arrays are flat, x fastest."""
+          auto_mean=False, want_neighbours=True, lvm_mean=None):
+    """Run the oracle over a whole grid.  Arrays are flat, x fastest.
+    ``lvm_mean`` (SK only): one mean per cell, as lvm_kriging."""
     nx, ny, nz = shape
     n = nx * ny * nz
     mask = np.ascontiguousarray(mask, np.uint8).reshape(-1)
@@ -107,6 +108,8 @@ def krige(kind, cov, radiuses, max_nb, shape, mask, values=None, indicators=None
     if means is not None:
         m[:len(np.atleast_1d(means))] = np.atleast_1d(means)
     mean_used = C.c_double(0)
+    lvm = None if lvm_mean is None else np.ascontiguousarray(lvm_mean, np.float32).reshape(-1)
+    assert lvm is None or (kind == SK and lvm.size == n)
     out = dict(
         value=np.zeros(n, np.float32), mask=np.zeros(n, np.uint8),
         var=np.zeros(n, np.float64), count=np.zeros(n, np.int32),
@@ -122,7 +125,7 @@ def krige(kind, cov, radiuses, max_nb, shape, mask, values=None, indicators=None
         _p(m, C.c_double), C.c_int(1 if auto_mean else 0), C.byref(mean_used), _p(out["value"], C.c_float),
         _p(out["mask"], C.c_ubyte), _p(out.get("prob"), C.c_double), _p(out["est64"], C.c_double),
         _p(out["var"], C.c_double), _p(out["count"], C.c_int), _p(out.get("nb"), C.c_int),
-        _p(out["path"], C.c_ubyte), _p(out["status"], C.c_int))
+        _p(out["path"], C.c_ubyte), _p(out["status"], C.c_int), _p(lvm, C.c_float))
     if rc != 0:
         raise ValueError("oracle rejected the parameters")
     if want_neighbours:
@@ -131,6 +134,33 @@ def krige(kind, cov, radiuses, max_nb, shape, mask, values=None, indicators=None
         out["prob"] = out["prob"].reshape(nind, n)
         out["est64"] = out["est64"].reshape(nind, n)
     out["mean"] = mean_used.value
+    return out
+
+
+def indicator_kriging(shape, data, mask, cats):
+    """Indicator kriging over categories (indicator_kriging.h:88-161).
+    ``cats``: one dict per category with keys cov (CovT), radiuses, max_nb,
+    marginal.  Returns dict(cat, mask, raw, prob, status); raw/prob/status are
+    (ncat, N)."""
+    n = int(np.prod(shape))
+    k = len(cats)
+    d = np.ascontiguousarray(data, np.uint8).reshape(-1)
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+    assert d.size == n and m.size == n
+    covs = (CovT * k)(*[c["cov"] for c in cats])
+    rad = np.ascontiguousarray([r for c in cats for r in c["radiuses"]], np.int32)
+    kmax = np.ascontiguousarray([c["max_nb"] for c in cats], np.int32)
+    marg = np.ascontiguousarray([c["marginal"] for c in cats], np.float64)
+    out = dict(cat=np.zeros(n, np.uint8), mask=np.zeros(n, np.uint8), raw=np.zeros(k * n, np.float32),
+               prob=np.zeros(k * n, np.float32), status=np.zeros(k * n, np.int32))
+    rc = lib().orc_indicator_kriging(
+        _p(d, C.c_ubyte), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], C.c_int(k), covs, _p(rad, C.c_int),
+        _p(kmax, C.c_int), _p(marg, C.c_double), _p(out["cat"], C.c_ubyte), _p(out["mask"], C.c_ubyte),
+        _p(out["raw"], C.c_float), _p(out["prob"], C.c_float), _p(out["status"], C.c_int))
+    if rc != 0:
+        raise ValueError("oracle rejected the parameters")
+    for key in ("raw", "prob", "status"):
+        out[key] = out[key].reshape(k, n)
     return out
 
 
@@ -201,6 +231,46 @@ def ref_median_ik(cov, radiuses, max_nb, shape, data, mask, marginal_probs, thre
     rc = r.ref_median_ik(_p(d, C.c_ubyte), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], *_cov_args(cov),
                          (C.c_int * 3)(*radiuses), C.c_int(max_nb), C.c_double(marginal_probs[0]),
                          C.c_double(marginal_probs[1]), _p(od, C.c_ubyte), _p(om, C.c_ubyte))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return od, om
+
+
+def ref_lvm_kriging(cov, radiuses, max_nb, shape, values, mask, mean_data, threads=1):
+    r = ref_lib()
+    r.ref_set_threads(C.c_int(threads))
+    v = np.ascontiguousarray(values, np.float32).reshape(-1).copy()
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1).copy()
+    md = np.ascontiguousarray(mean_data, np.float32).reshape(-1).copy()
+    ov, om = v.copy(), m.copy()
+    rc = r.ref_lvm_kriging(_p(v, C.c_float), _p(m, C.c_ubyte), _p(md, C.c_float), *[C.c_int(s) for s in shape],
+                           *_cov_args(cov), (C.c_int * 3)(*radiuses), C.c_int(max_nb), _p(ov, C.c_float),
+                           _p(om, C.c_ubyte))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return ov, om
+
+
+def ref_indicator_kriging(shape, data, mask, cats, threads=1):
+    """The reference's hpgl::indicator_kriging; ``cats`` as for indicator_kriging()."""
+    r = ref_lib()
+    r.ref_set_threads(C.c_int(threads))
+    k = len(cats)
+    d = np.ascontiguousarray(data, np.uint8).reshape(-1).copy()
+    m = np.ascontiguousarray(mask, np.uint8).reshape(-1).copy()
+    od, om = d.copy(), m.copy()
+    types = np.ascontiguousarray([c["cov"].type for c in cats], np.int32)
+    ranges = np.ascontiguousarray([x for c in cats for x in c["cov"].ranges], np.float64)
+    angles = np.ascontiguousarray([x for c in cats for x in c["cov"].angles], np.float64)
+    sills = np.ascontiguousarray([c["cov"].sill for c in cats], np.float64)
+    nuggets = np.ascontiguousarray([c["cov"].nugget for c in cats], np.float64)
+    rad = np.ascontiguousarray([x for c in cats for x in c["radiuses"]], np.int32)
+    kmax = np.ascontiguousarray([c["max_nb"] for c in cats], np.int32)
+    marg = np.ascontiguousarray([c["marginal"] for c in cats], np.float64)
+    rc = r.ref_indicator_kriging(_p(d, C.c_ubyte), _p(m, C.c_ubyte), *[C.c_int(s) for s in shape], C.c_int(k),
+                                 _p(types, C.c_int), _p(ranges, C.c_double), _p(angles, C.c_double),
+                                 _p(sills, C.c_double), _p(nuggets, C.c_double), _p(rad, C.c_int),
+                                 _p(kmax, C.c_int), _p(marg, C.c_double), _p(od, C.c_ubyte), _p(om, C.c_ubyte))
     if rc != 0:
         raise RuntimeError("reference raised")
     return od, om

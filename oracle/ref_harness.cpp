@@ -20,6 +20,7 @@
 #include "cov_model.h"
 #include "covariance_field.h"
 #include "hpgl_core.h"
+#include "ik_params.h"
 #include "median_ik.h"
 #include "my_kriging_weights.h"
 #include "ok_params.h"
@@ -134,6 +135,51 @@ int ref_median_ik(unsigned char* data, unsigned char* mask, int nx, int ny, int 
         p.m_marginal_probs[0] = p0;
         p.m_marginal_probs[1] = p1;
         hpgl::median_ik_for_two_indicators(p, grid, in, out);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// hpgl::lvm_kriging (the path behind hpgl_lvm_kriging): mean_data holds one
+// mean per cell; output buffers pre-filled with a clone of the input.
+int ref_lvm_kriging(float* data, unsigned char* mask, float* mean_data, int nx, int ny, int nz, int type,
+                    const double* ranges, const double* angles, double sill, double nugget, const int* radiuses,
+                    int max_nb, float* out_data, unsigned char* out_mask) {
+    try {
+        int n = nx * ny * nz;
+        hpgl::cont_property_array_t in(data, mask, n), out(out_data, out_mask, n);
+        hpgl::sugarbox_grid_t grid;
+        grid.init(nx, ny, nz);
+        hpgl::ok_params_t p;
+        fill_cov(p, type, ranges, angles, sill, nugget);
+        p.set_radiuses(radiuses[0], radiuses[1], radiuses[2]);
+        p.m_max_neighbours = max_nb;
+        hpgl::lvm_kriging(in, mean_data, grid, p, out);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// hpgl::indicator_kriging (the path behind hpgl_indicator_kriging) with
+// `ncat` categories; per category c: types[c], ranges/angles[3c..3c+2],
+// sills[c], nuggets[c], radiuses[3c..3c+2], max_nb[c], marg[c].
+int ref_indicator_kriging(unsigned char* data, unsigned char* mask, int nx, int ny, int nz, int ncat,
+                          const int* types, const double* ranges, const double* angles, const double* sills,
+                          const double* nuggets, const int* radiuses, const int* max_nb, const double* marg,
+                          unsigned char* out_data, unsigned char* out_mask) {
+    try {
+        int n = nx * ny * nz;
+        hpgl::indicator_property_array_t in(data, mask, n, ncat), out(out_data, out_mask, n, ncat);
+        hpgl::sugarbox_grid_t grid;
+        grid.init(nx, ny, nz);
+        hpgl::ik_params_t p;
+        for (int c = 0; c < ncat; ++c)
+            p.add_indicator((hpgl::covariance_type_t)types[c], ranges[3 * c], ranges[3 * c + 1], ranges[3 * c + 2],
+                            angles[3 * c], angles[3 * c + 1], angles[3 * c + 2], sills[c], nuggets[c],
+                            radiuses[3 * c], radiuses[3 * c + 1], radiuses[3 * c + 2], (size_t)max_nb[c], marg[c]);
+        hpgl::indicator_kriging(in, grid, p, out);
         return 0;
     } catch (const std::exception&) {
         return -1;

@@ -331,3 +331,109 @@ def test_invalid_parameters_raise():
         D.krige(0, (2, 2, 2), (1, 1, 1), 4, CovarianceModel(type=0, ranges=(0, 1, 1)), m, values=v)
     with pytest.raises(RuntimeError):
         D.krige(0, (2, 2, 2), (0, 1, 1), 4, CovarianceModel(type=0, ranges=(1, 1, 1)), m, values=v)
+
+
+# ------------------------------------------------- LVM and indicator kriging
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lvm_device_matches_oracle(orc, precision):
+    import torch
+    from geo_bsd import device as D, synthetic as S
+    wl = S.lvm_c1(1.0)
+    o = orc.krige(1, _cov(orc, wl), wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, lvm_mean=wl.lvm_mean)
+    dev = torch.device("cuda:0")
+    print(f"[gpu] lvm {wl.shape} precision={precision}", flush=True)
+    out = D.lvm_kriging(torch.from_numpy(wl.values).to(dev), torch.from_numpy(wl.mask).to(dev),
+                        torch.from_numpy(wl.lvm_mean).to(dev), wl.shape, wl.radiuses, wl.k, _model(wl),
+                        precision=precision, est64=True, status=True, neighbours=True)
+    torch.cuda.synchronize()
+    r = {k: t.cpu().numpy() for k, t in out.items()}
+    assert np.array_equal(r["status"], o["status"].astype(np.uint8))
+    un = wl.mask == 0
+    assert np.array_equal(r["count"][un], o["count"][un])
+    assert np.array_equal(_node_lists(wl, r["count"], r["nboff"])[un], o["nb"][un])
+    assert np.array_equal(r["mask"], o["mask"])
+    assert np.all(np.abs(r["value"] - o["value"]) <= 1e-5 * np.maximum(1.0, np.abs(o["value"])))
+    kr = o["status"] == orc.ST_KRIGED
+    tol = 1e-9 if precision == "fp64" else 1e-6
+    assert np.all(np.abs(r["est64"][kr] - o["est64"][kr]) <= tol)
+
+
+def test_geo_lvm_kriging_matches_oracle(orc):
+    import geo_bsd
+    from geo_bsd import synthetic as S
+    wl = S.lvm_c1(0.8)
+    nx, ny, nz = wl.shape
+    prop = geo_bsd.ContProperty(wl.values.reshape((nx, ny, nz), order="F"), wl.mask.reshape((nx, ny, nz), order="F"))
+    out = geo_bsd.lvm_kriging(prop, geo_bsd.SugarboxGrid(nx, ny, nz), wl.lvm_mean, wl.radiuses, wl.k, _model(wl))
+    o = orc.krige(1, _cov(orc, wl), wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, lvm_mean=wl.lvm_mean)
+    got = out.data.reshape(-1, order="F")
+    assert np.array_equal(out.mask.reshape(-1, order="F"), o["mask"])
+    assert np.all(np.abs(got - o["value"]) <= 1e-5 * np.maximum(1.0, np.abs(o["value"])))
+
+
+def _ik_categories(wl):
+    from geo_bsd import CovarianceModel
+    return [dict(cov_model=CovarianceModel(**c["cov"]), radiuses=c["radiuses"], max_neighbours=c["k"])
+            for c in wl.categories]
+
+
+def _ik_oracle(orc, wl):
+    cats = [dict(cov=orc.make_cov(c["cov"]["type"], c["cov"]["ranges"], c["cov"]["angles"], c["cov"]["sill"],
+                                  c["cov"]["nugget"]), radiuses=c["radiuses"], max_nb=c["k"], marginal=m)
+            for c, m in zip(wl.categories, wl.marginal)]
+    return orc.indicator_kriging(wl.shape, wl.values, wl.mask, cats)
+
+
+def _check_categories(cat, o, margin=1e-5):
+    """Categories equal wherever the oracle's two largest corrected
+    probabilities differ by more than `margin` (float32 sums can differ in the
+    last bit between two FP64 solvers)."""
+    p = np.sort(o["prob"], axis=0)
+    clear = (p[-1] - p[-2]) > margin if p.shape[0] > 1 else np.ones(cat.size, bool)
+    assert np.array_equal(cat[clear], o["cat"][clear])
+    assert (cat != o["cat"]).sum() <= max(3, cat.size // 10000)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_indicator_kriging_device_matches_oracle(orc, precision):
+    import torch
+    from geo_bsd import device as D, synthetic as S
+    wl = S.ik_c3(0.25)
+    o = _ik_oracle(orc, wl)
+    dev = torch.device("cuda:0")
+    print(f"[gpu] ik {wl.shape} precision={precision}", flush=True)
+    out = D.indicator_kriging(torch.from_numpy(wl.values).to(dev), torch.from_numpy(wl.mask).to(dev), wl.shape,
+                              _ik_categories(wl), wl.marginal, precision=precision)
+    torch.cuda.synchronize()
+    r = {k: t.cpu().numpy() for k, t in out.items()}
+    assert r["mask"].all()
+    tol = 2e-6 if precision == "fp64" else 5e-6
+    assert np.all(np.abs(r["prob"] - o["prob"]) <= tol)
+    _check_categories(r["category"], o)
+    inf = wl.mask == 1
+    assert np.array_equal(r["category"][inf], wl.values[inf])
+
+
+def test_geo_indicator_kriging_matches_oracle(orc):
+    """geo.indicator_kriging through hpgl_indicator_kriging (4 categories) and
+    its two-category form, which the reference sends to median_ik."""
+    import geo_bsd
+    from geo_bsd import synthetic as S
+    wl = S.ik_c3(0.2)
+    nx, ny, nz = wl.shape
+    sh = (nx, ny, nz)
+    prop = geo_bsd.IndProperty(wl.values.reshape(sh, order="F"), wl.mask.reshape(sh, order="F"), 4)
+    out = geo_bsd.indicator_kriging(prop, geo_bsd.SugarboxGrid(nx, ny, nz), _ik_categories(wl), wl.marginal)
+    assert out.indicator_count == 4 and out.mask.all()
+    _check_categories(out.data.reshape(-1, order="F"), _ik_oracle(orc, wl))
+    # two categories: median IK of (data == 1) with marginal 1 - p0
+    d2 = (wl.values >= 2).astype(np.uint8)
+    prop2 = geo_bsd.IndProperty(d2.reshape(sh, order="F"), wl.mask.reshape(sh, order="F"), 2)
+    cats = _ik_categories(wl)[:2]
+    out2 = geo_bsd.indicator_kriging(prop2, geo_bsd.SugarboxGrid(nx, ny, nz), cats, [0.5, 0.5])
+    c0 = wl.categories[0]
+    cov = orc.make_cov(c0["cov"]["type"], c0["cov"]["ranges"], c0["cov"]["angles"], c0["cov"]["sill"],
+                       c0["cov"]["nugget"])
+    k = orc.krige(2, cov, c0["radiuses"], c0["k"], wl.shape, wl.mask, indicators=d2[None], means=[0.5])
+    cls = (k["prob"][0].astype(np.float32) > 0.5).astype(np.uint8)
+    assert np.array_equal(out2.data.reshape(-1, order="F"), cls)

@@ -345,3 +345,118 @@ def test_library_introsort_heap_fallback(orc):
         a[i] = n - i
     for k in (a, np.roll(a, 5), a[::-1].copy()):
         assert np.array_equal(orc.std_sort_perm(k), D.host_introsort_perm(k))
+
+
+# ------------------------------------------- locally varying mean (LVM) kriging
+def _lvm_case(orc, scale=0.6):
+    from geo_bsd import synthetic as S
+    wl = S.lvm_c1(scale)
+    return wl, _cov(orc, wl)
+
+
+def test_lvm_matches_reference_bitwise(ref):
+    """lvm_kriging.cpp:16-39 through the reference's own build, three models."""
+    wl, _ = _lvm_case(ref)
+    for t in (0, 1, 2):
+        c = dict(wl.cov, type=t)
+        cov = ref.make_cov(c["type"], c["ranges"], c["angles"], c["sill"], c["nugget"])
+        rv, rm = ref.ref_lvm_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, wl.lvm_mean, threads=8)
+        o = ref.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, lvm_mean=wl.lvm_mean,
+                      want_neighbours=False)
+        assert np.array_equal(rm, o["mask"]), t
+        assert np.array_equal(rv, o["value"]), t
+
+
+def test_lvm_constant_mean_is_simple_kriging(orc):
+    """A constant mean field reduces LVM to SK with that mean, bit for bit."""
+    wl, cov = _lvm_case(orc)
+    m = np.float32(0.21)
+    o1 = orc.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values,
+                   lvm_mean=np.full(wl.mask.size, m, np.float32), want_neighbours=False)
+    o2 = orc.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, means=[float(m)],
+                   want_neighbours=False)
+    assert np.array_equal(o1["value"], o2["value"]) and np.array_equal(o1["status"], o2["status"])
+
+
+def test_lvm_uses_each_neighbours_own_mean(orc):
+    """Data equal to the mean field give residuals of zero at every neighbour,
+    so every kriged cell returns its own mean exactly; a combine that used the
+    centre's mean for the neighbours would not."""
+    wl, cov = _lvm_case(orc)
+    vals = np.where(wl.mask == 1, wl.lvm_mean, np.float32(-99)).astype(np.float32)
+    o = orc.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=vals, lvm_mean=wl.lvm_mean,
+                  want_neighbours=False)
+    un = wl.mask == 0
+    assert np.array_equal(o["value"][un], wl.lvm_mean[un])
+    # and a shifted field moves each estimate by the kriged residual sum only
+    o2 = orc.krige(1, cov, wl.radiuses, wl.k, wl.shape, wl.mask, values=vals + np.float32(0.5),
+                   lvm_mean=wl.lvm_mean, want_neighbours=False)
+    kr = o2["status"] == orc.ST_KRIGED
+    assert np.all(o2["value"][kr & un] > wl.lvm_mean[kr & un])
+
+
+def test_lvm_failure_gives_the_cells_mean(orc):
+    """mean_on_failure (cont_kriging.h:80-94) with a per-cell mean."""
+    shape = (12, 10, 3)
+    n = int(np.prod(shape))
+    mask = np.zeros(n, np.uint8)
+    lm = np.linspace(-1, 1, n).astype(np.float32)
+    cov = orc.make_cov(0, (3, 3, 2), sill=1.0, nugget=0.0)
+    o = orc.krige(1, cov, (3, 3, 2), 8, shape, mask, values=np.zeros(n, np.float32), lvm_mean=lm)
+    assert np.all(o["status"] == orc.ST_NO_NEIGHBOURS)
+    assert np.array_equal(o["value"], lm) and o["mask"].all()
+
+
+# -------------------------------------------------- indicator kriging (K > 2)
+def _ik_cats(orc, wl):
+    return [dict(cov=orc.make_cov(c["cov"]["type"], c["cov"]["ranges"], c["cov"]["angles"], c["cov"]["sill"],
+                                  c["cov"]["nugget"]), radiuses=c["radiuses"], max_nb=c["k"], marginal=m)
+            for c, m in zip(wl.categories, wl.marginal)]
+
+
+def test_indicator_kriging_matches_reference(ref):
+    """indicator_kriging.h:88-161 through the reference's own build: 4
+    categories with three distinct models, and 3 categories with one model."""
+    from geo_bsd import synthetic as S
+    wl = S.ik_c3(0.2)
+    cats = _ik_cats(ref, wl)
+    rd, rm = ref.ref_indicator_kriging(wl.shape, wl.values, wl.mask, cats, threads=8)
+    o = ref.indicator_kriging(wl.shape, wl.values, wl.mask, cats)
+    assert np.array_equal(rd, o["cat"]) and np.array_equal(rm, o["mask"])
+    assert len(np.unique(o["cat"])) == 4
+    d3 = (wl.values % 3).astype(np.uint8)
+    cats3 = [dict(cats[1], marginal=p) for p in (0.3, 0.4, 0.3)]
+    rd, rm = ref.ref_indicator_kriging(wl.shape, d3, wl.mask, cats3, threads=8)
+    o = ref.indicator_kriging(wl.shape, d3, wl.mask, cats3)
+    assert np.array_equal(rd, o["cat"]) and np.array_equal(rm, o["mask"])
+
+
+def test_indicator_kriging_properties(orc):
+    """Informed cells reproduce their category (the cell is its own first
+    neighbour, so its weight vector is a unit vector); probabilities after the
+    correction lie in [0, 1]; the category is the first largest one."""
+    from geo_bsd import synthetic as S
+    wl = S.ik_c3(0.15)
+    cats = _ik_cats(orc, wl)
+    o = orc.indicator_kriging(wl.shape, wl.values, wl.mask, cats)
+    inf = wl.mask == 1
+    assert np.array_equal(o["cat"][inf], wl.values[inf])
+    assert np.all((o["prob"] >= 0) & (o["prob"] <= 1))
+    assert np.array_equal(o["cat"], np.argmax(o["prob"], axis=0))
+    # the raw probabilities of a category are SK of its indicator (median-IK form)
+    for c in range(4):
+        ind = (wl.values == c).astype(np.uint8)[None]
+        k = orc.krige(2, cats[c]["cov"], cats[c]["radiuses"], cats[c]["max_nb"], wl.shape, wl.mask, indicators=ind,
+                      means=[cats[c]["marginal"]], want_neighbours=False)
+        assert np.allclose(o["raw"][c], k["prob"][0], rtol=0, atol=2e-6)
+        assert np.array_equal(o["status"][c], k["status"])
+
+
+def test_indicator_kriging_single_category(orc):
+    shape = (8, 8, 2)
+    n = int(np.prod(shape))
+    mask = np.zeros(n, np.uint8)
+    mask[::7] = 1
+    cats = [dict(cov=orc.make_cov(1, (4, 4, 2)), radiuses=(4, 4, 2), max_nb=6, marginal=1.0)]
+    o = orc.indicator_kriging(shape, np.zeros(n, np.uint8), mask, cats)
+    assert not o["cat"].any() and o["mask"].all()
