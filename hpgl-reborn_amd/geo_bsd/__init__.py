@@ -4,8 +4,8 @@ an AMD Instinct MI355X (gfx950).
 
 Import-compatible with the reference package's kriging entry points:
 ``from geo_bsd import *`` provides ordinary_kriging, simple_kriging,
-median_ik, simple_kriging_weights, SugarboxGrid, CovarianceModel,
-ContProperty, IndProperty and covariance.  ``geo_bsd.device`` adds the
+lvm_kriging, median_ik, indicator_kriging, simple_kriging_weights,
+SugarboxGrid, CovarianceModel, ContProperty, IndProperty and covariance.  ``geo_bsd.device`` adds the
 device-tensor entry points (variance output, mixed precision, multi-threshold
 indicator CDFs).
 """
