@@ -44,7 +44,7 @@ def build(verbose=True):
     objs.append(plan_o)
     # the HIP translation units compile in parallel
     procs = []
-    units = ["krige.hip", "loo.hip", "api.hip", "loo_m0.hip", "loo_m1.hip", "loo_m2.hip"]
+    units = ["krige.hip", "loo.hip", "api.hip", "inc_io.hip", "loo_m0.hip", "loo_m1.hip", "loo_m2.hip"]
     units += [f"solve_n{n}_k{k}.hip" for n in (32, 24, 16, 8) for k in (0, 1, 2)]
     for src in units:
         o = os.path.join(BUILD, src.replace(".hip", ".o"))

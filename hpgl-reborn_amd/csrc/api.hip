@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/hpglx.h"
+#include "inc_io.h"
 #include "introsort.h"
 #include "krige_kernels.h"
 #include "krige_launch.h"
@@ -127,6 +128,7 @@ struct Library {
     size_t tcalls = 0;
     hipEvent_t done = nullptr;   // completion of the last call's work
     hipStream_t io_stream = nullptr;
+    IncText text;                // pinned host copy of the last property file read or written
 };
 
 Library& lib() {
@@ -386,6 +388,96 @@ int run_indicator(Library& L, const hpgl_ik_params_t* P, int ncat, const int dim
         return -1;
     }
     return 0;
+}
+
+bool ensure_io_stream(Library& L, std::string& err) {
+    if (!L.io_stream && hipStreamCreateWithFlags(&L.io_stream, hipStreamNonBlocking) != hipSuccess) {
+        err = "hipStreamCreate failed";
+        return false;
+    }
+    if (!ensure_events(L, err)) return false;
+    if (hipStreamWaitEvent(L.io_stream, L.done, 0) != hipSuccess) {
+        err = "hipStreamWaitEvent failed";
+        return false;
+    }
+    return true;
+}
+
+// read_inc_file_float / read_inc_file_byte (read_inc_file.cpp:137-193) into
+// device buffers: the file is read on the host, its data section parsed on
+// the GPU.
+int read_property(Library& L, const char* filename, int as_int, float undef_f, int undef_i, int size, float* d_f,
+                  unsigned char* d_b, unsigned char* d_mask, const unsigned char* values, int values_count,
+                  hipStream_t st, std::string& err) {
+    if (!filename) {
+        err = "null file name";
+        return -1;
+    }
+    const long long n = inc_load_file(L.text, filename, err);
+    if (n < 0) return -1;
+    std::string name;
+    const long long off = inc_data_offset(L.text.host, n, name, err);
+    if (off < 0) return -1;
+    if (inc_parse(L.text.host + off, n - off, size, as_int, undef_f, undef_i, d_f, d_b, d_mask, L.ws, st, err) != 0)
+        return -1;
+    if (as_int && values && values_count > 0 &&
+        inc_remap_bytes(d_b, d_mask, size, values, values_count, L.ws, st, err) != 0)
+        return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        err = "hipStreamSynchronize failed";
+        return -1;
+    }
+    return 0;
+}
+
+// property_writer.cpp:44-139: INC (name line, values, "/") or GSLIB
+// ("HPGL saved GSLIB file", 1, name line, values).
+int write_property(Library& L, const char* filename, int gslib, const float* d_f, const unsigned char* d_b,
+                   const unsigned char* d_mask, int size, float undef_f, unsigned char undef_b,
+                   const unsigned char* remap, int remap_count, const char* name, hipStream_t st, std::string& err) {
+    if (!filename || !name) {
+        err = "null file or property name";
+        return -1;
+    }
+    const long long len =
+        inc_format(d_f, d_b, d_mask, size, undef_f, undef_b, remap, remap_count, L.text, L.ws, st, err);
+    if (len < 0) return -1;
+    FILE* f = std::fopen(filename, "w+");
+    if (!f) {
+        err = std::string("Can't open file '") + filename + "'.";
+        return -1;
+    }
+    bool ok = true;
+    if (gslib)
+        ok = std::fprintf(f, "HPGL saved GSLIB file\n%d\n%s\n", 1, name) >= 0;
+    else
+        ok = std::fprintf(f, "%s\n", name) >= 0;
+    if (ok && len > 0) ok = std::fwrite(L.text.host, 1, (size_t)len, f) == (size_t)len;
+    if (ok && !gslib) ok = std::fputs("/\n", f) >= 0;
+    if (std::fclose(f) != 0) ok = false;
+    if (!ok) {
+        err = std::string("Error writing file '") + filename + "'.";
+        return -1;
+    }
+    return 0;
+}
+
+// api.cpp:143-167: the values table for byte properties.
+std::vector<unsigned char> remap_table(Library& L, const unsigned char* values, int values_count, int indicator_count) {
+    std::vector<unsigned char> t;
+    if (values_count == indicator_count) {
+        t.assign(values, values + values_count);
+    } else if (values_count > indicator_count) {
+        write_msg(L, "Given " + std::to_string(values_count) + " values for " + std::to_string(indicator_count) +
+                         " indicators. Ignoring extra values.\n");
+        t.assign(values, values + indicator_count);
+    } else {
+        if (values_count > 0)
+            write_msg(L, "Warning: Given " + std::to_string(values_count) + " values for " +
+                             std::to_string(indicator_count) + " indicators. Ignoring values. Using [0, 1, 2 ..]\n");
+        for (int i = 0; i < indicator_count; ++i) t.push_back((unsigned char)i);
+    }
+    return t;
 }
 
 int fail(Library& L, const std::string& msg) {
@@ -718,6 +810,121 @@ void hpgl_indicator_kriging(hpgl_ind_masked_array_t* in, hpgl_ind_masked_array_t
     });
 }
 
+// ------------------------------------------------ property files (INC/GSLIB)
+int hpgl_read_inc_file_float(char* filename, float undefined_value, int size, float* data, unsigned char* mask) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (size > 0 && (!data || !mask)) return fail(L, "null output buffers");
+    if (!ensure_io_stream(L, err)) return fail(L, err);
+    const size_t n = size > 0 ? (size_t)size : 1;
+    float* d_f = (float*)L.ws.get("io_read_f", n * 4, err);
+    unsigned char* d_m = (unsigned char*)L.ws.get("io_read_mask", n, err);
+    if (!err.empty()) return fail(L, err);
+    if (read_property(L, filename, 0, undefined_value, 0, size, d_f, nullptr, d_m, nullptr, 0, L.io_stream, err) != 0)
+        return fail(L, err);
+    if (size > 0 && (hipMemcpy(data, d_f, (size_t)size * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(mask, d_m, (size_t)size, hipMemcpyDeviceToHost) != hipSuccess))
+        return fail(L, "device to host copy failed");
+    return 0;
+}
+
+int hpgl_read_inc_file_byte(char* filename, int undefined_value, int size, unsigned char* data, unsigned char* mask,
+                            unsigned char* values, int values_count) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (size > 0 && (!data || !mask)) return fail(L, "null output buffers");
+    if (!ensure_io_stream(L, err)) return fail(L, err);
+    const size_t n = size > 0 ? (size_t)size : 1;
+    unsigned char* d_b = (unsigned char*)L.ws.get("io_read_b", n, err);
+    unsigned char* d_m = (unsigned char*)L.ws.get("io_read_mask", n, err);
+    if (!err.empty()) return fail(L, err);
+    if (read_property(L, filename, 1, 0.0f, undefined_value, size, nullptr, d_b, d_m, values, values_count,
+                      L.io_stream, err) != 0)
+        return fail(L, err);
+    if (size > 0 && (hipMemcpy(data, d_b, (size_t)size, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(mask, d_m, (size_t)size, hipMemcpyDeviceToHost) != hipSuccess))
+        return fail(L, "device to host copy failed");
+    return 0;
+}
+
+namespace {
+int write_host_property(Library& L, const char* filename, int gslib, const float* f, const unsigned char* b,
+                        const unsigned char* mask, int size, float undef_f, unsigned char undef_b,
+                        const std::vector<unsigned char>& remap, const char* name) {
+    std::string err;
+    if (size > 0 && (!mask || (!f && !b))) return fail(L, "null property buffers");
+    if (!ensure_io_stream(L, err)) return fail(L, err);
+    const size_t n = size > 0 ? (size_t)size : 1;
+    float* d_f = f ? (float*)L.ws.get("io_write_f", n * 4, err) : nullptr;
+    unsigned char* d_b = b ? (unsigned char*)L.ws.get("io_write_b", n, err) : nullptr;
+    unsigned char* d_m = (unsigned char*)L.ws.get("io_write_mask", n, err);
+    if (!err.empty()) return fail(L, err);
+    if (size > 0 && ((f && hipMemcpy(d_f, f, (size_t)size * 4, hipMemcpyHostToDevice) != hipSuccess) ||
+                     (b && hipMemcpy(d_b, b, (size_t)size, hipMemcpyHostToDevice) != hipSuccess) ||
+                     hipMemcpy(d_m, mask, (size_t)size, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(L, "host to device copy failed");
+    if (write_property(L, filename, gslib, d_f, d_b, d_m, size, undef_f, undef_b, remap.data(), (int)remap.size(),
+                       name, L.io_stream, err) != 0)
+        return fail(L, err);
+    return 0;
+}
+
+int shape_volume_or_fail(Library& L, const hpgl_shape_t* s, int& n) {
+    std::string err;
+    n = shape_volume(s, err);
+    if (n < 0) return fail(L, err);
+    return 0;
+}
+}  // namespace
+
+int hpgl_write_inc_file_float(char* filename, hpgl_cont_masked_array_t* arr, float undefined_value, char* name) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    int n = 0;
+    if (!arr || shape_volume_or_fail(L, &arr->m_shape, n) != 0) return -1;
+    return write_host_property(L, filename, 0, arr->m_data, nullptr, arr->m_mask, n, undefined_value, 0, {}, name);
+}
+
+int hpgl_write_inc_file_byte(char* filename, hpgl_ind_masked_array_t* arr, int undefined_value, char* name,
+                             unsigned char* values, int values_count) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    int n = 0;
+    if (!arr || shape_volume_or_fail(L, &arr->m_shape, n) != 0) return -1;
+    const auto remap = remap_table(L, values, values_count, arr->m_indicator_count);
+    return write_host_property(L, filename, 0, nullptr, arr->m_data, arr->m_mask, n, 0.0f,
+                               (unsigned char)undefined_value, remap, name);
+}
+
+int hpgl_write_gslib_cont_property(hpgl_cont_masked_array_t* data, const char* filename, const char* name,
+                                   double undefined_value) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    int n = 0;
+    if (!data || shape_volume_or_fail(L, &data->m_shape, n) != 0) return -1;
+    return write_host_property(L, filename, 1, data->m_data, nullptr, data->m_mask, n, (float)undefined_value, 0, {},
+                               name);
+}
+
+int hpgl_write_gslib_byte_property(hpgl_ind_masked_array_t* data, const char* filename, const char* name,
+                                   double undefined_value, unsigned char* values, int values_count) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    int n = 0;
+    if (!data || shape_volume_or_fail(L, &data->m_shape, n) != 0) return -1;
+    const auto remap = remap_table(L, values, values_count, data->m_indicator_count);
+    return write_host_property(L, filename, 1, nullptr, data->m_data, data->m_mask, n, 0.0f,
+                               (unsigned char)(int)undefined_value, remap, name);
+}
+
 int hpgl_simple_kriging_weights(float* center, float* xs, float* ys, float* zs, int n, hpgl_cov_params_t* params,
                                 float* weights) {
     Library& L = lib();
@@ -877,6 +1084,57 @@ int hpglx_indicator_kriging(const hpgl_ik_params_t* params, int ncat, const int 
     std::string err;
     if (run_indicator(L, params, ncat, dims, data, mask, out_cat, out_mask, out_prob, precision, (hipStream_t)stream,
                       err) != 0)
+        return fail(L, err);
+    return 0;
+}
+
+int hpglx_read_inc_float(const char* filename, float undefined_value, int size, float* d_data, unsigned char* d_mask,
+                         void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!ensure_events(L, err)) return fail(L, err);
+    if (read_property(L, filename, 0, undefined_value, 0, size, d_data, nullptr, d_mask, nullptr, 0,
+                      (hipStream_t)stream, err) != 0)
+        return fail(L, err);
+    return 0;
+}
+
+int hpglx_read_inc_byte(const char* filename, int undefined_value, int size, unsigned char* d_data,
+                        unsigned char* d_mask, const unsigned char* values, int values_count, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!ensure_events(L, err)) return fail(L, err);
+    if (read_property(L, filename, 1, 0.0f, undefined_value, size, nullptr, d_data, d_mask, values, values_count,
+                      (hipStream_t)stream, err) != 0)
+        return fail(L, err);
+    return 0;
+}
+
+int hpglx_write_inc_float(const char* filename, const float* d_data, const unsigned char* d_mask, int size,
+                          float undefined_value, const char* name, int gslib, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (write_property(L, filename, gslib, d_data, nullptr, d_mask, size, undefined_value, 0, nullptr, 0, name,
+                       (hipStream_t)stream, err) != 0)
+        return fail(L, err);
+    return 0;
+}
+
+int hpglx_write_inc_byte(const char* filename, const unsigned char* d_data, const unsigned char* d_mask, int size,
+                         int undefined_value, const char* name, const unsigned char* values, int values_count,
+                         int gslib, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (write_property(L, filename, gslib, nullptr, d_data, d_mask, size, 0.0f, (unsigned char)undefined_value,
+                       values, values_count, name, (hipStream_t)stream, err) != 0)
         return fail(L, err);
     return 0;
 }

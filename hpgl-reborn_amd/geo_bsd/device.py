@@ -287,3 +287,61 @@ def host_introsort_perm(keys):
     perm = numpy.zeros(keys.size, numpy.int32)
     _hpgl_so.hpglx_host_introsort_perm(keys.ctypes.data_as(C.c_void_p), keys.size, perm.ctypes.data_as(C.c_void_p))
     return perm
+
+
+# ------------------------------------------------------------ property files
+def read_inc_float(filename, undefined_value, size, device="cuda:0"):
+    """INC file -> (data float32, mask uint8) device tensors; the text is
+    parsed on the GPU (work of hpgl_read_inc_file_float, api.cpp:53-75)."""
+    torch = _torch()
+    data = torch.empty(int(size), dtype=torch.float32, device=device)
+    mask = torch.empty(int(size), dtype=torch.uint8, device=device)
+    stream = torch.cuda.current_stream(data.device).cuda_stream
+    rc = _hpgl_so.hpglx_read_inc_float(str(filename).encode(), float(undefined_value), int(size), _ptr(data),
+                                       _ptr(mask), C.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("hpglx_read_inc_float: " + _hpgl_so.hpglx_last_error().decode())
+    return data, mask
+
+
+def read_inc_byte(filename, undefined_value, size, indicator_values=(), device="cuda:0"):
+    """INC file of integers -> (data uint8, mask uint8) device tensors, values
+    remapped to their index in indicator_values (api.cpp:77-116)."""
+    torch = _torch()
+    data = torch.empty(int(size), dtype=torch.uint8, device=device)
+    mask = torch.empty(int(size), dtype=torch.uint8, device=device)
+    vals = numpy.ascontiguousarray(numpy.asarray(indicator_values, numpy.uint8).reshape(-1))
+    stream = torch.cuda.current_stream(data.device).cuda_stream
+    rc = _hpgl_so.hpglx_read_inc_byte(str(filename).encode(), int(undefined_value), int(size), _ptr(data), _ptr(mask),
+                                      vals.ctypes.data_as(C.c_void_p) if vals.size else None, int(vals.size),
+                                      C.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError("hpglx_read_inc_byte: " + _hpgl_so.hpglx_last_error().decode())
+    return data, mask
+
+
+def write_inc(filename, data, mask, undefined_value, name, gslib=False, indicator_values=None):
+    """Device property -> INC (or GSLIB) file, printed on the GPU
+    (property_writer.cpp).  float32 data are written with "%E", uint8 data as
+    integers through indicator_values."""
+    torch = _torch()
+    n = data.numel()
+    _check_dev(mask, torch.uint8, n, "mask")
+    stream = torch.cuda.current_stream(data.device).cuda_stream
+    if data.dtype == torch.float32:
+        _require(data.is_contiguous(), "data must be contiguous")
+        rc = _hpgl_so.hpglx_write_inc_float(str(filename).encode(), _ptr(data), _ptr(mask), int(n),
+                                            float(undefined_value), name.encode(), 1 if gslib else 0,
+                                            C.c_void_p(stream))
+        fn = "hpglx_write_inc_float"
+    else:
+        _check_dev(data, torch.uint8, n, "data")
+        vals = numpy.ascontiguousarray(numpy.asarray(indicator_values if indicator_values is not None else [],
+                                                     numpy.uint8).reshape(-1))
+        rc = _hpgl_so.hpglx_write_inc_byte(str(filename).encode(), _ptr(data), _ptr(mask), int(n),
+                                           int(undefined_value), name.encode(),
+                                           vals.ctypes.data_as(C.c_void_p) if vals.size else None, int(vals.size),
+                                           1 if gslib else 0, C.c_void_p(stream))
+        fn = "hpglx_write_inc_byte"
+    if rc != 0:
+        raise RuntimeError(fn + ": " + _hpgl_so.hpglx_last_error().decode())

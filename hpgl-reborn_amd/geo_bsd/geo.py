@@ -21,12 +21,14 @@ from . import validation
 from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_IK_PARAMS, _HPGL_MEDIAN_IK_PARAMS,
                         _HPGL_OK_PARAMS, _HPGL_SHAPE, _HPGL_SK_PARAMS, _HPGL_COV_PARAMS, _hpgl_so,
                         hpgl_output_handler, hpgl_progress_handler, last_error)
-from .validation import (CriticalValidationError, GridValidator, ParameterValidator, ValidationError,
+from .validation import (CriticalValidationError, GridValidator, ParameterValidator, PathValidator, ValidationError,
                          ValidationWarning)
 
 __all__ = ["ContProperty", "IndProperty", "SugarboxGrid", "CovarianceModel", "covariance", "ordinary_kriging",
            "simple_kriging", "lvm_kriging", "median_ik", "indicator_kriging", "simple_kriging_weights", "calc_mean", "set_output_handler",
-           "set_progress_handler", "set_thread_num", "get_thread_num", "append_mask", "ValidationError",
+           "set_progress_handler", "set_thread_num", "get_thread_num", "append_mask", "load_cont_property",
+           "load_ind_property", "read_inc_file_float", "read_inc_file_byte", "write_property", "write_gslib_property",
+           "ValidationError",
            "CriticalValidationError", "ValidationWarning"]
 
 
@@ -387,3 +389,156 @@ def set_thread_num(n):
 
 def get_thread_num():
     return _hpgl_so.hpgl_get_thread_num()
+
+
+# ------------------------------------------------------------ property files
+def _native_error(prefix):
+    msg = last_error()
+    return RuntimeError(prefix + msg)
+
+
+def _load_prop_cont_slow(filename, undefined_value):
+    """geo.py:300-322: every line not starting with "--", every
+    whitespace-separated part that float() accepts."""
+    safe_path = PathValidator.validate_filepath(filename)
+    values, mask = [], []
+    with open(safe_path, "r", encoding="utf-8") as f:
+        for line in f:
+            if line.strip().startswith("--"):
+                continue
+            for part in line.split():
+                try:
+                    val = float(part.strip())
+                except (ValueError, TypeError):
+                    continue
+                values.append(val)
+                mask.append(0 if val == undefined_value else 1)
+    return ContProperty(numpy.array(values, dtype="float32"), numpy.array(mask, dtype="uint8"))
+
+
+def _load_prop_ind_slow(filename, undefined_value, ind_values):
+    """geo.py:324-352."""
+    dict_map = {v: i for i, v in enumerate(ind_values)}
+    safe_path = PathValidator.validate_filepath(filename)
+    values, mask = [], []
+    with open(safe_path, "r", encoding="utf-8") as f:
+        for line in f:
+            if line.strip().startswith("--"):
+                continue
+            for part in line.split():
+                try:
+                    val = int(part.strip())
+                    if val == undefined_value:
+                        values.append(255)
+                        mask.append(0)
+                    else:
+                        values.append(dict_map[val])
+                        mask.append(1)
+                except (ValueError, TypeError, KeyError):
+                    pass
+    return IndProperty(numpy.array(values, dtype="uint8", order="F"), numpy.array(mask, dtype="uint8", order="F"),
+                       len(ind_values))
+
+
+def _total(size):
+    if isinstance(size, tuple) and len(size) == 3:
+        GridValidator.validate_grid_dimensions(size[0], size[1], size[2])
+        return int(size[0]) * int(size[1]) * int(size[2])
+    return int(size)
+
+
+def read_inc_file_float(filename, undefined_value, size):
+    """geo.py:500-521: `size` values of an INC file (parsed on the GPU)."""
+    safe_path = PathValidator.validate_filepath(filename, must_exist=True)
+    n = _total(size)
+    data = numpy.zeros(n, dtype="float32", order="F")
+    mask = numpy.zeros(n, dtype="uint8", order="F")
+    rc = _hpgl_so.hpgl_read_inc_file_float(safe_path.encode("utf-8"), undefined_value, n, data, mask)
+    if rc != 0:
+        raise _native_error("read_inc_file_float failed: ")
+    return ContProperty(data, mask)
+
+
+def read_inc_file_byte(filename, undefined_value, size, indicator_values):
+    """geo.py:523-544: integer values as bytes, remapped to the index of
+    their value in indicator_values."""
+    safe_path = PathValidator.validate_filepath(filename, must_exist=True)
+    n = _total(size)
+    data = numpy.zeros(n, dtype="uint8", order="F")
+    mask = numpy.zeros(n, dtype="uint8", order="F")
+    vals = numpy.array(indicator_values, dtype="uint8")
+    rc = _hpgl_so.hpgl_read_inc_file_byte(safe_path.encode("utf-8"), undefined_value, n, data, mask, vals, len(vals))
+    if rc != 0:
+        raise _native_error("read_inc_file_byte failed: ")
+    return IndProperty(data, mask, len(indicator_values))
+
+
+def load_cont_property(filename, undefined_value, size=None):
+    """geo.py:490-498."""
+    safe_path = PathValidator.validate_filepath(filename, must_exist=True)
+    if size is None:
+        print("[WARNING]. load_cont_property: Size is not specified. Using slow and ineficcient method.")
+        return _load_prop_cont_slow(safe_path, undefined_value)
+    return read_inc_file_float(safe_path, undefined_value, size)
+
+
+def load_ind_property(filename, undefined_value, indicator_values, size=None):
+    """geo.py:546-551."""
+    if size is None:
+        print("[WARNING]. load_ind_property: Size is not specified. Using slow and ineficcient method.")
+        return _load_prop_ind_slow(filename, undefined_value, indicator_values)
+    return read_inc_file_byte(filename, undefined_value, size, indicator_values)
+
+
+def _file_shape(prop):
+    if prop.data.ndim == 3:
+        return _shape(prop.data.shape)
+    return _shape((prop.data.size, 1, 1))
+
+
+@accepts_tuple("prop", 0)
+def write_property(prop, filename, prop_name, undefined_value, indicator_values=None):
+    """geo.py:418-461: INC file (values printed on the GPU)."""
+    safe_path = PathValidator.validate_filepath(filename)
+    if indicator_values is None:
+        indicator_values = []
+    sh = _file_shape(prop)
+    if isinstance(prop, ContProperty):
+        marr = _HPGL_CONT_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_float)),
+                                       mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
+        rc = _hpgl_so.hpgl_write_inc_file_float(safe_path.encode("utf-8"), C.byref(marr), undefined_value,
+                                                prop_name.encode("utf-8"))
+    else:
+        vals = numpy.array(indicator_values, dtype="uint8")
+        marr = _HPGL_IND_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                      mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                      indicator_count=prop.indicator_count)
+        rc = _hpgl_so.hpgl_write_inc_file_byte(safe_path.encode("utf-8"), C.byref(marr), undefined_value,
+                                               prop_name.encode("utf-8"), vals.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                               len(indicator_values))
+    if rc != 0:
+        raise _native_error("write_property failed: ")
+
+
+@accepts_tuple("prop", 0)
+def write_gslib_property(prop, filename, prop_name, undefined_value, indicator_values=None):
+    """geo.py:463-488: GSLIB file (values printed on the GPU)."""
+    safe_path = PathValidator.validate_filepath(filename)
+    if indicator_values is None:
+        indicator_values = []
+    sh = _shape((prop.data.size, 1, 1)) if prop.data.ndim != 3 else _shape(prop.data.shape)
+    if isinstance(prop, ContProperty):
+        marr = _HPGL_CONT_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_float)),
+                                       mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
+        rc = _hpgl_so.hpgl_write_gslib_cont_property(C.byref(marr), safe_path.encode("utf-8"),
+                                                     prop_name.encode("utf-8"), float(undefined_value))
+    else:
+        vals = (C.c_ubyte * max(1, len(indicator_values)))(*indicator_values)
+        marr = _HPGL_IND_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                      mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                      indicator_count=prop.indicator_count)
+        rc = _hpgl_so.hpgl_write_gslib_byte_property(C.byref(marr), safe_path.encode("utf-8"),
+                                                     prop_name.encode("utf-8"), float(undefined_value), vals,
+                                                     len(indicator_values))
+    if rc != 0:
+        raise _native_error("write_gslib_property failed: ")

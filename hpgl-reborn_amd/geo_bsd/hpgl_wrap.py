@@ -133,6 +133,37 @@ _hpgl_so.hpgl_lvm_kriging.argtypes = [
     C.POINTER(_HPGL_OK_PARAMS),
     NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]), NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
     C.POINTER(_HPGL_SHAPE)]
+_hpgl_so.hpgl_read_inc_file_float.restype = C.c_int
+_hpgl_so.hpgl_read_inc_file_float.argtypes = [C.c_char_p, C.c_float, C.c_int,
+                                              NC.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"]),
+                                              NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"])]
+_hpgl_so.hpgl_read_inc_file_byte.restype = C.c_int
+_hpgl_so.hpgl_read_inc_file_byte.argtypes = [C.c_char_p, C.c_int, C.c_int,
+                                             NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
+                                             NC.ndpointer(dtype=numpy.ubyte, flags=["F", "W", "A"]),
+                                             NC.ndpointer(dtype=numpy.ubyte), C.c_int]
+_hpgl_so.hpgl_write_inc_file_float.restype = C.c_int
+_hpgl_so.hpgl_write_inc_file_float.argtypes = [C.c_char_p, C.POINTER(_HPGL_CONT_MASKED_ARRAY), C.c_float, C.c_char_p]
+_hpgl_so.hpgl_write_inc_file_byte.restype = C.c_int
+_hpgl_so.hpgl_write_inc_file_byte.argtypes = [C.c_char_p, C.POINTER(_HPGL_IND_MASKED_ARRAY), C.c_int, C.c_char_p,
+                                              C.POINTER(C.c_ubyte), C.c_int]
+_hpgl_so.hpgl_write_gslib_cont_property.restype = C.c_int
+_hpgl_so.hpgl_write_gslib_cont_property.argtypes = [C.POINTER(_HPGL_CONT_MASKED_ARRAY), C.c_char_p, C.c_char_p,
+                                                    C.c_double]
+_hpgl_so.hpgl_write_gslib_byte_property.restype = C.c_int
+_hpgl_so.hpgl_write_gslib_byte_property.argtypes = [C.POINTER(_HPGL_IND_MASKED_ARRAY), C.c_char_p, C.c_char_p,
+                                                    C.c_double, C.POINTER(C.c_ubyte), C.c_int]
+_hpgl_so.hpglx_read_inc_float.restype = C.c_int
+_hpgl_so.hpglx_read_inc_float.argtypes = [C.c_char_p, C.c_float, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+_hpgl_so.hpglx_read_inc_byte.restype = C.c_int
+_hpgl_so.hpglx_read_inc_byte.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                         C.c_void_p]
+_hpgl_so.hpglx_write_inc_float.restype = C.c_int
+_hpgl_so.hpglx_write_inc_float.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_char_p,
+                                           C.c_int, C.c_void_p]
+_hpgl_so.hpglx_write_inc_byte.restype = C.c_int
+_hpgl_so.hpglx_write_inc_byte.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_char_p,
+                                          C.c_void_p, C.c_int, C.c_int, C.c_void_p]
 _hpgl_so.hpgl_indicator_kriging.restype = None
 _hpgl_so.hpgl_indicator_kriging.argtypes = [C.POINTER(_HPGL_IND_MASKED_ARRAY), C.POINTER(_HPGL_IND_MASKED_ARRAY),
                                             C.POINTER(_HPGL_IK_PARAMS), C.c_int]

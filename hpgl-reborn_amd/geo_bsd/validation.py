@@ -57,6 +57,31 @@ def _bad(x):
     return numpy.isnan(x) or numpy.isinf(x)
 
 
+class PathValidator:
+    """validation.py:96-160: file paths without traversal."""
+
+    @staticmethod
+    def validate_filepath(filename, must_exist=False, allow_directories=False, allowed_extensions=None) -> str:
+        import os
+        import pathlib
+        if not filename:
+            raise CriticalValidationError("Filename cannot be empty", "filename")
+        path_str = os.path.normpath(str(filename))
+        if ".." in path_str.split(os.sep) or "../" in str(filename) or "..\\" in str(filename):
+            raise CriticalValidationError(f"Path traversal detected in filename: {filename}", "filename")
+        path = pathlib.Path(filename)
+        try:
+            resolved = path.resolve(strict=must_exist)
+        except (OSError, RuntimeError) as e:
+            if must_exist:
+                raise CriticalValidationError(f"File does not exist: {filename}", "filename") from e
+            resolved = path.resolve()
+        if allowed_extensions is not None and resolved.suffix.lower() not in [e.lower() for e in allowed_extensions]:
+            raise CriticalValidationError(f"File extension '{resolved.suffix}' not allowed. "
+                                          f"Allowed extensions: {allowed_extensions}", "filename")
+        return str(resolved)
+
+
 class GridValidator:
     @staticmethod
     def validate_grid_dimensions(x: int, y: int, z: int) -> None:

@@ -144,6 +144,40 @@ void hpgl_lvm_kriging(float* input_data, unsigned char* input_mask, struct hpgl_
 void hpgl_indicator_kriging(struct hpgl_ind_masked_array_t* in_data, struct hpgl_ind_masked_array_t* out_data,
                             struct hpgl_ik_params_t* params, int indicator_count);
 
+/* Property files.  The file is read on the host and its text parsed on the
+ * GPU (or printed on the GPU and written on the host); values and masks are
+ * host arrays as in the reference.  All return 0, or -1 with the message in
+ * hpgl_get_last_exception_message(). */
+
+/* Replaces api.cpp:53-75 (read_inc_file.cpp:137-165): the first line that
+ * starts with a letter names the property; then `size` values as
+ * sscanf("%f") reads them ("--" comments to the end of the line, '/' ends the
+ * data); mask = value != undefined_value. */
+int hpgl_read_inc_file_float(char* filename, float undefined_value, int size, float* data, unsigned char* mask);
+
+/* Replaces api.cpp:77-116 (read_inc_file.cpp:167-193): integers as
+ * sscanf("%d"), stored as bytes, mask = value != undefined_value; informed
+ * bytes equal to values[j] become j. */
+int hpgl_read_inc_file_byte(char* filename, int undefined_value, int size, unsigned char* data, unsigned char* mask,
+                            unsigned char* values, int values_count);
+
+/* Replaces api.cpp:118-141 (property_writer.cpp:44-63): the name line, one
+ * "%E" line per value (undefined_value where the mask is 0), then "/". */
+int hpgl_write_inc_file_float(char* filename, struct hpgl_cont_masked_array_t* arr, float undefined_value,
+                              char* name);
+
+/* Replaces api.cpp:143-198 (property_writer.cpp:65-85): "%d" lines of
+ * (signed char) values[data] for informed cells. */
+int hpgl_write_inc_file_byte(char* filename, struct hpgl_ind_masked_array_t* arr, int undefined_value, char* name,
+                             unsigned char* values, int values_count);
+
+/* Replace api.cpp:200-251 (property_writer.cpp:88-139): GSLIB files, header
+ * "HPGL saved GSLIB file", 1, the name line, then the values. */
+int hpgl_write_gslib_cont_property(struct hpgl_cont_masked_array_t* data, const char* filename, const char* name,
+                                   double undefined_value);
+int hpgl_write_gslib_byte_property(struct hpgl_ind_masked_array_t* data, const char* filename, const char* name,
+                                   double undefined_value, unsigned char* values, int values_count);
+
 #ifdef __cplusplus
 }
 #endif

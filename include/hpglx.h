@@ -102,6 +102,21 @@ int hpglx_indicator_kriging(const struct hpgl_ik_params_t* params, int ncat, con
                             const unsigned char* data, const unsigned char* mask, unsigned char* out_cat,
                             unsigned char* out_mask, float* out_prob, int precision, void* stream);
 
+/* Property files straight to and from device buffers (the work of
+ * hpgl_read_inc_file_float / _byte and hpgl_write_inc_file_float / _byte,
+ * api.cpp:53-251): the text is parsed and printed on the GPU.  These
+ * synchronise `stream` before returning.  gslib = 1 writes the GSLIB form
+ * (property_writer.cpp:88-139). */
+int hpglx_read_inc_float(const char* filename, float undefined_value, int size, float* d_data, unsigned char* d_mask,
+                         void* stream);
+int hpglx_read_inc_byte(const char* filename, int undefined_value, int size, unsigned char* d_data,
+                        unsigned char* d_mask, const unsigned char* values, int values_count, void* stream);
+int hpglx_write_inc_float(const char* filename, const float* d_data, const unsigned char* d_mask, int size,
+                          float undefined_value, const char* name, int gslib, void* stream);
+int hpglx_write_inc_byte(const char* filename, const unsigned char* d_data, const unsigned char* d_mask, int size,
+                         int undefined_value, const char* name, const unsigned char* values, int values_count,
+                         int gslib, void* stream);
+
 /* Decodes an entry of out_nboff into the neighbour's offset from the cell. */
 void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz);
 

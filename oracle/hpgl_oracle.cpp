@@ -21,6 +21,8 @@
 //                        median_ik.cpp:24-90, calc_mean.cpp:6-30,
 //                        lvm_kriging.cpp:16-39, indicator_kriging.h:88-161
 //   most probable class  src/geo_bsd/hpgl/cdf_utils.cpp:11-27
+//   property files       src/geo_bsd/hpgl/read_inc_file.cpp:8-193,
+//                        property_writer.cpp:18-139, api.cpp:94-109
 //   order relations      src/geo_bsd/hpgl/indicator_kriging.h:34-59
 //
 // The reference delegates three pieces of arithmetic to third-party code; the
@@ -36,7 +38,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -691,6 +695,116 @@ int orc_sk_weights(const orc_cov_t* cp, const float center[3], const float* xs, 
     }
     if (variance) *variance = v < 0 ? 0 : v;
     return 0;
+}
+
+// ---------------------------------------------------------- property files
+// The INC reader over an in-memory file text[0, n).  Header: lines are read
+// as up to 1023 characters, each read followed by a skip of whitespace; the
+// first line starting with a letter names the property.  Data: tokens of up
+// to 255 non-space characters; a token starting with "--" drops the rest of
+// its line, one starting with '/' ends the data (an error before `size`
+// values), any other goes through sscanf "%f" (as_int = 0) or "%d"
+// (as_int = 1; stored as a byte).  At the end of the text without '/', the
+// last token read is converted again for every remaining value.  Masks:
+// value != undefined.  Returns 0, or -1 with the message in err[0, errcap).
+int orc_read_inc(const char* text, long long n, int as_int, float undef_f, int undef_i, int size, float* data_f,
+                 unsigned char* data_b, unsigned char* mask, char* err, int errcap) {
+    auto space = [](unsigned char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r'; };
+    auto set_err = [&](const std::string& m) {
+        if (err && errcap > 0) std::snprintf(err, (size_t)errcap, "%s", m.c_str());
+        return -1;
+    };
+    long long pos = 0;
+    // header
+    for (;;) {
+        while (pos < n && space((unsigned char)text[pos])) ++pos;
+        if (pos >= n) return set_err("Property name not found.");
+        long long len = 0;
+        while (pos + len < n && text[pos + len] != '\n' && len < 1023) ++len;
+        const unsigned char first = (unsigned char)text[pos];
+        pos += len;
+        const bool named = (first >= 'A' && first <= 'Z') || (first >= 'a' && first <= 'z');
+        while (len == 1023) {  // the continuation reads of a long line
+            while (pos < n && space((unsigned char)text[pos])) ++pos;
+            len = 0;
+            while (pos + len < n && text[pos + len] != '\n' && len < 1023) ++len;
+            if (len == 0) break;
+            pos += len;
+        }
+        while (pos < n && space((unsigned char)text[pos])) ++pos;
+        if (named) break;
+    }
+    // data
+    std::string tok, last;
+    bool have_last = false;
+    int v = 0;
+    while (v < size) {
+        while (pos < n && space((unsigned char)text[pos])) ++pos;
+        if (pos < n) {
+            tok.clear();
+            while (pos < n && !space((unsigned char)text[pos]) && tok.size() < 255) tok.push_back(text[pos++]);
+            last = tok;
+            have_last = true;
+        } else {
+            if (!have_last) return set_err("Unexpected end of file.");
+            tok = last;  // fscanf failed and left the buffer as it was
+            if (tok.size() >= 2 && tok[0] == '-' && tok[1] == '-') return set_err("Unexpected end of file.");
+        }
+        if (tok.size() >= 2 && tok[0] == '-' && tok[1] == '-') {
+            while (pos < n && text[pos] != '\n') ++pos;
+            continue;
+        }
+        if (!tok.empty() && tok[0] == '/') return set_err("Unexpected end of data.");
+        if (as_int) {
+            int x;
+            if (std::sscanf(tok.c_str(), "%d", &x) != 1) return set_err("Error parsing '" + tok + "' string.");
+            data_b[v] = (unsigned char)x;
+            mask[v] = x == undef_i ? 0 : 1;
+        } else {
+            float x;
+            if (std::sscanf(tok.c_str(), "%f", &x) != 1) return set_err("Error parsing '" + tok + "' string.");
+            data_f[v] = x;
+            mask[v] = x == undef_f ? 0 : 1;
+        }
+        ++v;
+    }
+    return 0;
+}
+
+// Byte values equal to values[j] become j where the mask is set
+// (api.cpp:94-109).
+void orc_remap_bytes(unsigned char* data, const unsigned char* mask, int size, const unsigned char* values, int nv) {
+    for (int i = 0; i < size; ++i) {
+        if (!mask[i]) continue;
+        for (int j = 0; j < nv; ++j)
+            if (data[i] == values[j]) {
+                data[i] = (unsigned char)j;
+                break;
+            }
+    }
+}
+
+// The value lines of a property file (property_writer.cpp:18-26, 44-85):
+// floats as "%E\n" (undefined where the mask is 0), bytes as "%d\n" of the
+// signed char remap[value].  Writes at most cap bytes; returns the length.
+long long orc_format_values(const float* f, const unsigned char* b, const unsigned char* mask, int size,
+                            float undef_f, unsigned char undef_b, const unsigned char* remap, int remap_count,
+                            char* out, long long cap) {
+    long long len = 0;
+    char line[64];
+    for (int i = 0; i < size; ++i) {
+        int k;
+        if (f) {
+            k = std::snprintf(line, sizeof line, "%E\n", (double)(mask[i] ? f[i] : undef_f));
+        } else {
+            unsigned char v = undef_b;
+            if (mask[i]) v = b[i] < remap_count ? remap[b[i]] : b[i];
+            k = std::snprintf(line, sizeof line, "%d\n", (int)(signed char)v);
+        }
+        if (len + k <= cap) std::memcpy(out + len, line, (size_t)k);
+        len += k;
+    }
+    return len;
 }
 
 // libstdc++ std::sort permutation for a key array under "greater key first"

@@ -311,3 +311,78 @@ def ref_sk_weights(cov, center, xs, ys, zs):
                            _p(np.ascontiguousarray(ys, np.float32), C.c_float), _p(np.ascontiguousarray(zs, np.float32), C.c_float),
                            C.c_int(n), _p(w, C.c_double), C.byref(var))
     return got, w, var.value
+
+
+# ---------------------------------------------------------- property files
+def read_inc(text, size, as_int=False, undefined=-99):
+    """The INC reader (read_inc_file.cpp) over file bytes ``text``.
+    Returns (data, mask); raises ValueError with the reference's message."""
+    buf = C.create_string_buffer(bytes(text), len(text) + 1)
+    data_f = np.zeros(max(size, 1), np.float32)
+    data_b = np.zeros(max(size, 1), np.uint8)
+    mask = np.zeros(max(size, 1), np.uint8)
+    err = C.create_string_buffer(512)
+    rc = lib().orc_read_inc(buf, C.c_longlong(len(text)), C.c_int(1 if as_int else 0), C.c_float(undefined),
+                            C.c_int(int(undefined)), C.c_int(size), _p(data_f, C.c_float), _p(data_b, C.c_ubyte),
+                            _p(mask, C.c_ubyte), err, C.c_int(512))
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return (data_b if as_int else data_f)[:size], mask[:size]
+
+
+def remap_bytes(data, mask, values):
+    d = np.ascontiguousarray(data, np.uint8).copy()
+    v = np.ascontiguousarray(values, np.uint8)
+    lib().orc_remap_bytes(_p(d, C.c_ubyte), _p(np.ascontiguousarray(mask, np.uint8), C.c_ubyte), C.c_int(d.size),
+                          _p(v, C.c_ubyte), C.c_int(v.size))
+    return d
+
+
+def format_values(mask, floats=None, bytes_=None, undefined=-99.0, remap=None):
+    """The value lines a property writer prints (property_writer.cpp:18-85)."""
+    m = np.ascontiguousarray(mask, np.uint8)
+    f = None if floats is None else np.ascontiguousarray(floats, np.float32)
+    b = None if bytes_ is None else np.ascontiguousarray(bytes_, np.uint8)
+    r = np.zeros(1, np.uint8) if remap is None else np.ascontiguousarray(remap, np.uint8)
+    args = (_p(f, C.c_float), _p(b, C.c_ubyte), _p(m, C.c_ubyte), C.c_int(m.size), C.c_float(float(undefined)),
+            C.c_ubyte(int(undefined) & 0xFF), _p(r, C.c_ubyte), C.c_int(0 if remap is None else r.size))
+    lib().orc_format_values.restype = C.c_longlong
+    n = lib().orc_format_values(*args, None, C.c_longlong(0))
+    out = C.create_string_buffer(int(n) + 1)
+    lib().orc_format_values(*args, out, C.c_longlong(n))
+    return out.raw[:n]
+
+
+def ref_read_inc(path, size, as_int=False, undefined=-99):
+    """The reference's read_inc_file_float / _byte on a file."""
+    r = ref_lib()
+    data = np.zeros(max(size, 1), np.uint8 if as_int else np.float32)
+    mask = np.zeros(max(size, 1), np.uint8)
+    if as_int:
+        rc = r.ref_read_inc_byte(str(path).encode(), C.c_int(int(undefined)), C.c_int(size), _p(data, C.c_ubyte),
+                                 _p(mask, C.c_ubyte))
+    else:
+        rc = r.ref_read_inc_float(str(path).encode(), C.c_float(undefined), C.c_int(size), _p(data, C.c_float),
+                                  _p(mask, C.c_ubyte))
+    if rc != 0:
+        raise ValueError("reference raised")
+    return data[:size], mask[:size]
+
+
+def ref_write(path, name, mask, floats=None, bytes_=None, undefined=-99.0, remap=None, indicator_count=None,
+              gslib=False):
+    """The reference's property_writer_t on a float or byte property."""
+    r = ref_lib()
+    m = np.ascontiguousarray(mask, np.uint8).copy()
+    if floats is not None:
+        f = np.ascontiguousarray(floats, np.float32).copy()
+        rc = r.ref_write_float(str(path).encode(), name.encode(), _p(f, C.c_float), _p(m, C.c_ubyte), C.c_int(m.size),
+                               C.c_double(float(undefined)), C.c_int(1 if gslib else 0))
+    else:
+        b = np.ascontiguousarray(bytes_, np.uint8).copy()
+        t = np.ascontiguousarray(remap, np.uint8)
+        rc = r.ref_write_byte(str(path).encode(), name.encode(), _p(b, C.c_ubyte), _p(m, C.c_ubyte), C.c_int(m.size),
+                              C.c_int(indicator_count or t.size), C.c_ubyte(int(undefined) & 0xFF), _p(t, C.c_ubyte),
+                              C.c_int(t.size), C.c_int(1 if gslib else 0))
+    if rc != 0:
+        raise RuntimeError("reference raised")

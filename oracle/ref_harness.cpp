@@ -25,6 +25,7 @@
 #include "my_kriging_weights.h"
 #include "ok_params.h"
 #include "precalculated_covariance.h"
+#include "property_writer.h"
 #include "property_array.h"
 #include "sk_params.h"
 #include "sugarbox_grid.h"
@@ -180,6 +181,61 @@ int ref_indicator_kriging(unsigned char* data, unsigned char* mask, int nx, int 
                             angles[3 * c], angles[3 * c + 1], angles[3 * c + 2], sills[c], nuggets[c],
                             radiuses[3 * c], radiuses[3 * c + 1], radiuses[3 * c + 2], (size_t)max_nb[c], marg[c]);
         hpgl::indicator_kriging(in, grid, p, out);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// hpgl::read_inc_file_float / read_inc_file_byte (read_inc_file.cpp:137-193),
+// the readers behind hpgl_read_inc_file_float / _byte (the byte form
+// without api.cpp's value remapping).  Returns 0, or -1 when they throw.
+int ref_read_inc_float(const char* filename, float undef, int size, float* data, unsigned char* mask) {
+    try {
+        hpgl::read_inc_file_float(filename, undef, size, data, mask);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_read_inc_byte(const char* filename, int undef, int size, unsigned char* data, unsigned char* mask) {
+    try {
+        hpgl::read_inc_file_byte(filename, undef, size, data, mask);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// property_writer_t (property_writer.cpp:44-139): INC (gslib = 0) or GSLIB
+// (gslib = 1) text of a float property, or of a byte property with `remap`.
+int ref_write_float(const char* filename, const char* name, float* data, unsigned char* mask, int n, double undef,
+                    int gslib) {
+    try {
+        hpgl::property_writer_t w;
+        w.init(filename, name);
+        if (gslib)
+            w.write_gslib_double(std::make_shared<hpgl::cont_property_array_t>(data, mask, n), undef);
+        else
+            w.write_double(hpgl::cont_property_array_t(data, mask, n), undef);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_write_byte(const char* filename, const char* name, unsigned char* data, unsigned char* mask, int n,
+                   int indicator_count, unsigned char undef, const unsigned char* remap, int remap_count, int gslib) {
+    try {
+        hpgl::property_writer_t w;
+        w.init(filename, name);
+        std::vector<unsigned char> table(remap, remap + remap_count);
+        if (gslib)
+            w.write_gslib_byte(std::make_shared<hpgl::indicator_property_array_t>(data, mask, n, indicator_count),
+                               undef, table);
+        else
+            w.write_byte(hpgl::indicator_property_array_t(data, mask, n, indicator_count), undef, table);
         return 0;
     } catch (const std::exception&) {
         return -1;
