@@ -65,12 +65,22 @@ __host__ __device__ inline Summ summ_identity() {
     return s;
 }
 
+__host__ __device__ inline Branch pick(const Summ& B, int state) {
+    Branch r;  // B.b[state] by selects, so the summaries stay in registers
+    r.out = state ? B.b[1].out : B.b[0].out;
+    r.count = state ? B.b[1].count : B.b[0].count;
+    r.slash = state ? B.b[1].slash : B.b[0].slash;
+    r.last = state ? B.b[1].last : B.b[0].last;
+    return r;
+}
+
 // A then B.
 __host__ __device__ inline Summ compose(const Summ& A, const Summ& B) {
     Summ r;
+#pragma unroll
     for (int k = 0; k < 2; ++k) {
         const Branch& a = A.b[k];
-        const Branch& b = B.b[a.out];
+        const Branch b = pick(B, a.out);
         r.b[k].out = b.out;
         r.b[k].count = a.count + b.count;
         r.b[k].slash = a.slash >= 0 ? a.slash : (b.slash >= 0 ? a.count + b.slash : -1);
@@ -93,7 +103,8 @@ struct Seg {
     __device__ unsigned at(int j) const { return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 };
 
-__device__ inline Seg load_seg(const unsigned char* t, long long lo) {
+template <typename P>
+__device__ inline Seg load_seg(P t, long long lo) {
     Seg s;
     const uint4* p = reinterpret_cast<const uint4*>(t + lo);
     const uint4 a = p[0], b = p[1];
@@ -104,43 +115,74 @@ __device__ inline Seg load_seg(const unsigned char* t, long long lo) {
     return s;
 }
 
-// Walks the segment from state `s`; on_number(j, ordinal) for each live
-// number token starting at byte j (ordinal counted within the segment).
-template <typename F>
-__device__ inline Branch walk(const Seg& g, int s, F&& on_number) {
-    Branch r{s, 0, -1, TK_NONE};
-    int comment = s;
+// Per-byte classes of the segment as bit masks (bit j = byte j).
+struct Masks {
+    uint32_t ts;     // token starts: a non-space byte after a space
+    uint32_t cs;     // token starts of "--" (comment candidates)
+    uint32_t slash;  // token starts of '/'
+    uint32_t nl;     // newlines
+};
+
+__device__ inline Masks seg_masks(const Seg& g) {
+    uint32_t sp = 0, nl = 0, dash = 0, sl = 0;
 #pragma unroll
     for (int j = 0; j < IO_SEG; ++j) {
-        const unsigned c = g.at(j);
-        if (c == '\n') {
-            comment = 0;
-            continue;
-        }
-        if (is_space(c) || comment) continue;
-        const unsigned before = j == 0 ? g.prev : g.at(j - 1);
-        if (!is_space(before)) continue;  // inside a token
-        const unsigned after = j + 1 < IO_SEG ? g.at(j + 1) : g.next;
-        if (c == '-' && after == '-') {
-            comment = 1;
-            r.last = TK_COMMENT;
-            continue;
-        }
-        if (c == '/') {
-            if (r.slash < 0) r.slash = r.count;
-            continue;
-        }
-        on_number(j, r.count);
-        r.count++;
-        r.last = TK_NUMBER;
+        const uint32_t c = g.at(j), bit = 1u << j;
+        sp |= (c == ' ' || c - 9u < 5u) ? bit : 0u;
+        nl |= c == '\n' ? bit : 0u;
+        dash |= c == '-' ? bit : 0u;
+        sl |= c == '/' ? bit : 0u;
     }
-    r.out = comment;
+    Masks m;
+    m.ts = ~sp & ((sp << 1) | (is_space(g.prev) ? 1u : 0u));
+    m.cs = m.ts & dash & ((dash >> 1) | (g.next == '-' ? 0x80000000u : 0u));
+    m.slash = m.ts & sl;
+    m.nl = nl;
+    return m;
+}
+
+// The segment from start state `s`: a comment runs from a live "--" token to
+// the next newline.  `numbers` receives the live number-token starts.
+__device__ inline Branch walk_bits(const Masks& m, int s, uint32_t& numbers) {
+    uint32_t dead = 0, live_cs = 0;
+    int pos = 0;
+    while (pos < IO_SEG) {
+        const uint32_t from = ~0u << pos;
+        if (s) {
+            const uint32_t nl = m.nl & from;
+            if (!nl) {
+                dead |= from;
+                break;
+            }
+            const int n = __ffs(nl) - 1;
+            dead |= from & ((1u << n) - 1u);
+            s = 0;
+            pos = n + 1;
+        } else {
+            const uint32_t cs = m.cs & from;
+            if (!cs) break;
+            const int c = __ffs(cs) - 1;
+            live_cs |= 1u << c;
+            s = 1;
+            pos = c;
+        }
+    }
+    numbers = m.ts & ~dead & ~m.slash;
+    Branch r;
+    r.out = s;
+    r.count = __popc(numbers);
+    const uint32_t live_slash = m.slash & ~dead;
+    r.slash = live_slash ? __popc(numbers & ((1u << (__ffs(live_slash) - 1)) - 1u)) : -1;
+    const uint32_t ev = numbers | live_cs;
+    r.last = ev ? (((numbers >> (31 - __clz(ev))) & 1u) ? TK_NUMBER : TK_COMMENT) : TK_NONE;
     return r;
 }
 
-__device__ inline Summ summarise(const Seg& g) {
+__device__ inline Summ summarise(const Masks& m) {
     Summ s;
-    for (int k = 0; k < 2; ++k) s.b[k] = walk(g, k, [](int, int) {});
+    uint32_t unused;
+    s.b[0] = walk_bits(m, 0, unused);
+    s.b[1] = walk_bits(m, 1, unused);
     return s;
 }
 
@@ -155,12 +197,14 @@ __device__ inline Summ shfl_up_summ(const Summ& v, int d) {
     return r;
 }
 
-// Exclusive scan of the block's lane summaries; returns the lane's prefix
-// and (through `total`) the block's composition.
+// Exclusive scan of the block's lane summaries (BLOCK threads); returns the
+// lane's prefix and (through `total`) the block's composition.
+template <int BLOCK>
 __device__ inline Summ block_exclusive(const Summ& mine, Summ& total) {
-    __shared__ Summ wave_tot[IO_BLOCK / 64];
+    __shared__ Summ wave_tot[BLOCK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Summ inc = mine;
+#pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const Summ o = shfl_up_summ(inc, d);
         if (lane >= d) inc = compose(o, inc);
@@ -168,9 +212,12 @@ __device__ inline Summ block_exclusive(const Summ& mine, Summ& total) {
     if (lane == 63) wave_tot[wave] = inc;
     __syncthreads();
     Summ before = summ_identity();
-    for (int w = 0; w < wave; ++w) before = compose(before, wave_tot[w]);
     total = summ_identity();
-    for (int w = 0; w < IO_BLOCK / 64; ++w) total = compose(total, wave_tot[w]);
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        const Summ wt = wave_tot[w];
+        if (w < wave) before = compose(before, wt);
+        total = compose(total, wt);
+    }
     Summ excl = shfl_up_summ(inc, 1);
     if (lane == 0) excl = summ_identity();
     return compose(before, excl);
@@ -179,9 +226,9 @@ __device__ inline Summ block_exclusive(const Summ& mine, Summ& total) {
 __global__ __launch_bounds__(IO_BLOCK) void k_inc_summary(const unsigned char* t, long long n, Summ* chunk_summ) {
     const long long lo = (long long)blockIdx.x * IO_CHUNK + (long long)threadIdx.x * IO_SEG;
     Summ mine = summ_identity();
-    if (lo < n) mine = summarise(load_seg(t, lo));
+    if (lo < n) mine = summarise(seg_masks(load_seg(t, lo)));
     Summ total;
-    (void)block_exclusive(mine, total);
+    (void)block_exclusive<IO_BLOCK>(mine, total);
     if (threadIdx.x == 0) chunk_summ[blockIdx.x] = total;
 }
 
@@ -189,25 +236,14 @@ __global__ __launch_bounds__(IO_BLOCK) void k_inc_summary(const unsigned char* t
 // comment" at the start of the data, and the whole text's summary.
 __global__ __launch_bounds__(1024) void k_inc_chunks(const Summ* chunk_summ, int nchunks, Branch* chunk_in,
                                                      Branch* whole) {
-    __shared__ Summ part[1024];
     const int t = threadIdx.x;
     const int per = (nchunks + 1023) / 1024;
     const int a = min(nchunks, t * per), b = min(nchunks, a + per);
     Summ acc = summ_identity();
     for (int c = a; c < b; ++c) acc = compose(acc, chunk_summ[c]);
-    part[t] = acc;
-    __syncthreads();
-    if (t == 0) {
-        Summ run = summ_identity();
-        for (int i = 0; i < 1024; ++i) {
-            const Summ x = part[i];
-            part[i] = run;
-            run = compose(run, x);
-        }
-        *whole = run.b[0];
-    }
-    __syncthreads();
-    Summ run = part[t];
+    Summ total;
+    Summ run = block_exclusive<1024>(acc, total);
+    if (t == 0) *whole = total.b[0];
     for (int c = a; c < b; ++c) {
         chunk_in[c] = run.b[0];
         run = compose(run, chunk_summ[c]);
@@ -659,8 +695,7 @@ struct EmitArgs {
     int* long_ord;  // smallest ordinal of a token longer than 255 characters
 };
 
-__device__ inline void emit_token(const EmitArgs& a, long long pos, int v) {
-    const unsigned char* p = a.t + pos;
+__device__ inline void emit_token(const EmitArgs& a, const unsigned char* p, int v) {
     int L = 0;
     while (L <= MAX_TOKEN && !is_space(p[L])) ++L;  // padded buffer: a space always follows
     if (L > MAX_TOKEN) {
@@ -686,28 +721,51 @@ __device__ inline void emit_token(const EmitArgs& a, long long pos, int v) {
     }
 }
 
+// The chunk plus what a token starting in it can reach (256 characters and
+// the byte after), staged in LDS.
+constexpr int EMIT_STAGE = IO_CHUNK + 272;
+
 __global__ __launch_bounds__(IO_BLOCK) void k_inc_emit(EmitArgs a) {
-    const long long lo = (long long)blockIdx.x * IO_CHUNK + (long long)threadIdx.x * IO_SEG;
-    Seg g;
+    __shared__ uint4 stage[EMIT_STAGE / 16];
+    const long long base_byte = (long long)blockIdx.x * IO_CHUNK;
+    const uint4* src = reinterpret_cast<const uint4*>(a.t + base_byte);
+    for (int i = threadIdx.x; i < EMIT_STAGE / 16; i += IO_BLOCK) stage[i] = src[i];
+    __syncthreads();
+    const unsigned char* lds = reinterpret_cast<const unsigned char*>(stage);
+    const int off = threadIdx.x * IO_SEG;
+    const long long lo = base_byte + off;
+    Masks msk{0u, 0u, 0u, 0u};
     Summ mine = summ_identity();
     if (lo < a.n) {
-        g = load_seg(a.t, lo);
-        mine = summarise(g);
+        Seg g;
+        const uint4* p = reinterpret_cast<const uint4*>(lds + off);
+        const uint4 x = p[0], y = p[1];
+        g.w[0] = x.x; g.w[1] = x.y; g.w[2] = x.z; g.w[3] = x.w;
+        g.w[4] = y.x; g.w[5] = y.y; g.w[6] = y.z; g.w[7] = y.w;
+        g.prev = off > 0 ? lds[off - 1] : (base_byte > 0 ? a.t[base_byte - 1] : (unsigned)' ');
+        g.next = lds[off + IO_SEG];
+        msk = seg_masks(g);
+        mine = summarise(msk);
     }
     Summ total;
-    const Summ excl = block_exclusive(mine, total);
+    const Summ excl = block_exclusive<IO_BLOCK>(mine, total);
     if (lo >= a.n) return;
     const Branch cin = a.chunk_in[blockIdx.x];
-    const Branch lane_in = excl.b[cin.out];  // this lane's start, relative to the chunk
+    const Branch lane_in = pick(excl, cin.out);  // this lane's start, relative to the chunk
     const int state = lane_in.out;
     const int base = cin.count + lane_in.count;
     // values past the first '/' or past `size` are never stored
     const int slash = a.whole->slash;
     const int limit = slash >= 0 && slash < a.size ? slash : a.size;
-    walk(g, state, [&](int j, int k) {
-        const int v = base + k;
-        if (v < limit) emit_token(a, lo + j, v);
-    });
+    uint32_t starts;
+    (void)walk_bits(msk, state, starts);
+    int v = base;
+    while (starts) {
+        const int j = __ffs(starts) - 1;
+        starts &= starts - 1;
+        if (v < limit) emit_token(a, lds + off + j, v);
+        ++v;
+    }
 }
 
 // End of file before `size` values and no '/': fscanf fails and leaves the
@@ -919,6 +977,7 @@ __device__ inline int fmt_value(const FmtArgs& a, long long i, char* s) {
     return format_byte(v, s);
 }
 
+// Length of the value's line, newline included.
 __device__ inline int fmt_len(const FmtArgs& a, long long i) {
     const bool informed = a.mask[i] != 0;
     if (a.f) {
@@ -926,8 +985,13 @@ __device__ inline int fmt_len(const FmtArgs& a, long long i) {
         const int sign = (__float_as_uint(v) >> 31) ? 1 : 0;
         return (isnan(v) || isinf(v) ? 3 : 12) + sign + 1;
     }
-    char tmp[8];
-    return fmt_value(a, i, tmp) + 1;
+    unsigned char b = a.undef_b;
+    if (informed) {
+        const unsigned char d = a.b[i];
+        b = d < a.remap_count ? a.remap[d] : d;
+    }
+    const int v = (int)(signed char)b, av = v < 0 ? -v : v;
+    return (v < 0 ? 1 : 0) + (av >= 100 ? 3 : (av >= 10 ? 2 : 1)) + 1;
 }
 
 constexpr int SCAN_BLOCK = 1024;
@@ -975,15 +1039,21 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(long long* block_sum, int 
     }
 }
 
+// Each lane prints its line into LDS at its block-local offset; the block
+// then copies its text out with consecutive lanes on consecutive bytes.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_fmt_write(FmtArgs a, const long long* len, const long long* block_off,
                                                           char* out) {
+    __shared__ char buf[SCAN_BLOCK * 16];
     const long long i = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
-    if (i >= a.size) return;
-    char s[16];
-    int n = fmt_value(a, i, s);
-    s[n++] = '\n';
-    char* dst = out + block_off[blockIdx.x] + len[i];
-    for (int k = 0; k < n; ++k) dst[k] = s[k];
+    if (i < a.size) {
+        char* s = buf + len[i];
+        const int n = fmt_value(a, i, s);
+        s[n] = '\n';
+    }
+    __syncthreads();
+    const long long o = block_off[blockIdx.x];
+    const int nbytes = (int)(block_off[blockIdx.x + 1] - o);
+    for (int k = threadIdx.x; k < nbytes; k += SCAN_BLOCK) out[o + k] = buf[k];
 }
 
 bool host_isalpha(unsigned c) {
