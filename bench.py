@@ -11,6 +11,13 @@ The metric is kriged cells/s of the FP32-Cholesky + FP64-refinement solver
 ("mixed"); the plain FP64 solver is timed the same way and reported beside
 it, together with both solvers' largest estimate error against the FP64 CPU
 oracle.  Prints one JSON line.
+
+`--workload` selects one of the other workloads instead (same line layout,
+its own metric): `mik` (median IK, 5 thresholds, 200x200x100), `wells` (OK
+from 40 wells, 256x256x64, cluster path), `ik` (indicator kriging over 4
+categories, 200x200x100), `lvm` (kriging with a locally varying mean,
+200x200x40) and `io` (INC property file of 4M values: GPU parse and print).
+Each is timed beside the reference's own CPU build where it is present.
 """
 import argparse
 import ctypes as C
@@ -143,6 +150,144 @@ def loo_bench(torch, check_oracle):
     return out
 
 
+def _ref_threads():
+    return max(1, min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 16))
+
+
+def _timed(fn, steps, warmup, sync):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def _cpu(kind, fn, unit_count, unit, sample):
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    t0 = time.perf_counter()
+    fn()
+    t = time.perf_counter() - t0
+    return {"value": unit_count / t, "unit": unit, "cores": _ref_threads(), "kind": kind, "sample": sample,
+            "seconds": t}
+
+
+def side_workload(name, args):
+    """One JSON line for a workload other than the headline one."""
+    import numpy as np
+    import torch
+    from geo_bsd import CovarianceModel, device as D, synthetic as S
+    from oracle import oracle as O
+    dev = torch.device("cuda:0")
+    sync = torch.cuda.synchronize
+    th = _ref_threads()
+    line = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic (geo_bsd.synthetic)"}
+    per = {}
+    if name in ("mik", "wells", "lvm"):
+        wl = {"mik": S.mik_c3, "wells": S.ok_wells_c4, "lvm": lambda: S.lvm_c1(4.0)}[name]()
+        model = CovarianceModel(**wl.cov)
+        m = torch.from_numpy(wl.mask).to(dev)
+        cov = O.make_cov(wl.cov["type"], wl.cov["ranges"], wl.cov["angles"], wl.cov["sill"], wl.cov["nugget"])
+        if name == "mik":
+            z = torch.from_numpy(wl.values).to(dev)
+            call = lambda prec: D.indicator_cdf(z, m, wl.shape, wl.thresholds, wl.levels, wl.radiuses, wl.k, model,
+                                                precision=prec, variance=False)
+            units = wl.mask.size
+            ind = [(wl.values <= t).astype(np.uint8) for t in wl.thresholds]
+            ref = lambda: [O.ref_median_ik(cov, wl.radiuses, wl.k, wl.shape, i, wl.mask, (1 - p, p), threads=th)
+                           for i, p in zip(ind, wl.levels)]
+            sample = f"{len(ind)} reference median_ik runs, one per threshold indicator, full grid"
+        elif name == "wells":
+            v = torch.from_numpy(wl.values).to(dev)
+            call = lambda prec: D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
+                                        variance=False)
+            units = int((wl.mask == 0).sum())
+            ref = lambda: O.ref_ordinary_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, threads=th)
+            sample = "reference ordinary_kriging, full grid"
+        else:
+            v = torch.from_numpy(wl.values).to(dev)
+            lm = torch.from_numpy(wl.lvm_mean).to(dev)
+            call = lambda prec: D.lvm_kriging(v, m, lm, wl.shape, wl.radiuses, wl.k, model, precision=prec,
+                                              variance=False)
+            units = int((wl.mask == 0).sum())
+            ref = lambda: O.ref_lvm_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, wl.lvm_mean,
+                                            threads=th)
+            sample = "reference lvm_kriging, full grid"
+        for prec in ("fp64", "mixed"):
+            per[prec] = _timed(lambda: call(prec), args.steps, args.warmup, sync)
+        best = min(per, key=per.get)
+        line.update(metric=f"kriged cells/s on 1 MI355X ({name})", value=units / per[best], unit="kriged cells/s",
+                    ms_per_step=per[best] * 1e3, dtype="f64" if best == "fp64" else "f32+f64",
+                    config={"workload": wl.name, "cells": int(wl.mask.size), "kriged_cells": units,
+                            "best_precision": best},
+                    fp64_ms=per["fp64"] * 1e3, mixed_ms=per["mixed"] * 1e3)
+        if not args.no_cpu_baseline:
+            line["cpu_baseline"] = _cpu("reference", ref, units, "kriged cells/s", sample)
+    elif name == "ik":
+        wl = S.ik_c3()
+        cats = [dict(cov_model=CovarianceModel(**c["cov"]), radiuses=c["radiuses"], max_neighbours=c["k"])
+                for c in wl.categories]
+        d = torch.from_numpy(wl.values).to(dev)
+        m = torch.from_numpy(wl.mask).to(dev)
+        for prec in ("fp64", "mixed"):
+            per[prec] = _timed(lambda: D.indicator_kriging(d, m, wl.shape, cats, wl.marginal, precision=prec),
+                               args.steps, args.warmup, sync)
+        best = min(per, key=per.get)
+        units = wl.mask.size
+        line.update(metric="cells/s on 1 MI355X (indicator kriging, 4 categories)", value=units / per[best],
+                    unit="cells/s", ms_per_step=per[best] * 1e3, dtype="f64" if best == "fp64" else "f32+f64",
+                    config={"workload": wl.name, "cells": units, "categories": 4, "distinct_models": 3,
+                            "best_precision": best},
+                    fp64_ms=per["fp64"] * 1e3, mixed_ms=per["mixed"] * 1e3)
+        if not args.no_cpu_baseline:
+            ocats = [dict(cov=O.make_cov(c["cov"]["type"], c["cov"]["ranges"], c["cov"]["angles"], c["cov"]["sill"],
+                                         c["cov"]["nugget"]), radiuses=c["radiuses"], max_nb=c["k"], marginal=p)
+                     for c, p in zip(wl.categories, wl.marginal)]
+            line["cpu_baseline"] = _cpu("reference", lambda: O.ref_indicator_kriging(wl.shape, wl.values, wl.mask,
+                                                                                      ocats, threads=th),
+                                        units, "cells/s", "reference indicator_kriging, full grid")
+    elif name == "io":
+        import tempfile
+        import geo_bsd
+        n = 4_000_000
+        rng = np.random.default_rng(1)
+        f = (0.2 + 0.05 * rng.standard_normal(n)).astype(np.float32)
+        mk = (rng.random(n) > 0.01).astype(np.uint8)
+        tmp = tempfile.mkdtemp()
+        path = os.path.join(tmp, "prop.inc")
+        df, dm = torch.from_numpy(f).to(dev), torch.from_numpy(mk).to(dev)
+        per["write"] = _timed(lambda: D.write_inc(path, df, dm, -99.0, "PROP"), args.steps, args.warmup, sync)
+        size = os.path.getsize(path)
+        per["read"] = _timed(lambda: D.read_inc_float(path, -99.0, n), args.steps, args.warmup, sync)
+        per["geo_read"] = _timed(lambda: geo_bsd.read_inc_file_float(path, -99, n), 3, 1, sync)
+        per["geo_write"] = _timed(lambda: geo_bsd.write_property(geo_bsd.ContProperty(f, mk), path, "PROP", -99.0),
+                                  3, 1, sync)
+        line.update(metric="property values/s read from an INC file into HBM on 1 MI355X",
+                    value=n / per["read"], unit="values/s", ms_per_step=per["read"] * 1e3, dtype="u8->f32",
+                    config={"workload": f"INC file of {n} float values (%E text, {size / 1e6:.1f} MB), file in the "
+                                        "page cache", "values": n, "bytes": size},
+                    device_write_ms=per["write"] * 1e3, geo_read_ms=per["geo_read"] * 1e3,
+                    geo_write_ms=per["geo_write"] * 1e3)
+        if not args.no_cpu_baseline and O.ref_available():
+            ref_path = os.path.join(tmp, "ref.inc")
+            wr = _cpu("reference", lambda: O.ref_write(ref_path, "PROP", mk, floats=f, undefined=-99.0), n,
+                      "values/s", "reference property_writer_t (fprintf %E), 1 thread")
+            rd = _cpu("reference", lambda: O.ref_read_inc(ref_path, n), n, "values/s",
+                      "reference read_inc_file_float (fscanf/sscanf), 1 thread")
+            rd["cores"] = wr["cores"] = 1
+            line["cpu_baseline"] = rd
+            line["cpu_baseline_write"] = wr
+            line["files_identical"] = open(ref_path, "rb").read() == open(path, "rb").read()
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return line
+
+
 def oracle_est64(wl):
     from oracle import oracle as O
     cov = O.make_cov(wl.cov["type"], wl.cov["ranges"], wl.cov["angles"], wl.cov["sill"], wl.cov["nugget"])
@@ -157,6 +302,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
+    ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io"],
+                    help="ok (default): the headline OK 100x100x50 workload; others: see the module docstring")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "
                          "scaling memory counters)")
@@ -165,6 +312,9 @@ def main():
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the kriging path runs on a "
               "single GPU; see DESIGN.md for how it would shard).", file=sys.stderr)
         return 2
+    if args.workload != "ok":
+        print(json.dumps(side_workload(args.workload, args)), flush=True)
+        return 0
 
     import numpy as np
     import torch

@@ -119,7 +119,7 @@ def lvm_c1(scale=1.0, seed=6):
     trend = trend.reshape(-1, order="F")
     f = trend + moving_average_field(rng, shape, (7, 7, 3), 0.0, 0.03)
     m = random_mask(rng, n, 0.02)
-    return Workload(name="lvm_50x50x10", kind="sk", shape=shape, values=with_undefined(f, m), mask=m,
+    return Workload(name=f"lvm_{nx}x{ny}x{nz}", kind="sk", shape=shape, values=with_undefined(f, m), mask=m,
                     lvm_mean=trend, cov=dict(type=0, ranges=(10.0, 10.0, 5.0), angles=(0.0, 0.0, 0.0), sill=1.0,
                                              nugget=0.1),
                     radiuses=(10, 10, 5), k=12)
