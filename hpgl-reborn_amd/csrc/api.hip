@@ -14,6 +14,7 @@
 #include <tuple>
 #include <vector>
 
+#include "../../include/cvariogram.h"
 #include "../../include/hpglx.h"
 #include "inc_io.h"
 #include "introsort.h"
@@ -21,8 +22,14 @@
 #include "krige_launch.h"
 #include "loo.h"
 #include "plan.h"
+#include "variogram_plan.h"
 
 namespace hpglx {
+
+int vario_grid(const VarioTemplate& t, const float* d_data, const unsigned char* d_mask, const int shape[3],
+               int lag_count, int percent, float* d_result, Workspace& ws, hipStream_t st, std::string& err);
+int vario_points(const VarioTemplate& t, const float* d_pts4, int n, int lag_count, float* d_result, Workspace& ws,
+                 hipStream_t st, std::string& err);
 
 // Model evaluation on the device for scattered coordinates (cov_model.h:63-77,
 // 146-183); the transform comes from the host.
@@ -478,6 +485,22 @@ std::vector<unsigned char> remap_table(Library& L, const unsigned char* values, 
         for (int i = 0; i < indicator_count; ++i) t.push_back((unsigned char)i);
     }
     return t;
+}
+
+VarioTemplate vario_template(const variogram_search_template_t& t) {
+    VarioTemplate v;
+    v.lag_width = t.m_lag_width;
+    v.lag_separation = t.m_lag_separation;
+    v.tol_distance = t.m_tol_distance;
+    v.first_lag_distance = t.m_first_lag_distance;
+    v.num_lags = t.m_num_lags;
+    const vector_t* d[3] = {&t.m_ellipsoid.m_direction1, &t.m_ellipsoid.m_direction2, &t.m_ellipsoid.m_direction3};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) v.dir[i][j] = d[i]->m_data[j];
+    v.R[0] = t.m_ellipsoid.m_R1;
+    v.R[1] = t.m_ellipsoid.m_R2;
+    v.R[2] = t.m_ellipsoid.m_R3;
+    return v;
 }
 
 int fail(Library& L, const std::string& msg) {
@@ -1137,6 +1160,128 @@ int hpglx_write_inc_byte(const char* filename, const unsigned char* d_data, cons
                        values, values_count, name, (hipStream_t)stream, err) != 0)
         return fail(L, err);
     return 0;
+}
+
+// ---------------------------------------------------- experimental variogram
+void fill_ellipsoid_directions(ellipsoid_t* ell, double azimuth, double dip, double rotation) {
+    double dir[3][3];
+    vario_fill_directions(azimuth, dip, rotation, dir);
+    vector_t* d[3] = {&ell->m_direction1, &ell->m_direction2, &ell->m_direction3};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) d[i]->m_data[j] = dir[i][j];
+}
+
+int hpglx_calc_variograms(const variogram_search_template_t* templ, const float* d_data, const unsigned char* d_mask,
+                          const int dims[3], int lag_count, int percent, float* d_result, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!templ || !d_data || !d_mask || !dims || !d_result) return fail(L, "null argument");
+    if (lag_count <= 0) return fail(L, "calc_variograms: num_lags must be positive");
+    if (!ensure_events(L, err)) return fail(L, err);
+    hipStream_t st = (hipStream_t)stream;
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) return fail(L, "hipStreamWaitEvent failed");
+    if (vario_grid(vario_template(*templ), d_data, d_mask, dims, lag_count, percent, d_result, L.ws, st, err) != 0 ||
+        hipEventRecord(L.done, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return fail(L, err.empty() ? "variogram failed" : err);
+    return 0;
+}
+
+int hpglx_calc_variograms_points(const variogram_search_template_t* templ, const float* d_xyzv, int n, int lag_count,
+                                 float* d_result, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!templ || (n > 0 && !d_xyzv) || !d_result) return fail(L, "null argument");
+    if (lag_count <= 0) return fail(L, "calc_variograms_from_point_set: num_lags must be positive");
+    if (!ensure_events(L, err)) return fail(L, err);
+    hipStream_t st = (hipStream_t)stream;
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) return fail(L, "hipStreamWaitEvent failed");
+    if (vario_points(vario_template(*templ), d_xyzv, n, lag_count, d_result, L.ws, st, err) != 0 ||
+        hipEventRecord(L.done, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return fail(L, err.empty() ? "variogram failed" : err);
+    return 0;
+}
+
+void calc_variograms(variogram_search_template_t* templ, hard_data_t* data, float* result, int result_length,
+                     int percentToUse) {
+    Library& L = lib();
+    std::string err;
+    {
+        std::lock_guard<std::mutex> g(L.mu);
+        L.last_error.clear();
+        const int lag_count = std::min(templ->m_num_lags, result_length);
+        if (lag_count <= 0) return;
+        for (int i = 0; i < 3; ++i)
+            if (data->m_data_shape[i] != data->m_mask_shape[i]) {
+                L.last_error = "calc_variograms: data and mask shapes differ";
+                return;
+            }
+        const int* sh = data->m_data_shape;
+        const long long n = (long long)sh[0] * sh[1] * sh[2];
+        if (n <= 0) {
+            for (int l = 0; l < lag_count; ++l) result[l] = 0.0f;
+            return;
+        }
+        // the strided host arrays, gathered x fastest
+        std::vector<float> hv((size_t)n);
+        std::vector<unsigned char> hm((size_t)n);
+        size_t k = 0;
+        for (int z = 0; z < sh[2]; ++z)
+            for (int y = 0; y < sh[1]; ++y)
+                for (int x = 0; x < sh[0]; ++x, ++k) {
+                    hv[k] = data->m_data[(long long)data->m_data_strides[0] * x +
+                                         (long long)data->m_data_strides[1] * y +
+                                         (long long)data->m_data_strides[2] * z];
+                    hm[k] = data->m_mask[(long long)data->m_mask_strides[0] * x +
+                                         (long long)data->m_mask_strides[1] * y +
+                                         (long long)data->m_mask_strides[2] * z];
+                }
+        float* d_v = (float*)L.ws.get("vario_in_v", (size_t)n * 4, err);
+        unsigned char* d_m = (unsigned char*)L.ws.get("vario_in_m", (size_t)n, err);
+        float* d_r = (float*)L.ws.get("vario_out", (size_t)lag_count * 4, err);
+        if (!err.empty() || hipMemcpy(d_v, hv.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_m, hm.data(), (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+            L.last_error = err.empty() ? "host to device copy failed" : err;
+            return;
+        }
+        if (!ensure_events(L, err) ||
+            vario_grid(vario_template(*templ), d_v, d_m, sh, lag_count, percentToUse, d_r, L.ws, nullptr, err) != 0 ||
+            hipMemcpy(result, d_r, (size_t)lag_count * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            L.last_error = err.empty() ? "variogram failed" : err;
+            return;
+        }
+    }
+}
+
+void calc_variograms_from_point_set(variogram_search_template_t* templ, cont_point_set_t* ps, float* result,
+                                    int result_length) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int lag_count = std::min(templ->m_num_lags, result_length);
+    if (lag_count <= 0) return;
+    const int n = ps->size > 0 ? ps->size : 0;
+    std::vector<float> pts((size_t)std::max(1, n) * 4);
+    for (int i = 0; i < n; ++i) {
+        pts[4 * (size_t)i + 0] = ps->xs[i];
+        pts[4 * (size_t)i + 1] = ps->ys[i];
+        pts[4 * (size_t)i + 2] = ps->zs[i];
+        pts[4 * (size_t)i + 3] = ps->values[i];
+    }
+    float* d_p = (float*)L.ws.get("vario_pts", pts.size() * 4, err);
+    float* d_r = (float*)L.ws.get("vario_out", (size_t)lag_count * 4, err);
+    if (!err.empty() || hipMemcpy(d_p, pts.data(), pts.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        L.last_error = err.empty() ? "host to device copy failed" : err;
+        return;
+    }
+    if (!ensure_events(L, err) ||
+        vario_points(vario_template(*templ), d_p, n, lag_count, d_r, L.ws, nullptr, err) != 0 ||
+        hipMemcpy(result, d_r, (size_t)lag_count * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        L.last_error = err.empty() ? "variogram failed" : err;
 }
 
 void hpglx_unpack_offset(int packed, int* dx, int* dy, int* dz) {

@@ -23,6 +23,8 @@
 //   most probable class  src/geo_bsd/hpgl/cdf_utils.cpp:11-27
 //   property files       src/geo_bsd/hpgl/read_inc_file.cpp:8-193,
 //                        property_writer.cpp:18-139, api.cpp:94-109
+//   variogram            src/geo_bsd/_cvariogram/variograms.cpp:27-506,
+//                        ellipsoid.cpp:37-72
 //   order relations      src/geo_bsd/hpgl/indicator_kriging.h:34-59
 //
 // The reference delegates three pieces of arithmetic to third-party code; the
@@ -805,6 +807,136 @@ long long orc_format_values(const float* f, const unsigned char* b, const unsign
         len += k;
     }
     return len;
+}
+
+// ------------------------------------------------------ experimental variogram
+// Directions of the search ellipsoid: columns of Rz(azimuth) * Ry(dip) *
+// Rx(rotation), with the reference's pi to 11 digits; products summed from
+// 0.0 over the inner index.
+void orc_vario_directions(double azimuth, double dip, double rotation, double* dirs9) {
+    const double k = 3.14159265358 / 180.0;
+    const double a = azimuth * k, d = dip * k, r = rotation * k;
+    const double A[9] = {cos(a), -sin(a), 0, sin(a), cos(a), 0, 0, 0, 1};
+    const double B[9] = {cos(d), 0, -sin(d), 0, 1, 0, sin(d), 0, cos(d)};
+    const double C[9] = {1, 0, 0, 0, cos(r), -sin(r), 0, sin(r), cos(r)};
+    double AB[9], M[9];
+    mat3_mul(A, B, AB);
+    mat3_mul(AB, C, M);
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 3; ++i) dirs9[3 * c + i] = M[3 * i + c];
+}
+
+namespace {
+struct Tunnel {
+    const double* dir;  // 3 x 3, row = direction
+    double R2, R3, tol;
+    bool operator()(double v0, double v1, double v2) const {
+        double s[3];
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            acc += v0 * dir[3 * k + 0];
+            acc += v1 * dir[3 * k + 1];
+            acc += v2 * dir[3 * k + 2];
+            s[k] = acc;
+        }
+        const double a = s[1] / R2, b = s[2] / R3;
+        const double dist = sqrt(a * a + b * b);
+        return dist <= 1.0 && tol * dist <= s[0];
+    }
+};
+}  // namespace
+
+// Semivariogram of a dense grid (x fastest), every informed pair at every
+// in-tunnel offset of the search window, accumulated sequentially in the
+// reference's order (offsets i, j, k ascending from the window's bounds
+// converted to int; cells z, y, x ascending), each pair added to every lag
+// whose [ceil, floor] range its distance falls in.  All pairs are used.
+int orc_vario_grid(const double* dirs9, double R2, double R3, double lag_width, double lag_sep, double tol,
+                   int num_lags, double first, const float* data, const unsigned char* mask, int nx, int ny, int nz,
+                   int lag_count, float* result) {
+    Tunnel in{dirs9, R2, R3, tol};
+    double mn[3] = {1e10, 1e10, 1e10}, mx[3] = {-1e10, -1e10, -1e10};
+    for (int i = 0; i < 2; ++i)
+        for (int j = -1; j <= 1; j += 2)
+            for (int k = -1; k <= 1; k += 2)
+                for (int c = 0; c < 3; ++c) {
+                    double di = dirs9[c] * lag_sep;
+                    di = di * num_lags;
+                    di = di * i;
+                    double dj = dirs9[3 + c] * R2;
+                    dj = dj * j;
+                    double dk = dirs9[6 + c] * R3;
+                    dk = dk * k;
+                    double v = di + dj;
+                    v = v + dk;
+                    if (v < mn[c]) mn[c] = v;
+                    if (v > mx[c]) mx[c] = v;
+                }
+    std::vector<double> sum(lag_count, 0.0);
+    std::vector<long long> cnt(lag_count, 0);
+    for (int ox = (int)mn[0]; ox <= mx[0]; ++ox)
+        for (int oy = (int)mn[1]; oy <= mx[1]; ++oy)
+            for (int oz = (int)mn[2]; oz <= mx[2]; ++oz) {
+                if (!in(ox, oy, oz)) continue;
+                const double dist = sqrt((double)ox * ox + (double)oy * oy + (double)oz * oz);
+                int lo = (int)ceil((dist - lag_width / 2 - first) / lag_sep);
+                int hi = (int)floor((dist + lag_width / 2 - first) / lag_sep);
+                if (lo >= lag_count || hi < 0) continue;
+                if (hi >= lag_count) hi = lag_count - 1;
+                if (lo < 0) lo = 0;
+                for (int z = 0; z < nz; ++z)
+                    for (int y = 0; y < ny; ++y)
+                        for (int x = 0; x < nx; ++x) {
+                            const long long i1 = x + (long long)nx * (y + (long long)ny * z);
+                            if (!mask[i1]) continue;
+                            const int x2 = x + ox, y2 = y + oy, z2 = z + oz;
+                            if (x2 < 0 || y2 < 0 || z2 < 0 || x2 >= nx || y2 >= ny || z2 >= nz) continue;
+                            const long long i2 = x2 + (long long)nx * (y2 + (long long)ny * z2);
+                            if (!mask[i2]) continue;
+                            double d = (double)data[i1] - (double)data[i2];
+                            d = d * d;
+                            for (int l = lo; l <= hi; ++l) {
+                                sum[l] += d;
+                                cnt[l] += 1;
+                            }
+                        }
+            }
+    for (int l = 0; l < lag_count; ++l) result[l] = cnt[l] > 0 ? (float)(sum[l] / cnt[l] / 2) : 0.0f;
+    return 0;
+}
+
+// Semivariogram of a point set over every ordered pair (self pairs too):
+// coordinate differences in float, lag l = [l * sep + first -/+ width / 2).
+int orc_vario_points(const double* dirs9, double R2, double R3, double lag_width, double lag_sep, double tol,
+                     double first, const float* xs, const float* ys, const float* zs, const float* vs, int n,
+                     int lag_count, float* result) {
+    Tunnel in{dirs9, R2, R3, tol};
+    std::vector<double> st(lag_count), en(lag_count), sum(lag_count, 0.0);
+    std::vector<long long> cnt(lag_count, 0);
+    for (int l = 0; l < lag_count; ++l) {
+        const double c = l * lag_sep + first;
+        st[l] = c - lag_width / 2;
+        en[l] = c + lag_width / 2;
+    }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) {
+            const double v0 = (float)(xs[a] - xs[b]), v1 = (float)(ys[a] - ys[b]), v2 = (float)(zs[a] - zs[b]);
+            if (!in(v0, v1, v2)) continue;
+            double q = 0.0;
+            q += v0 * v0;
+            q += v1 * v1;
+            q += v2 * v2;
+            const double dist = sqrt(q);
+            const double dv = (float)(vs[a] - vs[b]);
+            const double var = dv * dv;
+            for (int l = 0; l < lag_count; ++l)
+                if (st[l] <= dist && dist < en[l]) {
+                    sum[l] += var;
+                    cnt[l] += 1;
+                }
+        }
+    for (int l = 0; l < lag_count; ++l) result[l] = cnt[l] > 0 ? (float)(sum[l] / cnt[l] / 2) : 0.0f;
+    return 0;
 }
 
 // libstdc++ std::sort permutation for a key array under "greater key first"

@@ -386,3 +386,99 @@ def ref_write(path, name, mask, floats=None, bytes_=None, undefined=-99.0, remap
                               C.c_int(t.size), C.c_int(1 if gslib else 0))
     if rc != 0:
         raise RuntimeError("reference raised")
+
+
+# ------------------------------------------------------ experimental variogram
+def vario_directions(azimuth, dip, rotation):
+    d = np.zeros(9, np.float64)
+    lib().orc_vario_directions(C.c_double(azimuth), C.c_double(dip), C.c_double(rotation), _p(d, C.c_double))
+    return d.reshape(3, 3)
+
+
+def vario_grid(t, data, mask, shape, lag_count=None):
+    """``t``: dict(dirs (3x3, rows = directions), R2, R3, lag_width, lag_sep,
+    tol, num_lags, first).  data / mask flat, x fastest."""
+    n = t["num_lags"] if lag_count is None else lag_count
+    out = np.zeros(n, np.float32)
+    dirs = np.ascontiguousarray(t["dirs"], np.float64).reshape(-1)
+    lib().orc_vario_grid(_p(dirs, C.c_double), C.c_double(t["R2"]), C.c_double(t["R3"]),
+                         C.c_double(t["lag_width"]), C.c_double(t["lag_sep"]), C.c_double(t["tol"]),
+                         C.c_int(t["num_lags"]), C.c_double(t["first"]),
+                         _p(np.ascontiguousarray(data, np.float32), C.c_float),
+                         _p(np.ascontiguousarray(mask, np.uint8), C.c_ubyte), *[C.c_int(s) for s in shape],
+                         C.c_int(n), _p(out, C.c_float))
+    return out
+
+
+def vario_points(t, xs, ys, zs, vs, lag_count=None):
+    n = t["num_lags"] if lag_count is None else lag_count
+    out = np.zeros(n, np.float32)
+    dirs = np.ascontiguousarray(t["dirs"], np.float64).reshape(-1)
+    f = lambda a: _p(np.ascontiguousarray(a, np.float32), C.c_float)
+    lib().orc_vario_points(_p(dirs, C.c_double), C.c_double(t["R2"]), C.c_double(t["R3"]),
+                           C.c_double(t["lag_width"]), C.c_double(t["lag_sep"]), C.c_double(t["tol"]),
+                           C.c_double(t["first"]), f(xs), f(ys), f(zs), f(vs), C.c_int(len(xs)), C.c_int(n),
+                           _p(out, C.c_float))
+    return out
+
+
+class _VecT(C.Structure):
+    _fields_ = [("data", C.c_double * 3)]
+
+
+class _EllT(C.Structure):
+    _fields_ = [("d1", _VecT), ("d2", _VecT), ("d3", _VecT), ("R1", C.c_double), ("R2", C.c_double),
+                ("R3", C.c_double)]
+
+
+class _TemplT(C.Structure):
+    _fields_ = [("lag_width", C.c_double), ("lag_separation", C.c_double), ("tol_distance", C.c_double),
+                ("num_lags", C.c_int), ("first_lag_distance", C.c_double), ("ell", _EllT)]
+
+
+class _HardT(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("mask", C.c_void_p), ("data_shape", C.c_int * 3),
+                ("data_strides", C.c_int * 3), ("mask_shape", C.c_int * 3), ("mask_strides", C.c_int * 3)]
+
+
+class _PtsT(C.Structure):
+    _fields_ = [("xs", C.c_void_p), ("ys", C.c_void_p), ("zs", C.c_void_p), ("values", C.c_void_p),
+                ("size", C.c_int)]
+
+
+def _ref_template(t, R1=1.0):
+    e = _EllT()
+    for k, name in enumerate(("d1", "d2", "d3")):
+        getattr(e, name).data[:] = list(np.asarray(t["dirs"])[k])
+    e.R1, e.R2, e.R3 = R1, t["R2"], t["R3"]
+    return _TemplT(t["lag_width"], t["lag_sep"], t["tol"], t["num_lags"], t["first"], e)
+
+
+def ref_vario_directions(azimuth, dip, rotation):
+    e = _EllT()
+    ref_lib().fill_ellipsoid_directions(C.byref(e), C.c_double(azimuth), C.c_double(dip), C.c_double(rotation))
+    return np.array([list(e.d1.data), list(e.d2.data), list(e.d3.data)])
+
+
+def ref_vario_grid(t, data, mask, shape, lag_count=None):
+    """The reference's calc_variograms (percentToUse 100) on dense x-fastest arrays."""
+    n = t["num_lags"] if lag_count is None else lag_count
+    d = np.ascontiguousarray(data, np.float32)
+    m = np.ascontiguousarray(mask, np.uint8)
+    nx, ny, nz = shape
+    st = (C.c_int * 3)(1, nx, nx * ny)
+    hd = _HardT(d.ctypes.data, m.ctypes.data, (C.c_int * 3)(*shape), st, (C.c_int * 3)(*shape), st)
+    out = np.zeros(n, np.float32)
+    tp = _ref_template(t)
+    ref_lib().calc_variograms(C.byref(tp), C.byref(hd), _p(out, C.c_float), C.c_int(n), C.c_int(100))
+    return out
+
+
+def ref_vario_points(t, xs, ys, zs, vs, lag_count=None):
+    n = t["num_lags"] if lag_count is None else lag_count
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (xs, ys, zs, vs)]
+    ps = _PtsT(arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data, arrs[3].ctypes.data, len(xs))
+    out = np.zeros(n, np.float32)
+    tp = _ref_template(t)
+    ref_lib().calc_variograms_from_point_set(C.byref(tp), C.byref(ps), _p(out, C.c_float), C.c_int(n))
+    return out
