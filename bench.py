@@ -16,7 +16,9 @@ oracle.  Prints one JSON line.
 its own metric): `mik` (median IK, 5 thresholds, 200x200x100), `wells` (OK
 from 40 wells, 256x256x64, cluster path), `ik` (indicator kriging over 4
 categories, 200x200x100), `lvm` (kriging with a locally varying mean,
-200x200x40) and `io` (INC property file of 4M values: GPU parse and print).
+200x200x40), `io` (INC property file of 4M values: GPU parse and print) and
+`variogram` (experimental semivariogram of the 200x200x100 median-IK hard
+data, 40,000 cells, and of 20,000 scattered points).
 Each is timed beside the reference's own CPU build where it is present.
 """
 import argparse
@@ -283,6 +285,44 @@ def side_workload(name, args):
             line["cpu_baseline"] = rd
             line["cpu_baseline_write"] = wr
             line["files_identical"] = open(ref_path, "rb").read() == open(path, "rb").read()
+    elif name == "variogram":
+        from geo_bsd import cvariogram as CV
+        wl = S.mik_c3()
+        f = np.where(wl.mask == 1, wl.values, np.float32(0)).astype(np.float32)
+        templ = CV.VariogramSearchTemplate(2.0, 2.0, 1.0, 20, 0.0, CV.Ellipsoid(40.0, 3.0, 1.5, 30.0, 0.0, 0.0))
+        df, dm = torch.from_numpy(f).to(dev), torch.from_numpy(wl.mask).to(dev)
+        per["grid"] = _timed(lambda: CV.calc_variograms_device(templ, df, dm, wl.shape), args.steps, args.warmup,
+                             sync)
+        lw = S.loo_c5(nsets=1)
+        npts = lw.xyz.shape[0]
+        pts = torch.from_numpy(np.concatenate([lw.xyz, lw.values[:, None]], 1).astype(np.float32)).to(dev)
+        per["points"] = _timed(lambda: CV.calc_variograms_points_device(templ, pts), max(1, args.steps // 5), 1, sync)
+        ninf = int(wl.mask.sum())
+        g = CV.calc_variograms_device(templ, df, dm, wl.shape).cpu().numpy()
+        line.update(metric="hard-data cells/s through the grid experimental variogram on 1 MI355X",
+                    value=ninf / per["grid"], unit="informed cells/s", dtype="f64", ms_per_step=per["grid"] * 1e3,
+                    config={"workload": f"semivariogram of {ninf} hard data on a 200x200x100 grid, 20 lags of 2 "
+                                        "cells, azimuth 30, tunnel half-widths 3 x 1.5; and of 20,000 scattered "
+                                        "points", "informed_cells": ninf},
+                    points_ms=per["points"] * 1e3, point_pairs_per_s=npts * npts / per["points"],
+                    lags=[float(x) for x in g])
+        if not args.no_cpu_baseline and O.ref_available():
+            e = templ.templ.ellipsoid
+            t = dict(dirs=np.array([list(e.direction1.data), list(e.direction2.data), list(e.direction3.data)]),
+                     R2=3.0, R3=1.5, lag_width=2.0, lag_sep=2.0, tol=1.0, num_lags=20, first=0.0)
+            t0 = time.perf_counter()
+            rg = O.ref_vario_grid(t, f, wl.mask, wl.shape)
+            tg = time.perf_counter() - t0
+            line["cpu_baseline"] = {"value": ninf / tg, "unit": "informed cells/s", "cores": 1, "kind": "reference",
+                                    "sample": "reference calc_variograms on the full grid, 1 thread (as it runs)",
+                                    "seconds": tg}
+            line["max_rel_diff_vs_reference"] = float(np.max(np.abs(g - rg) / np.maximum(np.abs(rg), 1e-30)))
+            sub = 3000
+            rp = _cpu("reference", lambda: O.ref_vario_points(t, lw.xyz[:sub, 0], lw.xyz[:sub, 1], lw.xyz[:sub, 2],
+                                                              lw.values[:sub]),
+                      sub * sub, "pairs/s", f"reference calc_variograms_from_point_set on {sub} points, 1 thread")
+            rp["cores"] = 1
+            line["cpu_baseline_points"] = rp
     else:
         raise SystemExit(f"unknown workload {name}")
     return line
@@ -302,7 +342,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
-    ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io"],
+    ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io", "variogram"],
                     help="ok (default): the headline OK 100x100x50 workload; others: see the module docstring")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "

@@ -8,12 +8,13 @@
 // cells) sums its pairs in a fixed order, the chunks of an offset are summed
 // in order, and one lane per lag adds the offsets in the reference's window
 // order.  Point-set form (calc_variograms_from_point_set): every ordered
-// pair of points; a block holds 128 first points, streams the second points
-// through LDS and keeps per-lane per-lag sums in LDS; blocks are summed in
-// order.  Products and sums use the _rn intrinsics so that nothing is fused:
+// pair of points; a block holds 128 first points, streams a 2048-point range
+// of second points through LDS and keeps per-lane per-lag sums in LDS;
+// blocks are summed in order.  Products and sums use the _rn intrinsics so that nothing is fused:
 // each squared difference and each distance rounds as the reference's does.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -30,6 +31,7 @@ constexpr int VCELLS = 16;       // cells per thread in the compaction
 constexpr int VCHUNK = 2048;     // informed cells per block in the pair kernel
 constexpr int PB = 128;          // first points per block in the point-set kernel
 constexpr int PLAGS = 64;        // lags per pass of the point-set kernel
+constexpr int PJ = 2048;         // second points per block of the point-set kernel
 
 __global__ __launch_bounds__(VB) void k_vario_count(const unsigned char* mask, long long n, int* block_cnt) {
     __shared__ int part[VB];
@@ -153,27 +155,47 @@ __global__ __launch_bounds__(VB) void k_vario_pairs(PairArgs a) {
     }
 }
 
-// One lane per lag: the offsets' chunk sums in order, then the offsets in
-// window order (each offset feeds lags lag_min..lag_max).
-__global__ void k_vario_lags(const double* psum, const int* pcnt, int nchunks, const int2* lag_range, int noff,
-                             int lag_count, float* result) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= lag_count) return;
-    double sum = 0.0;
-    long long cnt = 0;
-    for (int o = 0; o < noff; ++o) {
+// One lane per offset: its chunk sums in chunk order.
+__global__ void k_vario_offsets(const double* psum, const int* pcnt, int nchunks, int noff, double* osum,
+                                long long* ocnt) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= noff) return;
+    double s = 0.0;
+    long long c = 0;
+    for (int k = 0; k < nchunks; ++k) {
+        s = __dadd_rn(s, psum[(long long)o * nchunks + k]);
+        c += pcnt[(long long)o * nchunks + k];
+    }
+    osum[o] = s;
+    ocnt[o] = c;
+}
+
+// One block per lag: the offsets feeding it (lag_min <= l <= lag_max), each
+// lane a strided share in window order, then a fixed tree over the lanes.
+__global__ __launch_bounds__(VB) void k_vario_lags(const double* osum, const long long* ocnt, const int2* lag_range,
+                                                   int noff, float* result) {
+    __shared__ double ss[VB];
+    __shared__ long long sc[VB];
+    const int l = blockIdx.x;
+    double s = 0.0;
+    long long c = 0;
+    for (int o = threadIdx.x; o < noff; o += VB) {
         const int2 lr = lag_range[o];
         if (l < lr.x || l > lr.y) continue;
-        double s = 0.0;
-        long long c = 0;
-        for (int k = 0; k < nchunks; ++k) {
-            s = __dadd_rn(s, psum[(long long)o * nchunks + k]);
-            c += pcnt[(long long)o * nchunks + k];
-        }
-        sum = __dadd_rn(sum, s);
-        cnt += c;
+        s = __dadd_rn(s, osum[o]);
+        c += ocnt[o];
     }
-    result[l] = cnt > 0 ? (float)(sum / (double)cnt / 2) : 0.0f;
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int d = VB / 2; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            ss[threadIdx.x] = __dadd_rn(ss[threadIdx.x], ss[threadIdx.x + d]);
+            sc[threadIdx.x] += sc[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) result[l] = sc[0] > 0 ? (float)(ss[0] / (double)sc[0] / 2) : 0.0f;
 }
 
 struct DevTemplate {
@@ -205,6 +227,7 @@ struct PointArgs {
     const double* lag_start;
     const double* lag_end;
     int lag0, nl;       // lags [lag0, lag0 + nl) in this pass
+    double far2;        // pairs with squared distance above this reach no lag of the pass
     double* psum;       // [block][nl]
     long long* pcnt;
 };
@@ -227,20 +250,26 @@ __global__ __launch_bounds__(PB) void k_vario_points(PointArgs a) {
     }
     const int i1 = blockIdx.x * PB + t;
     const float4 p1 = i1 < a.n ? a.pts[i1] : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int j0 = 0; j0 < a.n; j0 += PB) {
+    const int jend = min(a.n, (int)(blockIdx.y + 1) * PJ);
+    for (int j0 = blockIdx.y * PJ; j0 < jend; j0 += PB) {
         __syncthreads();
-        if (j0 + t < a.n) tile[t] = a.pts[j0 + t];
+        if (j0 + t < jend) tile[t] = a.pts[j0 + t];
         __syncthreads();
         if (i1 >= a.n) continue;
-        const int jn = min(PB, a.n - j0);
+        const int jn = min(PB, jend - j0);
         for (int j = 0; j < jn; ++j) {
             const float4 p2 = tile[j];
             const double v0 = (double)(p1.x - p2.x), v1 = (double)(p1.y - p2.y), v2 = (double)(p1.z - p2.z);
-            if (!in_tunnel_dev(a.t, v0, v1, v2)) continue;
             double q = 0.0;
             q = __dadd_rn(q, __dmul_rn(v0, v0));
             q = __dadd_rn(q, __dmul_rn(v1, v1));
             q = __dadd_rn(q, __dmul_rn(v2, v2));
+            // exact skips: too far for any lag of this pass (sqrt is monotone,
+            // far2 sits above the last lag end), or behind the tunnel's axis
+            // (tol * dist >= 0 > ss1 when tol >= 0)
+            if (q > a.far2) continue;
+            if (a.t.tol >= 0.0 && dot3d(v0, v1, v2, a.t.dir[0]) < 0.0) continue;
+            if (!in_tunnel_dev(a.t, v0, v1, v2)) continue;
             const double dist = __dsqrt_rn(q);
             const double dv = (double)(p1.w - p2.w);
             const double var = __dmul_rn(dv, dv);
@@ -260,21 +289,36 @@ __global__ __launch_bounds__(PB) void k_vario_points(PointArgs a) {
             s = __dadd_rn(s, asum[l * PB + k]);
             c += acnt[l * PB + k];
         }
-        a.psum[(long long)blockIdx.x * a.nl + l] = s;
-        a.pcnt[(long long)blockIdx.x * a.nl + l] = c;
+        const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+        a.psum[b * a.nl + l] = s;
+        a.pcnt[b * a.nl + l] = c;
     }
 }
 
-__global__ void k_vario_points_lags(const double* psum, const long long* pcnt, int nblocks, int nl, float* result) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nl) return;
+// One block per lag: the pair blocks' partial sums, each lane a strided
+// share in block order, then a fixed tree over the lanes.
+__global__ __launch_bounds__(VB) void k_vario_points_lags(const double* psum, const long long* pcnt, int nblocks,
+                                                          int nl, float* result) {
+    __shared__ double ss[VB];
+    __shared__ long long sc[VB];
+    const int l = blockIdx.x;
     double s = 0.0;
     long long c = 0;
-    for (int b = 0; b < nblocks; ++b) {
+    for (int b = threadIdx.x; b < nblocks; b += VB) {
         s = __dadd_rn(s, psum[(long long)b * nl + l]);
         c += pcnt[(long long)b * nl + l];
     }
-    result[l] = c > 0 ? (float)(s / (double)c / 2) : 0.0f;
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int d = VB / 2; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) {
+            ss[threadIdx.x] = __dadd_rn(ss[threadIdx.x], ss[threadIdx.x + d]);
+            sc[threadIdx.x] += sc[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) result[l] = sc[0] > 0 ? (float)(ss[0] / (double)sc[0] / 2) : 0.0f;
 }
 
 DevTemplate dev_template(const VarioTemplate& t) {
@@ -352,8 +396,15 @@ int vario_grid(const VarioTemplate& t, const float* d_data, const unsigned char*
             hipLaunchKernelGGL(k_vario_pairs, dim3(nchunks, std::min(65535, noff - o0)), dim3(VB), 0, st, b);
         }
     }
-    hipLaunchKernelGGL(k_vario_lags, dim3((lag_count + 63) / 64), dim3(64), 0, st, (const double*)psum,
-                       (const int*)pcnt, nchunks, (const int2*)d_lr, (ninf > 0 ? noff : 0), lag_count, d_result);
+    auto* osum = (double*)ws.get("vario_osum", (size_t)std::max(1, noff) * 8, err);
+    auto* ocnt = (long long*)ws.get("vario_ocnt", (size_t)std::max(1, noff) * 8, err);
+    if (!err.empty()) return -1;
+    const int nused = ninf > 0 ? noff : 0;
+    if (nused > 0)
+        hipLaunchKernelGGL(k_vario_offsets, dim3((nused + VB - 1) / VB), dim3(VB), 0, st, (const double*)psum,
+                           (const int*)pcnt, nchunks, nused, osum, ocnt);
+    hipLaunchKernelGGL(k_vario_lags, dim3(lag_count), dim3(VB), 0, st, (const double*)osum, (const long long*)ocnt,
+                       (const int2*)d_lr, nused, d_result);
     if (hipGetLastError() != hipSuccess) {
         err = "variogram kernels failed to launch";
         return -1;
@@ -369,7 +420,8 @@ int vario_points(const VarioTemplate& t, const float* d_pts4, int n, int lag_cou
     }
     std::vector<double> ls, le;
     vario_lag_bounds(t, lag_count, ls, le);
-    const int nblocks = std::max(1, (n + PB - 1) / PB);
+    const int nbx = std::max(1, (n + PB - 1) / PB), nby = std::max(1, (n + PJ - 1) / PJ);
+    const int nblocks = nbx * nby;
     auto* d_ls = (double*)ws.get("vario_ls", (size_t)std::max(1, lag_count) * 8, err);
     auto* d_le = (double*)ws.get("vario_le", (size_t)std::max(1, lag_count) * 8, err);
     auto* psum = (double*)ws.get("vario_ppsum", (size_t)nblocks * PLAGS * 8, err);
@@ -392,6 +444,9 @@ int vario_points(const VarioTemplate& t, const float* d_pts4, int n, int lag_cou
             a.lag_end = d_le;
             a.lag0 = l0;
             a.nl = nl;
+            double top = 0.0;
+            for (int l = l0; l < l0 + nl; ++l) top = std::max(top, le[l]);
+            a.far2 = top > 0.0 ? (top * (1.0 + 1e-9)) * (top * (1.0 + 1e-9)) : -1.0;
             a.psum = psum;
             a.pcnt = pcnt;
             const size_t lds = PB * sizeof(float4) + (size_t)nl * PB * (8 + 4) + (size_t)nl * 16;
@@ -401,9 +456,9 @@ int vario_points(const VarioTemplate& t, const float* d_pts4, int n, int lag_cou
                 err = "hipFuncSetAttribute failed";
                 return -1;
             }
-            hipLaunchKernelGGL(k_vario_points, dim3(nblocks), dim3(PB), lds, st, a);
+            hipLaunchKernelGGL(k_vario_points, dim3(nbx, nby), dim3(PB), lds, st, a);
         }
-        hipLaunchKernelGGL(k_vario_points_lags, dim3((nl + 63) / 64), dim3(64), 0, st, (const double*)psum,
+        hipLaunchKernelGGL(k_vario_points_lags, dim3(nl), dim3(VB), 0, st, (const double*)psum,
                            (const long long*)pcnt, n > 0 ? nblocks : 0, nl, d_result + l0);
     }
     if (hipGetLastError() != hipSuccess) {
