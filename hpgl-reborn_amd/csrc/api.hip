@@ -1358,6 +1358,10 @@ void hpglx_set_default_precision(int precision) {
 }
 
 void hpglx_host_introsort_perm(const double* keys, int n, int* perm) {
+    hpglx_host_introsort_first(keys, n, n, perm);
+}
+
+void hpglx_host_introsort_first(const double* keys, int n, int kneed, int* perm) {
     struct HostStore {
         struct Item {
             double key;
@@ -1372,7 +1376,7 @@ void hpglx_host_introsort_perm(const double* keys, int n, int* perm) {
     st.v.resize(n > 0 ? n : 0);
     for (int i = 0; i < n; ++i) st.v[i] = HostStore::Item{keys[i], i};
     Introsort<HostStore> s(st);
-    s.sort(n);
+    s.sort(n, kneed);
     for (int i = 0; i < n; ++i) perm[i] = st.v[i].off;
 }
 

@@ -156,8 +156,17 @@ struct Introsort {
 
     // Recursion of the right-hand part is replaced by an explicit stack of
     // (first, last, depth) ranges, visited in the same order.
-    HPGLX_HD void sort(int n) {
+    //
+    // With kneed < n only positions [0, kneed) are wanted: a right-hand part
+    // that starts at or after kneed is left unsorted.  That changes nothing
+    // before it: partitioning never moves elements across a cut, the
+    // left-hand part continues with the same depth, and the final insertion
+    // sort cannot move an element leftwards across a cut (an element is
+    // never "less" than one on the left of a cut), so it only has to run up
+    // to the first unsorted part.
+    HPGLX_HD void sort(int n, int kneed = 0x7fffffff) {
         if (n <= 0) return;
+        int lim = n;  // [0, lim) ends up fully sorted
         const int kThreshold = 16;
         struct Range { int first, last, depth; };
         Range stack[64];
@@ -177,6 +186,12 @@ struct Introsort {
                 const int mid = first + (last - first) / 2;
                 move_median_to_first(first, first + 1, mid, last - 1);
                 const int cut = unguarded_partition(first + 1, last, first);
+                if (cut >= kneed) {
+                    // [cut, last) is not wanted: go on with [first, cut)
+                    lim = cut < lim ? cut : lim;
+                    last = cut;
+                    continue;
+                }
                 // Defer [first, cut) with the current depth, descend into [cut, last).
                 stack[sp].first = first;
                 stack[sp].last = cut;
@@ -190,12 +205,13 @@ struct Introsort {
             last = stack[sp].last;
             depth = stack[sp].depth;
         }
-        // final_insertion_sort
-        if (n > kThreshold) {
+        // final_insertion_sort, over [0, lim): the same steps as over [0, n)
+        // for every element before lim
+        if (lim > kThreshold) {
             insertion_sort(0, kThreshold);
-            for (int i = kThreshold; i != n; ++i) unguarded_linear_insert(i);
+            for (int i = kThreshold; i != lim; ++i) unguarded_linear_insert(i);
         } else {
-            insertion_sort(0, n);
+            insertion_sort(0, lim);
         }
     }
 };

@@ -487,7 +487,7 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
                 }
         if (overflow) atomicAdd(a.overflow, 1);
         Introsort<LaneStore> srt(st);
-        srt.sort(m);
+        srt.sort(m, a.kmax);
         const int n = m < a.kmax ? m : a.kmax;
         for (int i = 0; i < n; ++i) a.nboff[cell * a.kmax + i] = st.get(i).off;
         a.count[cell] = n | CNT_CLUSTER;
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
                 continue;
             }
             Introsort<WordStore> srt(st);
-            srt.sort(m);
+            srt.sort(m, K);
             const int n = m < K ? m : K;
             const unsigned qmask = (1u << a.qbits) - 1u;
             for (int i = 0; i < n; ++i) {
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(Clust
         st.qbits = a.qbits;
         st.stride = CL_LIST_BLOCK;
         Introsort<WordStore> srt(st);
-        srt.sort(m);
+        srt.sort(m, K);
         const int n = m < K ? m : K;
         for (int r = 0; r < n; ++r) {
             // recover the node from its running index over the 27 clusters

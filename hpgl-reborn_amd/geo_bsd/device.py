@@ -289,6 +289,16 @@ def host_introsort_perm(keys):
     return perm
 
 
+def host_introsort_first(keys, kneed):
+    """Host run of the device introsort when only the first kneed slots are
+    wanted (slot -> input position; slots from kneed on are unspecified)."""
+    keys = numpy.ascontiguousarray(keys, numpy.float64)
+    perm = numpy.zeros(keys.size, numpy.int32)
+    _hpgl_so.hpglx_host_introsort_first(keys.ctypes.data_as(C.c_void_p), keys.size, int(kneed),
+                                        perm.ctypes.data_as(C.c_void_p))
+    return perm
+
+
 # ------------------------------------------------------------ property files
 def read_inc_float(filename, undefined_value, size, device="cuda:0"):
     """INC file -> (data float32, mask uint8) device tensors; the text is

@@ -184,6 +184,8 @@ _hpgl_so.hpglx_set_default_precision.restype = None
 _hpgl_so.hpglx_set_default_precision.argtypes = [C.c_int]
 _hpgl_so.hpglx_host_introsort_perm.restype = None
 _hpgl_so.hpglx_host_introsort_perm.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+_hpgl_so.hpglx_host_introsort_first.restype = None
+_hpgl_so.hpglx_host_introsort_first.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
 _hpgl_so.hpglx_knn.restype = C.c_int
 _hpgl_so.hpglx_knn.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
 _hpgl_so.hpglx_loo_state_size.restype = C.c_size_t

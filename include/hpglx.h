@@ -144,6 +144,9 @@ void hpglx_set_default_precision(int precision);
 /* Host run of the device introsort (for tests): perm[i] = input position of
  * the element sorted to slot i, ordering keys descending. */
 void hpglx_host_introsort_perm(const double* keys, int n, int* perm);
+/* The same when only the first kneed slots are wanted: perm[0, kneed) equals
+ * the full sort's (the cluster search asks for max_neighbours). */
+void hpglx_host_introsort_first(const double* keys, int n, int kneed, int* perm);
 
 /* Leave-one-out ordinary kriging of scattered points (no counterpart in the
  * reference; built on its covariance model, cov_model.h:146-183, and its OK

@@ -328,6 +328,28 @@ def test_library_introsort_matches_std_sort(orc):
         assert np.array_equal(orc.std_sort_perm(keys), D.host_introsort_perm(keys)), (trial, n, nv)
 
 
+def test_library_partial_introsort_matches_std_sort(orc):
+    """Only the first kneed slots wanted (ranges that start at or after kneed
+    are left unsorted): those slots equal std::sort's, incl. the heapsort
+    branch and kneed below the insertion threshold."""
+    from geo_bsd import device as D
+    rng = np.random.default_rng(1)
+    for trial in range(4000):
+        n = int(rng.integers(0, 400)) if trial % 10 else int(rng.integers(400, 3000))
+        nv = int(rng.integers(1, 8)) if trial % 3 else int(rng.integers(1, n + 2))
+        keys = rng.integers(0, nv, n).astype(np.float64)
+        if trial % 7 == 0:
+            keys = np.sort(keys)
+        if trial % 13 == 0:
+            keys = np.concatenate([np.arange(n // 2), np.arange(n - n // 2)[::-1]]).astype(np.float64)
+        kneed = int(rng.choice([1, 5, 8, 16, 17, 24, 32, 64, max(1, n // 2), n + 3]))
+        full = orc.std_sort_perm(keys)
+        part = D.host_introsort_first(keys, kneed)
+        k = min(kneed, n)
+        assert np.array_equal(full[:k], part[:k]), (trial, n, nv, kneed)
+        assert np.array_equal(np.sort(part), np.arange(n))  # still a permutation
+
+
 def test_library_introsort_heap_fallback(orc):
     """Inputs that drive introsort past its depth limit (heapsort branch)."""
     from geo_bsd import device as D
