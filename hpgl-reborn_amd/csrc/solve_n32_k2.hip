@@ -5,7 +5,7 @@ namespace hpglx {
 
 void dispatch_solve_32_2(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev,
                            hipStream_t st) {
-    dispatch_kind<32, 8, 4, 2>(sa, mixed, n, nelem, ev, st);
+    dispatch_kind<32, 8, 8, 2>(sa, mixed, n, nelem, ev, st);
 }
 
 }  // namespace hpglx
