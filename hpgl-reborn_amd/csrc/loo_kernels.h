@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_loo_forward(LooArgs a) {
         for (int s = c / G; s < S; ++s) {
             const int r = G * s + t;
             const double v = (r == c) ? q.sill : pair_cov<MODEL>(q, px[s] - cx, py[s] - cy, pz[s] - cz);
-            L[slot_off<G>(s) + c] = r < a.k ? v : ((r == c) ? 1.0 : 0.0);
+            L[slot_off<G>(s) + c] = r < a.k ? -v : ((r == c) ? -1.0 : 0.0);  // Chol takes -A
         }
     }
     double bq[S];

@@ -4,6 +4,10 @@
 // rows r = G*s + t of the n x n system (s = 0 .. NMAX/G - 1 "slots") and keeps,
 // for slot s, the columns 0 .. G*s + G - 1 in registers: a ragged lower
 // triangle with static indices, about NMAX^2 / (2G) + NMAX/2 values per lane.
+//   * The factor is stored negated: the caller fills -A, the factor leaves
+//     -L.  Every update is then an add of a product (v_fmac), into which the
+//     compiler folds the broadcast operand as DPP, so a broadcast costs no
+//     instruction of its own.
 //   * Right-looking Cholesky: at step j the owner of row j broadcasts the
 //     pivot; every lane scales its column-j entries; then for each c > j the
 //     entry L[c][j] is broadcast from its owner and every slot holding column
@@ -127,6 +131,13 @@ __device__ __forceinline__ double rsqrt_t(double x) {
     return __builtin_fma(0.5 * y, e, y);
 }
 
+__device__ __forceinline__ float fmadd(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fmadd(double a, double b, double c) {
+    return __builtin_fma(a, b, c);
+}
+
 // Offset of slot s in the ragged row storage.
 template <int G>
 __host__ __device__ constexpr int slot_off(int s) {
@@ -139,32 +150,85 @@ struct Chol {
     static constexpr int S = NMAX / G;
     static constexpr int TOT = slot_off<G>(S);
 
-    // In-place Cholesky; D[s] = 1 / L[r_s][r_s].  Group-uniform result.
+    // In-place Cholesky of the negated matrix: on entry the lower triangle
+    // holds -A, on return -L (the diagonal entries are not read again) and
+    // D[s] = 1 / L[r_s][r_s].  Group-uniform result.
     static __device__ __forceinline__ bool factor(T (&L)[TOT], T (&D)[S], T& minpiv, int t) {
         bool ok = true;
         minpiv = T(3.0e38);
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) {
             const int q = j % G, sj = j / G;
-            const T piv = gbc<G>(L[slot_off<G>(sj) + j], q);
+            const T piv = -gbc<G>(L[slot_off<G>(sj) + j], q);
             ok = ok && (piv > T(0));
             minpiv = piv < minpiv ? piv : minpiv;
             const T rs = rsqrt_t(piv);
-            const T d = piv * rs;
             if (t == q) D[sj] = rs;
 #pragma unroll
-            for (int s = sj; s < S; ++s) {
-                T& e = L[slot_off<G>(s) + j];
-                if (s == sj)
-                    e = (t == q) ? d : e * rs;
-                else
-                    e = e * rs;
-            }
+            for (int s = sj; s < S; ++s) L[slot_off<G>(s) + j] *= rs;
+            // -A[s][c] + L[s][j] L[c][j] = (-L[s][j]) (-L[c][j]) + (-A[s][c])
 #pragma unroll
             for (int c = j + 1; c < NMAX; ++c) {
                 const T lc = gbc<G>(L[slot_off<G>(c / G) + j], c % G);
 #pragma unroll
-                for (int s = c / G; s < S; ++s) L[slot_off<G>(s) + c] -= L[slot_off<G>(s) + j] * lc;
+                for (int s = c / G; s < S; ++s)
+                    L[slot_off<G>(s) + c] = fmadd(L[slot_off<G>(s) + j], lc, L[slot_off<G>(s) + c]);
+            }
+        }
+        return ok;
+    }
+
+    // Blocked form for float with groups of 4 lanes.  Lane t holds rows
+    // 4s+t, so the 4x4 tile (s, b) of the lower triangle is the lane's
+    // registers L[slot_off(s) + 4b .. 4b+3] (row 4s+t, columns 4b..4b+3).
+    // Per panel of 4 columns the panel is factored with broadcasts as above;
+    // every trailing tile then takes the panel's rank-4 update on the matrix
+    // cores: v_mfma_f32_4x4x1_16b_f32 forms 16 independent 4x4 outer
+    // products per wave, one per group, D[i][j] (register i, lane 4g+j) +=
+    // A[i] (lane 4g+i) * B[j] (lane 4g+j).  With A = column entries of tile
+    // row b and B = those of tile row s, register i of lane t receives
+    // (-L[4b+i][k]) (-L[4s+t][k]), the update of entry (4s+t, 4b+i).  Each
+    // entry takes its updates in column order, as in factor().
+    static __device__ __forceinline__ bool factor_mfma(T (&L)[TOT], T (&D)[S], T& minpiv, int t) {
+        static_assert(G == 4, "4x4 tiles need groups of 4 lanes");
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        bool ok = true;
+        minpiv = T(3.0e38);
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = 4 * p + q;
+                const T piv = -gbc<G>(L[slot_off<G>(p) + j], q);
+                ok = ok && (piv > T(0));
+                minpiv = piv < minpiv ? piv : minpiv;
+                const T rs = rsqrt_t(piv);
+                if (t == q) D[p] = rs;
+#pragma unroll
+                for (int s = p; s < S; ++s) L[slot_off<G>(s) + j] *= rs;
+#pragma unroll
+                for (int c = j + 1; c < 4 * p + 4; ++c) {
+                    const T lc = gbc<G>(L[slot_off<G>(p) + j], c % 4);
+#pragma unroll
+                    for (int s = p; s < S; ++s)
+                        L[slot_off<G>(s) + c] = fmadd(L[slot_off<G>(s) + j], lc, L[slot_off<G>(s) + c]);
+                }
+            }
+#pragma unroll
+            for (int b = p + 1; b < S; ++b) {
+#pragma unroll
+                for (int s = b; s < S; ++s) {
+                    T* tile = &L[slot_off<G>(s) + 4 * b];
+                    f4 acc = {tile[0], tile[1], tile[2], tile[3]};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        acc = __builtin_amdgcn_mfma_f32_4x4x1f32(L[slot_off<G>(b) + 4 * p + k],
+                                                                 L[slot_off<G>(s) + 4 * p + k], acc, 0, 0, 0);
+                    tile[0] = acc[0];
+                    tile[1] = acc[1];
+                    tile[2] = acc[2];
+                    tile[3] = acc[3];
+                }
             }
         }
         return ok;
@@ -176,14 +240,67 @@ struct Chol {
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) {
             const int q = j % G, sj = j / G;
-            const V yj = gbc<G>(z[sj] * (V)D[sj], q);
+            const V yl = z[sj] * (V)D[sj];  // y_j on its owner
+            const V yj = gbc<G>(yl, q);
 #pragma unroll
             for (int s = sj; s < S; ++s) {
-                const V upd = z[s] - (V)L[slot_off<G>(s) + j] * yj;
+                const V upd = fmadd((V)L[slot_off<G>(s) + j], yj, z[s]);
                 if (s == sj)
-                    z[s] = (t == q) ? yj : ((t > q) ? upd : z[s]);
+                    z[s] = (t == q) ? yl : ((t > q) ? upd : z[s]);
                 else
                     z[s] = upd;
+            }
+        }
+    }
+
+    // Two right-hand sides at once: independent chains interleaved.
+    template <typename V>
+    static __device__ __forceinline__ void forward2(const T (&L)[TOT], const T (&D)[S], V (&z)[S], V (&u)[S],
+                                                    int t) {
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            const int q = j % G, sj = j / G;
+            const V zl = z[sj] * (V)D[sj], ul = u[sj] * (V)D[sj];
+            const V zj = gbc<G>(zl, q), uj = gbc<G>(ul, q);
+#pragma unroll
+            for (int s = sj; s < S; ++s) {
+                const V lv = (V)L[slot_off<G>(s) + j];
+                const V zu = fmadd(lv, zj, z[s]), uu = fmadd(lv, uj, u[s]);
+                if (s == sj) {
+                    z[s] = (t == q) ? zl : ((t > q) ? zu : z[s]);
+                    u[s] = (t == q) ? ul : ((t > q) ? uu : u[s]);
+                } else {
+                    z[s] = zu;
+                    u[s] = uu;
+                }
+            }
+        }
+    }
+
+    template <typename V>
+    static __device__ __forceinline__ void backward2(const T (&L)[TOT], const T (&D)[S], V (&x)[S], V (&y)[S],
+                                                     int t) {
+#pragma unroll
+        for (int i = NMAX - 1; i >= 0; --i) {
+            const int q = i % G, si = i / G;
+            V px = V(0), py = V(0);
+#pragma unroll
+            for (int s = si; s < S; ++s) {
+                const V lv = (V)L[slot_off<G>(s) + i];
+                const V ax = lv * x[s], ay = lv * y[s];
+                if (s == si) {
+                    px += (t > q) ? ax : V(0);
+                    py += (t > q) ? ay : V(0);
+                } else {
+                    px += ax;
+                    py += ay;
+                }
+            }
+            px = gsum<G>(px);
+            py = gsum<G>(py);
+            if (t == q) {
+                x[si] = (x[si] + px) * (V)D[si];
+                y[si] = (y[si] + py) * (V)D[si];
             }
         }
     }
@@ -204,7 +321,7 @@ struct Chol {
                     p += prod;
             }
             p = gsum<G>(p);
-            if (t == q) x[si] = (x[si] - p) * (V)D[si];
+            if (t == q) x[si] = (x[si] + p) * (V)D[si];
         }
     }
 };
@@ -301,7 +418,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
                 v = (T)a.ctab32[center + tr[s] - tcol];
             else
                 v = (T)a.ctab[center + tr[s] - tcol];
-            L[slot_off<G>(s) + c] = live ? v : ((r == c) ? T(1) : T(0));
+            L[slot_off<G>(s) + c] = live ? -v : ((r == c) ? T(-1) : T(0));
         }
     }
     const unsigned c0x = (unsigned)OFF_BIAS * 8u, c0y = (unsigned)OFF_BIAS * (OX * 8u),
@@ -312,7 +429,11 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         bq[s] = (G * s + t < n) ? v : 0.0;
     }
     T minpiv;
-    const bool ok = CH::factor(L, D, minpiv, t);
+    bool ok;
+    if constexpr (MIXED && G == 4)
+        ok = CH::factor_mfma(L, D, minpiv, t);
+    else
+        ok = CH::factor(L, D, minpiv, t);
     if (MIXED && (!ok || !((double)minpiv >= a.pivot_tol * a.c0))) {
         if (t == 0) {
             const int pos = atomicAdd(a.n_fallback, 1);
@@ -325,7 +446,22 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         return true;
     }
     double sv[S], ov[S];
-    {
+    if (OKK) {
+        // A s = b and A o = 1 side by side
+        T z[S], u[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            z[s] = (T)bq[s];
+            u[s] = (G * s + t < n) ? T(1) : T(0);
+        }
+        CH::forward2(L, D, z, u, t);
+        CH::backward2(L, D, z, u, t);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            sv[s] = (double)z[s];
+            ov[s] = (double)u[s];
+        }
+    } else {
         T z[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) z[s] = (T)bq[s];
@@ -333,15 +469,6 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         CH::backward(L, D, z, t);
 #pragma unroll
         for (int s = 0; s < S; ++s) sv[s] = (double)z[s];
-    }
-    if (OKK) {
-        T z[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) z[s] = (G * s + t < n) ? T(1) : T(0);
-        CH::forward(L, D, z, t);
-        CH::backward(L, D, z, t);
-#pragma unroll
-        for (int s = 0; s < S; ++s) ov[s] = (double)z[s];
     }
     // Weights from the (possibly float) solutions.
     if (OKK) {
