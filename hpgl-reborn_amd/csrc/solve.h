@@ -347,12 +347,15 @@ __device__ __forceinline__ unsigned adiff(unsigned a, unsigned b, unsigned acc) 
 // offset difference with row strides ox (x fastest) and oxy.  Each neighbour
 // offset is kept as three biased, pre-scaled components
 // ((d + OFF_BIAS) * stride * 8), so a pair's byte address is the sum of three
-// absolute differences.
+// absolute differences.  In the packed form (PACK, the LDS table of the
+// template path, whose offsets satisfy |d| <= E/2) x and y share one word as
+// two 16-bit fields ((d + E/2) * stride * 8, each below 32768), and one
+// v_sad_u16 forms both their absolute differences.
 constexpr int OFF_BIAS = 1024;
 
 // Weights of one cell: w[s] for the lane's rows, mu (OK), status.  Returns
 // false when the mixed form hands the cell to the FP64 form.
-template <int NMAX, int G, bool MIXED, int KIND, bool LTAB>
+template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK>
 __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, const double* ltab,
                                              unsigned ox, unsigned oxy, double (&w)[NMAX / G], double (&bq)[NMAX / G],
                                              int (&dnode)[NMAX / G], double& mu, int& status) {
@@ -383,14 +386,26 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         if (r >= n) dx = dy = dz = 0;
         tr[s] = (dz * SY + dy) * SX + dx;
         dnode[s] = (dz * g.ny + dy) * g.nx + dx;
-        ux[s] = (unsigned)(dx + OFF_BIAS) * 8u;
-        uy[s] = (unsigned)(dy + OFF_BIAS) * (OX * 8u);
+        if (PACK) {
+            ux[s] = (unsigned)(dx + (a.ex >> 1)) * 8u | ((unsigned)(dy + (a.ey >> 1)) * (OX * 8u)) << 16;
+            uy[s] = 0u;
+        } else {
+            ux[s] = (unsigned)(dx + OFF_BIAS) * 8u;
+            uy[s] = (unsigned)(dy + OFF_BIAS) * (OX * 8u);
+        }
         uz[s] = (unsigned)(dz + OFF_BIAS) * (OXY * 8u);
     }
     const char* lb = (const char*)ltab;
-    // LDS covariance of my slot s with the neighbour (cx, cy, cz).
+    // The packed form reads LDS only: its 32-bit base is the accumulator of
+    // the first absolute difference.
+    typedef const double __attribute__((address_space(3)))* lds_dp;
+    const unsigned lbase = PACK ? (unsigned)(uintptr_t)(lds_dp)ltab : 0u;
+    // Covariance of my slot s with the neighbour (cx, cy, cz).
     auto lcov = [&](int s, unsigned cx, unsigned cy, unsigned cz) -> double {
-        return *(const double*)(lb + adiff(ux[s], cx, adiff(uy[s], cy, adiff(uz[s], cz, 0u))));
+        if constexpr (PACK)
+            return *(lds_dp)(uintptr_t)adiff(uz[s], cz, __builtin_amdgcn_sad_u16(ux[s], cx, lbase));
+        else
+            return *(const double*)(lb + adiff(ux[s], cx, adiff(uy[s], cy, adiff(uz[s], cz, 0u))));
     };
     T L[CH::TOT];
     T D[S];
@@ -400,7 +415,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         unsigned cx = 0, cy = 0, cz = 0;
         if (LTAB) {
             cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
-            cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+            cy = PACK ? 0u : (unsigned)gbc<G>((int)uy[c / G], c % G);
             cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
         } else {
             tcol = gbc<G>(tr[c / G], c % G);
@@ -421,8 +436,9 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
             L[slot_off<G>(s) + c] = live ? -v : ((r == c) ? T(-1) : T(0));
         }
     }
-    const unsigned c0x = (unsigned)OFF_BIAS * 8u, c0y = (unsigned)OFF_BIAS * (OX * 8u),
-                   c0z = (unsigned)OFF_BIAS * (OXY * 8u);
+    const unsigned c0x = PACK ? (unsigned)(a.ex >> 1) * 8u | ((unsigned)(a.ey >> 1) * (OX * 8u)) << 16
+                              : (unsigned)OFF_BIAS * 8u;
+    const unsigned c0y = (unsigned)OFF_BIAS * (OX * 8u), c0z = (unsigned)OFF_BIAS * (OXY * 8u);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const double v = LTAB ? lcov(s, c0x, c0y, c0z) : a.ctab[center + tr[s]];
@@ -505,7 +521,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
                     unsigned cx = 0, cy = 0, cz = 0;
                     if (LTAB) {
                         cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
-                        cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+                        cy = PACK ? 0u : (unsigned)gbc<G>((int)uy[c / G], c % G);
                         cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
                     } else {
                         tcol = gbc<G>(tr[c / G], c % G);
@@ -551,7 +567,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
                     unsigned cx = 0, cy = 0, cz = 0;
                     if (LTAB) {
                         cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
-                        cy = (unsigned)gbc<G>((int)uy[c / G], c % G);
+                        cy = PACK ? 0u : (unsigned)gbc<G>((int)uy[c / G], c % G);
                         cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
                     } else {
                         tcol = gbc<G>(tr[c / G], c % G);
@@ -584,7 +600,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 // Kriges one cell with the group of G lanes; t = lane index in the group,
 // cnt = the cell's count[] entry, ltab/ox/oxy = the octant table and its
 // strides (LTAB form).
-template <int NMAX, int G, bool MIXED, int KIND, bool LTAB>
+template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK = false>
 __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int cnt, int t, const double* ltab,
                                            unsigned ox, unsigned oxy) {
     constexpr int S = NMAX / G;
@@ -606,7 +622,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
     if (n == 0) {
         status = ST_NO_NEIGHBOURS;
-    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB>(a, cell, t, n, ltab, ox, oxy, w, bq, dnode, mu, status)) {
+    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB, PACK>(a, cell, t, n, ltab, ox, oxy, w, bq, dnode, mu, status)) {
         return;  // queued for the FP64 form
     }
     if (status != ST_KRIGED) {
@@ -769,7 +785,7 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
 // Persistent form with the octant covariance table staged in LDS: every
 // template-path cell; cluster-path cells (pair offsets beyond the octant
 // table) are left to k_solve over the cluster cell list.
-template <int NMAX, int G, bool MIXED, int KIND, int BLOCK>
+template <int NMAX, int G, bool MIXED, int KIND, int BLOCK, bool PACK>
 __global__ __launch_bounds__(BLOCK) void k_solve_lds(SolveArgs a, int nelem) {
     extern __shared__ double otab_lds[];
     for (int i = threadIdx.x; i < nelem; i += BLOCK) otab_lds[i] = a.otab[i];
@@ -781,7 +797,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_lds(SolveArgs a, int nelem) {
     for (long long cell = gid; cell < a.g.n; cell += ngroups) {
         const int cnt = a.count[cell];
         if (cnt >= 0 && (cnt & CNT_CLUSTER)) continue;
-        solve_cell<NMAX, G, MIXED, KIND, true>(a, cell, cnt, t, otab_lds, ox, oxy);
+        solve_cell<NMAX, G, MIXED, KIND, true, PACK>(a, cell, cnt, t, otab_lds, ox, oxy);
     }
 }
 

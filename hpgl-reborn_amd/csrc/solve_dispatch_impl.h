@@ -44,11 +44,11 @@ void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st)
 
 // Persistent LDS-table form: as many blocks as are resident at once.  Returns
 // false (nothing launched) when the table does not fit a block's LDS.
-template <int NMAX, int G, bool MIXED, int KIND, int BLOCK>
+template <int NMAX, int G, bool MIXED, int KIND, int BLOCK, bool PACK>
 bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipStream_t st) {
     static std::mutex mu;
     static std::map<std::pair<int, int>, int> grids;  // (device, LDS bytes) -> grid
-    const void* fn = (const void*)&k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>;
+    const void* fn = (const void*)&k_solve_lds<NMAX, G, MIXED, KIND, BLOCK, PACK>;
     const int lds = nelem * (int)sizeof(double);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
@@ -70,7 +70,7 @@ bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipS
         }
     }
     if (grid <= 0) return false;
-    hipExtLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK),
+    hipExtLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK, PACK>), dim3((unsigned)grid), dim3(BLOCK),
                           (uint32_t)lds, st, ev[0], ev[1], 0, sa, nelem);
     return true;
 }
@@ -82,10 +82,18 @@ bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipS
 template <int NMAX, int G64, int G32, int KIND>
 void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev, hipStream_t st) {
     constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
+    // Packed x/y table addresses when both 16-bit fields stay below 32768:
+    // x up to E_x * 8, y up to E_y * (E_x + 1) * 8.
+    const bool pack = (long long)sa.ex * 8 < 32768 && (long long)sa.ey * (sa.ex + 1) * 8 < 32768;
     bool lds = false;
-    if (nelem > 0)
-        lds = mixed ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32>(sa, nelem, ev, st)
-                    : launch_solve_lds<NMAX, G64, false, KIND, 256>(sa, nelem, ev, st);
+    if (nelem > 0) {
+        if (mixed)
+            lds = pack ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32, true>(sa, nelem, ev, st)
+                       : launch_solve_lds<NMAX, G32, true, KIND, BLOCK32, false>(sa, nelem, ev, st);
+        else
+            lds = pack ? launch_solve_lds<NMAX, G64, false, KIND, 256, true>(sa, nelem, ev, st)
+                       : launch_solve_lds<NMAX, G64, false, KIND, 256, false>(sa, nelem, ev, st);
+    }
     if (!lds) {
         if (mixed)
             launch_solve<NMAX, G32, true, KIND>(sa, n, ev, st);

@@ -149,6 +149,27 @@ def test_rotated_model_global_table(orc):
     assert np.abs(gm["est64"][kr] - o["est64"][kr]).max() <= 1e-6
 
 
+@pytest.mark.parametrize("radius", [31, 32])
+def test_table_address_forms(orc, radius):
+    """The LDS table's packed x/y addresses need E_y (E_x + 1) 8 < 32768
+    (E = 2 r): radius 31 packs with fields just under the bound, radius 32
+    takes the unpacked LDS form; both must match the oracle."""
+    wl = _wl("ok", 0.3)
+    wl.radiuses = (radius, radius, 1)
+    wl.k = 16
+    o = _oracle(orc, wl)
+    g = _device(wl)
+    kr = o["status"] == orc.ST_KRIGED
+    assert kr.sum() > 0
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    assert np.array_equal(_node_lists(wl, g["count"], g["nboff"])[kr], o["nb"][kr].astype(np.int64))
+    assert np.allclose(g["est64"][kr], o["est64"][kr], rtol=1e-9, atol=1e-12)
+    assert np.allclose(g["variance"][kr], o["var"][kr], rtol=1e-9, atol=1e-12)
+    gm = _device(wl, "mixed")
+    assert np.array_equal(gm["status"], o["status"].astype(np.uint8))
+    assert np.abs(gm["est64"][kr] - o["est64"][kr]).max() <= 1e-6
+
+
 @pytest.mark.parametrize("name,scale", CASES)
 def test_estimates_fp64(orc, name, scale):
     wl = _wl(name, scale)
