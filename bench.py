@@ -41,8 +41,8 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (packed)
 # known size (--calibrate-copy; profiles/hbm_traffic_ok_100x100x50.md):
 # FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, KiB x 1024.
 TRAFFIC_BYTES = {
-    "k_solve_lds<24,4,mixed,OK>": 71.96e6,
-    "k_solve_lds<24,4,fp64,OK>": 70.92e6,
+    "k_solve_lds<24,4,mixed,OK>": 71.81e6,
+    "k_solve_lds<24,4,fp64,OK>": 70.85e6,
 }
 # Algorithmic bytes of one solve launch: per kriged cell 4 B count + 96 B
 # offsets in, 13 B estimate/mask/variance out; counts of the informed cells,
