@@ -332,9 +332,15 @@ __global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int head) {
         if (!__any(want)) break;
         const int tu = __builtin_amdgcn_readfirstlane(t0);
         // 32 probes into a bit mask (mask bytes are 0 or 1), no branches.
+        // Every probe of the step is issued before the first is used, so the
+        // step waits for the LDS once.
+        unsigned pv[TMPL_U];
+#pragma unroll
+        for (int u = 0; u < TMPL_U; ++u) pv[u] = box[lbase + tdel[tu + u]];
+        __builtin_amdgcn_sched_barrier(0);
         unsigned bits = 0;
 #pragma unroll
-        for (int u = 0; u < TMPL_U; ++u) bits |= (unsigned)box[lbase + tdel[tu + u]] << u;
+        for (int u = 0; u < TMPL_U; ++u) bits |= pv[u] << u;
         const int left = a.tsize - t0;
         if (left < TMPL_U) bits &= (1u << left) - 1u;
         if (!want) bits = 0;
