@@ -2,9 +2,10 @@
 
     python hpgl-reborn_amd/build.py
 
-* plan.cpp and variogram_plan.cpp are compiled with g++ -ffp-contract=off:
-  they produce the covariance keys that order the search template and the
-  variogram's tunnel and lag decisions, which must round exactly as the
+* plan.cpp, variogram_plan.cpp and sgs_plan.cpp are compiled with g++
+  -ffp-contract=off: they produce the covariance keys that order the search
+  template, the variogram's tunnel and lag decisions and the simulation's
+  deviates and normal-score transforms, which must round exactly as the
   reference's own g++-built code does.
 * The .hip sources are compiled with hipcc for gfx950 only, with packed FP32
   instructions disabled.
@@ -40,14 +41,14 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for cpp in ("plan.cpp", "variogram_plan.cpp"):
+    for cpp in ("plan.cpp", "variogram_plan.cpp", "sgs_plan.cpp"):
         o = os.path.join(BUILD, cpp.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, cpp), "-o", o])
         objs.append(o)
     # the HIP translation units compile in parallel
     procs = []
-    units = ["krige.hip", "loo.hip", "api.hip", "inc_io.hip", "variogram.hip", "loo_m0.hip", "loo_m1.hip",
-             "loo_m2.hip"]
+    units = ["krige.hip", "loo.hip", "api.hip", "inc_io.hip", "variogram.hip", "sgs.hip", "loo_m0.hip",
+             "loo_m1.hip", "loo_m2.hip"]
     units += [f"solve_n{n}_k{k}.hip" for n in (32, 24, 16, 8) for k in (0, 1, 2)]
     for src in units:
         o = os.path.join(BUILD, src.replace(".hip", ".o"))

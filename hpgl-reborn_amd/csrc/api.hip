@@ -22,6 +22,7 @@
 #include "krige_launch.h"
 #include "loo.h"
 #include "plan.h"
+#include "sgs.h"
 #include "variogram_plan.h"
 
 namespace hpglx {
@@ -561,6 +562,55 @@ bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned ch
     return true;
 }
 
+// Sequential Gaussian simulation over host copies of the property
+// (sequential_gaussian_simulation.cpp:18-78): the normal-score transforms and
+// the path, draws and levels on the host, the levels on the GPU.  v and m (N)
+// are updated in place; simmask (host, N) may be null.
+int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vector<float>& v,
+            std::vector<unsigned char>& m, const unsigned char* simmask, const hpgl_non_parametric_cdf_t* cdf,
+            hipStream_t st, std::string& err) {
+    if (p->m_kriging_kind != KIND_OK && p->m_kriging_kind != KIND_SK) {
+        err = "sequential simulation takes simple (1) or ordinary (0) kriging";
+        return -1;
+    }
+    CovParams c;
+    c.type = p->m_covariance_type;
+    for (int i = 0; i < 3; ++i) {
+        c.ranges[i] = p->m_ranges[i];
+        c.angles[i] = p->m_angles[i];
+    }
+    c.sill = p->m_sill;
+    c.nugget = p->m_nugget;
+    // Both kinds krige over the precalculated table of the radius box, zero
+    // outside it (sequential_simulation.h:72): the simple-kriging plan.
+    DevPlan* plan = get_plan(L, KIND_SK, c, p->m_radiuses, p->m_max_neighbours, err);
+    if (!plan) return -1;
+    const long long n = (long long)v.size();
+    const bool transform = cdf && cdf->m_size > 0 && cdf->m_values && cdf->m_probs;
+    if (transform) sgs_to_normal_scores(v.data(), m.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
+    SgsSchedule s;
+    if (!sgs_schedule(dims, p->m_radiuses, m.data(), simmask, p->m_seed, s, err)) return -1;
+    auto* dv = (float*)L.ws.get("sgs_io_val", (size_t)n * 4, err);
+    auto* dm = (unsigned char*)L.ws.get("sgs_io_mask", (size_t)n, err);
+    if (!err.empty()) return -1;
+    if (!ensure_events(L, err)) return -1;
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess ||
+        hipMemcpyAsync(dv, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dm, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        err = "host to device copy failed";
+        return -1;
+    }
+    if (sgs_run(*plan, L.ws, p->m_kriging_kind, dims, dv, dm, s, st, err) != 0) return -1;
+    if (hipMemcpyAsync(v.data(), dv, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(m.data(), dm, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(L.done, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+        err = "device to host copy failed";
+        return -1;
+    }
+    if (transform) sgs_from_normal_scores(v.data(), m.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
+    return 0;
+}
+
 template <typename F>
 void timed_call(Library& L, const char* name, long long cells, F&& f) {
     auto t0 = std::chrono::steady_clock::now();
@@ -702,6 +752,41 @@ void hpgl_simple_kriging(float* input_data, unsigned char* input_mask, hpgl_shap
             e = "device to host copy failed";
             return -1;
         }
+        return 0;
+    });
+}
+
+// api.cpp:485-516.  The stationary mean is accepted and, as in the
+// reference, not used (simple kriging runs with mean 0 in normal-score space).
+void hpgl_sgs_simulation(hpgl_cont_masked_array_t* data, hpgl_sgs_params_t* params, hpgl_non_parametric_cdf_t* cdf,
+                         double* mean, hpgl_ubyte_array_t* simulation_mask) {
+    (void)mean;
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(&data->m_shape, err);
+    if (n < 0) {
+        L.last_error = err;
+        return;
+    }
+    if (simulation_mask && shape_volume(&simulation_mask->m_shape, err) != n) {
+        L.last_error = err.empty() ? "simulation mask and property sizes differ" : err;
+        return;
+    }
+    timed_call(L, "Sequential Gaussian Simulation", n, [&](std::string& e) -> int {
+        if (!L.io_stream && hipStreamCreateWithFlags(&L.io_stream, hipStreamNonBlocking) != hipSuccess) {
+            e = "hipStreamCreate failed";
+            return -1;
+        }
+        std::vector<float> v(data->m_data, data->m_data + n);
+        std::vector<unsigned char> m(data->m_mask, data->m_mask + n);
+        const int dims[3] = {data->m_shape.m_data[0], data->m_shape.m_data[1], data->m_shape.m_data[2]};
+        if (run_sgs(L, params, dims, v, m, simulation_mask ? simulation_mask->m_data : nullptr, cdf, L.io_stream,
+                    e) != 0)
+            return -1;
+        std::memcpy(data->m_data, v.data(), (size_t)n * 4);
+        std::memcpy(data->m_mask, m.data(), (size_t)n);
         return 0;
     });
 }
@@ -1108,6 +1193,33 @@ int hpglx_indicator_kriging(const hpgl_ik_params_t* params, int ncat, const int 
     if (run_indicator(L, params, ncat, dims, data, mask, out_cat, out_mask, out_prob, precision, (hipStream_t)stream,
                       err) != 0)
         return fail(L, err);
+    return 0;
+}
+
+int hpglx_sgs_simulation(float* values, unsigned char* mask, const int dims[3], const hpgl_sgs_params_t* params,
+                         const hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!values || !mask || !dims || !params) return fail(L, "hpglx_sgs_simulation: null argument");
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || n > 2147483647LL)
+        return fail(L, "hpglx_sgs_simulation: invalid grid dimensions");
+    const hipStream_t st = (hipStream_t)stream;
+    std::vector<float> v((size_t)n);
+    std::vector<unsigned char> m((size_t)n), sm(simulation_mask ? (size_t)n : 0);
+    if (hipMemcpyAsync(v.data(), values, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(m.data(), mask, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (simulation_mask &&
+         hipMemcpyAsync(sm.data(), simulation_mask, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(L, "hpglx_sgs_simulation: device to host copy failed");
+    if (run_sgs(L, params, dims, v, m, simulation_mask ? sm.data() : nullptr, cdf, st, err) != 0) return fail(L, err);
+    if (hipMemcpyAsync(values, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(mask, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(L, "hpglx_sgs_simulation: host to device copy failed");
     return 0;
 }
 

@@ -1,0 +1,208 @@
+// sgs.hip -- sequential Gaussian simulation, level by level (see sgs.h).
+//
+// Reference: src/geo_bsd/hpgl/sequential_simulation.h:51-119 (the walk),
+// kriging_interpolation.h:269-313 (search, weights, combine with variance),
+// sequential_gaussian_simulation.cpp:18-78 (SK with mean 0 or OK, both over
+// the precalculated covariance table of the radius box), sample.h:14-25 and
+// gaussian_distribution.h:56-89 (the draw).
+#include "sgs.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "krige_kernels.h"
+#include "solve_dispatch.h"
+
+namespace hpglx {
+namespace {
+
+// Padded 0/1 mask (halo of r cells, x pitch px) from the cell mask.
+__global__ void k_sgs_pmask(GridDims g, const unsigned char* mask, unsigned char* pmask) {
+    const long long P = (long long)g.px * g.py * g.pz;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % g.px) - g.rx, y = (int)((i / g.px) % g.py) - g.ry, z = (int)(i / ((long long)g.px * g.py)) - g.rz;
+        const bool in = x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz;
+        pmask[i] = in ? (mask[((long long)z * g.ny + y) * g.nx + x] != 0) : 0;
+    }
+}
+
+// Template entries as deltas in the padded mask and as packed offsets.
+__global__ void k_sgs_tdelta(GridDims g, int T, const int* tdx, const int* tdy, const int* tdz, int* tdelta,
+                             int* tpack) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    tdelta[t] = (tdz[t] * g.py + tdy[t]) * g.px + tdx[t];
+    tpack[t] = pack_off(tdx[t], tdy[t], tdz[t]);
+}
+
+// One wave per listed cell: the template is probed 64 entries at a time and
+// the hits, ranked by a ballot, are the first kmax informed cells in template
+// order (sugarbox_neighbour_lookup.h:37-62).
+__global__ __launch_bounds__(256) void k_sgs_search(GridDims g, int kmax, int T, const int* tdelta, const int* tpack,
+                                                    const unsigned char* pmask, const int* cells, int ncells,
+                                                    int* count, int* nboff) {
+    const int lane = threadIdx.x & 63;
+    const int w = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (w >= ncells) return;  // whole waves leave together
+    const int cell = cells[w];
+    const int x = cell % g.nx, y = (cell / g.nx) % g.ny, z = cell / (g.nx * g.ny);
+    const long long pbase = ((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
+    int cnt = 0;
+    for (int t0 = 0; t0 < T && cnt < kmax; t0 += 64) {
+        const int t = t0 + lane;
+        const bool hit = t < T && pmask[pbase + tdelta[t]] != 0;
+        const unsigned long long b = __ballot(hit);
+        const int rank = cnt + __popcll(b & ((1ull << lane) - 1ull));
+        if (hit && rank < kmax) nboff[(long long)cell * kmax + rank] = tpack[t];
+        cnt += __popcll(b);
+    }
+    if (lane == 0) count[cell] = cnt < kmax ? cnt : kmax;
+}
+
+// The draw of each listed cell (sample.h:14-25): a kriged cell takes
+// gaussian_cdf_t(estimate, variance).inverse(u) = z sqrt(variance) + estimate
+// (products and sums rounded singly, as the reference's host code), any other
+// the standard deviate; a draw outside (0, 1) (z = NaN) gives the mean itself.
+// The value becomes data: mask and padded mask are set.
+__global__ void k_sgs_sample(GridDims g, const int* cells, int ncells, const double* z, const unsigned char* status,
+                             const float* est, const double* var, float* values, unsigned char* mask,
+                             unsigned char* pmask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells) return;
+    const int cell = cells[i];
+    const double d = z[i];
+    float v;
+    if (status[cell] == ST_KRIGED) {
+        const double m = (double)est[cell];
+        v = isnan(d) ? (float)m : (float)__dadd_rn(__dmul_rn(d, __dsqrt_rn(var[cell])), m);
+    } else {
+        v = isnan(d) ? 0.0f : (float)d;
+    }
+    values[cell] = v;
+    mask[cell] = 1;
+    const int x = cell % g.nx, y = (cell / g.nx) % g.ny, zc = cell / (g.nx * g.ny);
+    pmask[((long long)(zc + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx)] = 1;
+}
+
+unsigned blocks_of(long long work, int per) {
+    return (unsigned)std::max(1LL, (work + per - 1) / per);
+}
+
+}  // namespace
+
+int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_values, unsigned char* d_mask,
+            const SgsSchedule& s, hipStream_t st, std::string& err) {
+    const PlanHost& h = plan.h;
+    if (kind != KIND_OK && kind != KIND_SK) {
+        err = "sequential simulation takes simple or ordinary kriging";
+        return -1;
+    }
+    if (h.kmax > 32) {
+        err = "max_neighbours above 32 is not supported by the GPU solver";
+        return -1;
+    }
+    if (!plan.upload(err)) return -1;
+    GridDims g;
+    g.nx = dims[0];
+    g.ny = dims[1];
+    g.nz = dims[2];
+    g.n = (long long)g.nx * g.ny * g.nz;
+    g.rx = h.r[0];
+    g.ry = h.r[1];
+    g.rz = h.r[2];
+    g.px = g.nx + 2 * g.rx;
+    g.py = g.ny + 2 * g.ry;
+    g.pz = g.nz + 2 * g.rz;
+    g.cx = g.cy = g.cz = 0;
+    const long long n = g.n, P = (long long)g.px * g.py * g.pz;
+    const int K = h.kmax, T = plan.tsize;
+    const long long m = (long long)s.cells.size();
+    const int L = (int)s.level_off.size() - 1;
+    if (m == 0) return 0;
+    auto* pmask = (unsigned char*)ws.get("sgs_pmask", (size_t)P, err);
+    auto* count = (int*)ws.get("sgs_count", (size_t)n * 4, err);
+    auto* nboff = (int*)ws.get("sgs_nboff", (size_t)n * K * 4, err);
+    auto* est = (float*)ws.get("sgs_est", (size_t)n * 4, err);
+    auto* var = (double*)ws.get("sgs_var", (size_t)n * 8, err);
+    auto* status = (unsigned char*)ws.get("sgs_status", (size_t)n, err);
+    auto* tdelta = (int*)ws.get("sgs_tdelta", (size_t)std::max(T, 1) * 4, err);
+    auto* tpack = (int*)ws.get("sgs_tpack", (size_t)std::max(T, 1) * 4, err);
+    auto* cells = (int*)ws.get("sgs_cells", (size_t)m * 4, err);
+    auto* zdev = (double*)ws.get("sgs_z", (size_t)m * 8, err);
+    auto* lcount = (int*)ws.get("sgs_level_count", (size_t)L * 4, err);
+    if (!err.empty()) return -1;
+    std::vector<int> lc((size_t)L);
+    for (int l = 0; l < L; ++l) lc[(size_t)l] = s.level_off[(size_t)l + 1] - s.level_off[(size_t)l];
+    if (hipMemcpyAsync(cells, s.cells.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(zdev, s.z.data(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(lcount, lc.data(), (size_t)L * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+        err = "host to device copy failed";
+        return -1;
+    }
+    hipLaunchKernelGGL(k_sgs_pmask, dim3((unsigned)std::min<long long>(4096, blocks_of(P, 256))), dim3(256), 0, st, g,
+                       (const unsigned char*)d_mask, pmask);
+    if (T > 0)
+        hipLaunchKernelGGL(k_sgs_tdelta, dim3(blocks_of(T, 256)), dim3(256), 0, st, g, T, (const int*)plan.tdx,
+                           (const int*)plan.tdy, (const int*)plan.tdz, tdelta, tpack);
+
+    SolveArgs so;
+    std::memset(&so, 0, sizeof so);
+    so.g = g;
+    so.kind = kind;
+    so.kmax = K;
+    so.values = d_values;
+    so.mask = d_mask;
+    so.mean = 0.0;  // SK: single_mean_t(0) (sequential_gaussian_simulation.cpp:37-43)
+    so.count = count;
+    so.nboff = nboff;
+    so.ctab = plan.ctab;
+    so.ctab32 = plan.ctab32;
+    so.wx = h.W[0];
+    so.wy = h.W[1];
+    so.wz = h.W[2];
+    so.otab = nullptr;
+    so.otab_w = plan.otab_w;
+    so.ex = h.E[0];
+    so.ey = h.E[1];
+    so.ez = h.E[2];
+    so.c0 = h.c0();
+    so.out_val = est;
+    so.out_var = var;
+    so.out_status = status;
+    const hipEvent_t no_ev[2] = {nullptr, nullptr};
+    for (int l = 0; l < L; ++l) {
+        const int off = s.level_off[(size_t)l], cnt = lc[(size_t)l];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_sgs_search, dim3(blocks_of((long long)cnt * 64, 256)), dim3(256), 0, st, g, K, T,
+                           (const int*)tdelta, (const int*)tpack, (const unsigned char*)pmask,
+                           (const int*)(cells + off), cnt, count, nboff);
+        so.cell_list = cells + off;
+        so.n_list = lcount + l;
+        if (K <= 8)
+            dispatch_solve_8(so, kind, false, cnt, 0, no_ev, st);
+        else if (K <= 16)
+            dispatch_solve_16(so, kind, false, cnt, 0, no_ev, st);
+        else if (K <= 24)
+            dispatch_solve_24(so, kind, false, cnt, 0, no_ev, st);
+        else
+            dispatch_solve_32(so, kind, false, cnt, 0, no_ev, st);
+        hipLaunchKernelGGL(k_sgs_sample, dim3(blocks_of(cnt, 256)), dim3(256), 0, st, g, (const int*)(cells + off),
+                           cnt, (const double*)(zdev + off), (const unsigned char*)status, (const float*)est,
+                           (const double*)var, d_values, d_mask, pmask);
+    }
+    if (hipGetLastError() != hipSuccess) {
+        err = "sequential simulation kernel launch failed";
+        return -1;
+    }
+    // the schedule's host vectors are read by the copies above
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        err = "sequential simulation failed on the device";
+        return -1;
+    }
+    return 0;
+}
+
+}  // namespace hpglx

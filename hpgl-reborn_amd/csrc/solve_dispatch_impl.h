@@ -82,6 +82,15 @@ bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipS
 template <int NMAX, int G64, int G32, int KIND>
 void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev, hipStream_t st) {
     constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
+    // A caller's own cell list (n = its largest length): the global-table
+    // form over the list only.
+    if (sa.cell_list) {
+        if (mixed)
+            launch_solve_list<NMAX, G32, true, KIND>(sa, n, st);
+        else
+            launch_solve_list<NMAX, G64, false, KIND>(sa, n, st);
+        return;
+    }
     // Packed x/y table addresses when both 16-bit fields stay below 32768:
     // x up to E_x * 8, y up to E_y * (E_x + 1) * 8.
     const bool pack = (long long)sa.ex * 8 < 32768 && (long long)sa.ey * (sa.ex + 1) * 8 < 32768;

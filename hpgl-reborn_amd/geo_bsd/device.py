@@ -213,6 +213,45 @@ def indicator_kriging(data, mask, grid, categories, marginal_probs, precision="f
     return out
 
 
+def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed, kriging_type="sk", cdf=None,
+                   simulation_mask=None, stream=None):
+    """Sequential Gaussian simulation in place on device tensors (work of
+    hpgl_sgs_simulation, api.cpp:485-516): float32 values and uint8 mask of
+    N cells are overwritten with the realization; the model is used as
+    given (geo_bsd.sgs_simulation scales it to unit sill first).  cdf =
+    (values, probs) host arrays of the normal-score transform, or None;
+    simulation_mask: uint8 tensor (1 = simulate) or None.  Returns the
+    number of cells informed afterwards."""
+    torch = _torch()
+    import numpy as np
+    from .hpgl_wrap import _HPGL_SGS_PARAMS, hpgl_non_parametric_cdf_t
+    nx, ny, nz = _dims(grid)
+    n = nx * ny * nz
+    _check_dev(values, torch.float32, n, "values")
+    _check_dev(mask, torch.uint8, n, "mask")
+    if simulation_mask is not None:
+        _check_dev(simulation_mask, torch.uint8, n, "simulation_mask")
+    _require(kriging_type in ("sk", "ok"), 'kriging_type must be "sk" or "ok"')
+    p = _HPGL_SGS_PARAMS(covariance_type=int(cov_model.type), ranges=tuple(float(r) for r in cov_model.ranges),
+                         angles=tuple(float(a) for a in cov_model.angles), sill=float(cov_model.sill),
+                         nugget=float(cov_model.nugget), radiuses=tuple(int(r) for r in radiuses),
+                         max_neighbours=int(max_neighbours), kriging_kind=1 if kriging_type == "sk" else 0,
+                         seed=int(seed), min_neighbours=0)
+    cdf_arg = None
+    if cdf is not None:
+        cv = np.ascontiguousarray(cdf[0], np.float32)
+        cp = np.ascontiguousarray(cdf[1], np.float32)
+        cdf_arg = C.byref(hpgl_non_parametric_cdf_t(values=cv.ctypes.data_as(C.POINTER(C.c_float)),
+                                                    probs=cp.ctypes.data_as(C.POINTER(C.c_float)), size=cv.size))
+    if stream is None:
+        stream = torch.cuda.current_stream(values.device).cuda_stream
+    rc = _hpgl_so.hpglx_sgs_simulation(_ptr(values), _ptr(mask), (C.c_int * 3)(nx, ny, nz), C.byref(p), cdf_arg,
+                                       _ptr(simulation_mask) if simulation_mask is not None else None, stream)
+    if rc != 0:
+        raise RuntimeError("sgs_simulation: " + _hpgl_so.hpglx_last_error().decode())
+    return int(mask.sum().item())
+
+
 def indicator_cdf(zfield, mask, grid, thresholds, marginal, radiuses, max_neighbours, cov_model, precision="fp64",
                   order_relations=True, **kw):
     """Median IK with several thresholds sharing one model: one solve per cell,

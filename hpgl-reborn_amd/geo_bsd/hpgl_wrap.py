@@ -60,6 +60,26 @@ class __hpgl_cov_params_t(C.Structure):
 _HPGL_COV_PARAMS = __hpgl_cov_params_t
 
 
+class _HPGL_UBYTE_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_ubyte)), ("shape", _HPGL_SHAPE)]
+
+
+class _HPGL_KRIGING_KIND:
+    ordinary = 0
+    simple = 1
+
+
+class _HPGL_SGS_PARAMS(C.Structure):
+    _fields_ = [("covariance_type", C.c_int), ("ranges", C.c_double * 3), ("angles", C.c_double * 3),
+                ("sill", C.c_double), ("nugget", C.c_double), ("radiuses", C.c_int * 3),
+                ("max_neighbours", C.c_int), ("kriging_kind", C.c_int), ("seed", C.c_long),
+                ("min_neighbours", C.c_int)]
+
+
+class hpgl_non_parametric_cdf_t(C.Structure):
+    _fields_ = [("values", C.POINTER(C.c_float)), ("probs", C.POINTER(C.c_float)), ("size", C.c_longlong)]
+
+
 class _HPGLX_GRID_REQUEST(C.Structure):
     """hpglx_grid_request_t (include/hpglx.h)."""
     _fields_ = [
@@ -170,6 +190,13 @@ _hpgl_so.hpgl_indicator_kriging.argtypes = [C.POINTER(_HPGL_IND_MASKED_ARRAY), C
 _hpgl_so.hpglx_indicator_kriging.restype = C.c_int
 _hpgl_so.hpglx_indicator_kriging.argtypes = [C.POINTER(_HPGL_IK_PARAMS), C.c_int, C.c_int * 3, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+_hpgl_so.hpgl_sgs_simulation.restype = None
+_hpgl_so.hpgl_sgs_simulation.argtypes = [C.POINTER(_HPGL_CONT_MASKED_ARRAY), C.POINTER(_HPGL_SGS_PARAMS),
+                                         C.POINTER(hpgl_non_parametric_cdf_t), C.POINTER(C.c_double),
+                                         C.POINTER(_HPGL_UBYTE_ARRAY)]
+_hpgl_so.hpglx_sgs_simulation.restype = C.c_int
+_hpgl_so.hpglx_sgs_simulation.argtypes = [C.c_void_p, C.c_void_p, C.c_int * 3, C.POINTER(_HPGL_SGS_PARAMS),
+                                          C.POINTER(hpgl_non_parametric_cdf_t), C.c_void_p, C.c_void_p]
 _hpgl_so.hpglx_template.restype = C.c_int
 _hpgl_so.hpglx_template.argtypes = [C.POINTER(__hpgl_cov_params_t), C.c_int * 3, C.c_void_p, C.c_int]
 _hpgl_so.hpglx_set_profiling.restype = None

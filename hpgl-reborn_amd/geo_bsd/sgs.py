@@ -1,0 +1,84 @@
+"""Sequential Gaussian simulation (the reference's sgs.py:52-160) on the
+MI355X path: hpgl_sgs_simulation in libhpgl.so."""
+import ctypes as C
+
+import numpy
+
+from .cdf import CdfData
+from .geo import ContProperty, CovarianceModel, _check_native, _clone_prop, _shape, accepts_tuple
+from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_KRIGING_KIND, _HPGL_SGS_PARAMS, _HPGL_UBYTE_ARRAY,
+                        _hpgl_so, hpgl_non_parametric_cdf_t)
+from .validation import GridValidator, ParameterValidator
+
+
+def _empty_clone(prop):
+    """geo.py _empty_clone: zero data, nothing informed."""
+    return ContProperty(numpy.zeros(prop.data.shape, dtype="float32", order="F"),
+                        numpy.zeros(prop.mask.shape, dtype="uint8", order="F"))
+
+
+def normed_cov_model(cov_model):
+    """sgs.py:43-50: the model scaled to unit sill."""
+    coef = cov_model.sill
+    return CovarianceModel(cov_model.type, cov_model.ranges, cov_model.angles, cov_model.sill / coef,
+                           cov_model.nugget / coef)
+
+
+@accepts_tuple("prop", 0)
+def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, seed, kriging_type="sk", mean=None,
+                   use_harddata=True, use_regions=False, region_size=None, mask=None, force_single_thread=False,
+                   force_parallel=False, min_neighbours=0, **params):
+    """Performs sequential Gaussian simulation (sgs.py:52-160).
+
+    cdf_data: None (no transform) or CdfData of the normal-score transform.
+    radiuses, max_neighbours, cov_model: the kriging search and model (the
+    model is scaled to unit sill).  seed: path and draws.  kriging_type:
+    "sk" (mean 0 in normal-score space) or "ok".  mean: accepted and unused,
+    as in the reference.  use_harddata: condition on the property's informed
+    cells.  mask: 1 where cells are simulated (None: every cell).
+    use_regions / force_* / min_neighbours: accepted for compatibility; the
+    reference's one-path algorithm ignores them as well.
+    Returns a new ContProperty.
+    """
+    GridValidator.validate_grid_dimensions(grid.x, grid.y, grid.z)
+    valid_radiuses = tuple(int(r) for r in ParameterValidator.validate_radius(radiuses, "radiuses"))
+    ParameterValidator.validate_max_neighbors(max_neighbours)
+    ParameterValidator.validate_covariance_parameters(cov_model.sill, cov_model.nugget, cov_model.ranges,
+                                                      cov_model.angles)
+    ParameterValidator.validate_seed(seed)
+    ParameterValidator.validate_min_neighbors(min_neighbours, max_neighbours)
+    if kriging_type not in ("sk", "ok"):
+        raise ValueError('kriging_type must be "sk" or "ok"')
+    if mean is not None and not numpy.isscalar(mean):
+        raise NotImplementedError("sgs_simulation: a locally varying mean is not supported")
+    prop.fix_shape(grid)
+    cov_model = normed_cov_model(cov_model)
+    out_prop = _clone_prop(prop) if use_harddata else _empty_clone(prop)
+    sgsp = _HPGL_SGS_PARAMS(covariance_type=cov_model.type, ranges=tuple(cov_model.ranges),
+                            angles=tuple(cov_model.angles), sill=cov_model.sill, nugget=cov_model.nugget,
+                            radiuses=valid_radiuses, max_neighbours=max_neighbours,
+                            kriging_kind={"sk": _HPGL_KRIGING_KIND.simple, "ok": _HPGL_KRIGING_KIND.ordinary}[
+                                kriging_type], seed=int(seed), min_neighbours=min_neighbours)
+    sh = _shape((grid.x, grid.y, grid.z))
+    cdf = None
+    keep = []
+    if cdf_data is not None:
+        assert isinstance(cdf_data, CdfData)
+        cv = numpy.ascontiguousarray(cdf_data.values, numpy.float32)
+        cp = numpy.ascontiguousarray(cdf_data.probs, numpy.float32)
+        keep += [cv, cp]
+        cdf = C.byref(hpgl_non_parametric_cdf_t(values=cv.ctypes.data_as(C.POINTER(C.c_float)),
+                                                probs=cp.ctypes.data_as(C.POINTER(C.c_float)), size=cv.size))
+    sim = None
+    if mask is not None:
+        m = numpy.require(mask, "uint8", "F")
+        if m.size != grid.x * grid.y * grid.z:
+            raise ValueError("sgs_simulation: the mask does not match the grid")
+        keep.append(m)
+        sim = C.byref(_HPGL_UBYTE_ARRAY(data=m.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh))
+    data = _HPGL_CONT_MASKED_ARRAY(data=out_prop.data.ctypes.data_as(C.POINTER(C.c_float)),
+                                   mask=out_prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
+    mean_arg = C.byref(C.c_double(mean)) if mean is not None else None
+    _hpgl_so.hpgl_sgs_simulation(C.byref(data), C.byref(sgsp), cdf, mean_arg, sim)
+    _check_native("sgs_simulation")
+    return out_prop

@@ -134,6 +134,20 @@ class ParameterValidator:
                 "Performance may be degraded.", "max_neighbors")
 
     @staticmethod
+    def validate_min_neighbors(min_neighbors: int, max_neighbors: int) -> None:
+        """validation.py:373-388 of the reference."""
+        if min_neighbors > max_neighbors:
+            raise CriticalValidationError(f"Min neighbors {min_neighbors} exceeds max neighbors {max_neighbors}",
+                                          "min_neighbors")
+
+    @staticmethod
+    def validate_seed(seed: int) -> None:
+        """The simulation path needs a seed >= 0 (the reference's generator
+        asserts on a negative start)."""
+        if int(seed) < 0:
+            raise CriticalValidationError(f"Seed {seed} is negative", "seed")
+
+    @staticmethod
     def validate_covariance_parameters(sill: float, nugget: float, ranges: Optional[Tuple] = None,
                                        angles: Optional[Tuple] = None) -> None:
         if _bad(sill):

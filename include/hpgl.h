@@ -93,6 +93,34 @@ struct hpgl_ik_params_t {
     double m_marginal_prob;
 };
 
+/* src/geo_bsd/hpgl/api.h:45-49 */
+struct hpgl_ubyte_array_t {
+    unsigned char* m_data;
+    struct hpgl_shape_t m_shape;
+};
+
+/* src/geo_bsd/hpgl/api.h:84-96; m_kriging_kind 0 = ordinary, 1 = simple */
+struct hpgl_sgs_params_t {
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+    int m_radiuses[3];
+    int m_max_neighbours;
+    int m_kriging_kind;
+    long m_seed;
+    int m_min_neighbours;
+};
+
+/* src/geo_bsd/hpgl/api.h:148-153: ascending values and their cumulative
+ * probabilities */
+struct hpgl_non_parametric_cdf_t {
+    float* m_values;
+    float* m_probs;
+    long long m_size;
+};
+
 /* Replaces api.cpp:38-41.  Message of the last failed call ("" if none). */
 char* hpgl_get_last_exception_message(void);
 
@@ -128,6 +156,17 @@ int hpgl_simple_kriging_weights(float* center_coords, float* neighbours_x, float
  * (1 when the kriged P(data == 1) exceeds 0.5) and mask 1. */
 void hpgl_median_ik(struct hpgl_ind_masked_array_t* in_data, struct hpgl_median_ik_params_t* params,
                     struct hpgl_ind_masked_array_t* out_data);
+
+/* Replaces api.cpp:485-516 (geo_bsd.sgs_simulation): sequential Gaussian
+ * simulation in place on `data` along the reference's random path with its
+ * draws (seed), simple kriging with mean 0 or ordinary kriging over the
+ * covariance of the radius box, an optional normal-score transform (cdf, may
+ * be null) and an optional mask of the cells to simulate (== 1; may be null).
+ * `mean` is accepted and unused, as in the reference.  Errors: see
+ * hpgl_get_last_exception_message(). */
+void hpgl_sgs_simulation(struct hpgl_cont_masked_array_t* data, struct hpgl_sgs_params_t* params,
+                         struct hpgl_non_parametric_cdf_t* cdf, double* mean,
+                         struct hpgl_ubyte_array_t* simulation_mask);
 
 /* Replaces api.cpp:412-455 (geo.lvm_kriging, lvm_kriging.cpp:16-39): simple
  * kriging where every cell has its own mean, mean_data (N floats, the

@@ -26,6 +26,13 @@
 //   variogram            src/geo_bsd/_cvariogram/variograms.cpp:27-506,
 //                        ellipsoid.cpp:37-72
 //   order relations      src/geo_bsd/hpgl/indicator_kriging.h:34-59
+//   sequential Gaussian  src/geo_bsd/hpgl/sequential_simulation.h:51-119,
+//   simulation           sequential_gaussian_simulation.cpp:18-78,
+//                        path_random_generator.cpp:20-94,
+//                        bs_random_generator.cpp:8-24, sample.h:14-25,
+//                        gaussian_distribution.h:8-93,
+//                        non_parametric_cdf.h (non_parametric_cdf_2_t),
+//                        cdf_utils.h:14-25, kriging_interpolation.h:269-313
 //
 // The reference delegates three pieces of arithmetic to third-party code; the
 // oracle calls or restates them as follows:
@@ -39,6 +46,7 @@
 //     agrees with any LAPACK to rounding (checked against numpy in tests).
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -401,6 +409,101 @@ CellResult weights(const Setup& s, int node, const std::vector<int>& nb) {
     res.var = v < 0 ? 0 : v;
     res.status = ST_KRIGED;
     return res;
+}
+
+// ------------------------------------------------------------------ SGS
+// path_random_generator.cpp:20-60 and :89-94: a full-period linear
+// congruential walk over [0, size).  The prime table the reference compiles
+// in (primes.txt, not shipped) is the ascending list of primes; the part the
+// generator reads (every prime up to size, and the first one beyond) is
+// regenerated by a sieve.
+struct PathLcg {
+    long long A = 1, C = 0, M = 1, R = 0;
+};
+
+bool path_init(PathLcg& st, long long size, int seed) {
+    if (size < 1 || size > 10000000) return false;
+    const long long lim = 2 * size + 16;  // a prime lies in (size, 2 size]
+    std::vector<char> comp((size_t)lim + 1, 0);
+    std::vector<long long> pr;
+    for (long long i = 2; i <= lim; ++i) {
+        if (comp[(size_t)i]) continue;
+        pr.push_back(i);
+        for (long long j = i * i; j <= lim; j += i) comp[(size_t)j] = 1;
+    }
+    size_t idx = 0;
+    st.A = 1;
+    st.M = size;
+    while (idx < pr.size()) {
+        if (pr[idx] > size) break;
+        if (size % pr[idx] == 0) st.A *= pr[idx];
+        ++idx;
+    }
+    if (st.M % 4 == 0) st.A *= 2;
+    st.A += 1;
+    idx = 0;
+    while (idx + 1 < pr.size() && pr[idx + 1] <= size / 3) ++idx;
+    for (long long k = (long long)idx; k >= 0; k--) {
+        st.C = pr[(size_t)k];
+        if (floor(double(size) / double(st.C)) != (double(size) / double(st.C))) break;
+    }
+    st.R = seed % size;
+    return true;
+}
+
+long long path_next(PathLcg& st) {
+    const long long r = st.R;
+    st.R = (st.A * st.R + st.C) % st.M;
+    return r;
+}
+
+// gaussian_distribution.h:8-93: the CDF (Abramowitz-Stegun 26.2.17) and the
+// quantile (rational approximation); inverse() returns the mean outside (0, 1).
+double gauss_prob(double x, double mean, double var) {
+    const double p = 0.2316419;
+    const double b[] = {0.319381530, -0.356563782, 1.781477937, -1.821255978, 1.330274429};
+    double sx = (x - mean) / sqrt(var);
+    const bool neg = sx < 0;
+    if (neg) sx = -sx;
+    const double t = 1 / (1 + p * sx);
+    const double zx = 1 / sqrt(2 * 3.14159265358) * exp(-sx * sx / 2);
+    double result = 1 - zx * ((((b[4] * t + b[3]) * t + b[2]) * t + b[1]) * t + b[0]) * t;
+    if (neg) result = 1 - result;
+    return result;
+}
+
+double gauss_inverse(double pr, double mean, double var) {
+    if (pr <= 0.0 || pr >= 1.0) return mean;
+    const double ps[] = {-0.322232431088, -1.0, -0.342242088547, -0.0204231210245, -0.453642210148e-4};
+    const double qs[] = {0.0993484626060, 0.588581570495, 0.531103462366, 0.103537752850, 0.38560700634e-2};
+    double p2 = (pr > 0.5) ? (1 - pr) : pr;
+    if (p2 < 1e-150) p2 = 1e-150;
+    const double y = sqrt(log(1 / (p2 * p2)));
+    double xp = y + ((((ps[4] * y + ps[3]) * y + ps[2]) * y + ps[1]) * y + ps[0]) /
+                        ((((qs[4] * y + qs[3]) * y + qs[2]) * y + qs[1]) * y + qs[0]);
+    if (pr < 0.5) xp = -xp;
+    return xp * sqrt(var) + mean;
+}
+
+// non_parametric_cdf.h, non_parametric_cdf_2_t: piecewise-linear CDF over
+// ascending values (float arithmetic), prob() and inverse().
+float np_prob(const float* vals, const float* probs, long long size, float value) {
+    const float* it = std::lower_bound(vals, vals + size, value);
+    if (it == vals + size) return probs[size - 1];
+    if (it == vals) return probs[0];
+    const long long i2 = it - vals, i1 = i2 - 1;
+    const float x1 = vals[i1], x2 = vals[i2], y1 = probs[i1], y2 = probs[i2];
+    return y1 + (y2 - y1) / (x2 - x1) * (value - x1);
+}
+
+float np_inverse(const float* vals, const float* probs, long long size, float prob) {
+    if (size <= 0) return 0;
+    const float* it = std::lower_bound(probs, probs + size, prob);
+    if (it == probs) return vals[0];
+    if (it == probs + size) return vals[size - 1];
+    const long long i2 = it - probs, i1 = i2 - 1;
+    const float x1 = vals[i1], x2 = vals[i2], y1 = probs[i1], y2 = probs[i2];
+    return x1 + (x2 - x1) / (y2 - y1) * (prob - y1);
 }
 
 }  // namespace
@@ -946,6 +1049,102 @@ void orc_std_sort_perm(const double* keys, int n, int* perm) {
     for (int i = 0; i < n; ++i) e[i] = Entry{i, keys[i]};
     std::sort(e.begin(), e.end(), [](const Entry& a, const Entry& b) { return a.cov > b.cov; });
     for (int i = 0; i < n; ++i) perm[i] = e[i].index;
+}
+
+
+// One uniform draw as the reference takes it (bs_random_generator.cpp:19-23):
+// std::uniform_real_distribution<double>(0, 1) over std::mt19937 seeded with
+// (uint32) seed.  Writes the first `n` draws.
+void orc_mt_uniform(long seed, int n, double* out) {
+    std::mt19937 gen;
+    gen.seed(static_cast<std::mt19937::result_type>(seed));
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    for (int i = 0; i < n; ++i) out[i] = dist(gen);
+}
+
+// The first `n` nodes of the reference's random path over `size` nodes.
+int orc_sgs_path(long long size, int seed, int n, long long* out) {
+    PathLcg st;
+    if (!path_init(st, size, seed)) return -1;
+    for (int i = 0; i < n; ++i) out[i] = path_next(st);
+    return 0;
+}
+
+void orc_gauss(const double* x, int n, double mean, double var, int inverse, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = inverse ? gauss_inverse(x[i], mean, var) : gauss_prob(x[i], mean, var);
+}
+
+// Sequential Gaussian simulation (sequential_simulation.h:51-119 with
+// sequential_gaussian_simulation.cpp:18-78).  kind 0 = ordinary, 1 = simple
+// (mean 0); covariances from the precalculated table of the radius box (0
+// outside) for both kinds; neighbours by the template walk only.  values and
+// mask (N each) are updated in place; simmask (may be null) selects the cells
+// to simulate (== 1); cdf_size > 0 applies the normal-score transform before
+// and the back transform after.  out_order (may be null, N ints) receives the
+// simulated nodes in path order, -1 padded.  Returns the number simulated, or
+// -1 on invalid parameters.
+long long orc_sgs(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax, int nx, int ny, int nz,
+                  float* values, unsigned char* mask, const unsigned char* simmask, long seed,
+                  const float* cdf_vals, const float* cdf_probs, long long cdf_size, int* out_order) {
+    if (kmax < 1 || (kind != OK && kind != SK)) return -1;
+    Setup s;
+    if (!setup(s, SK, (const CovParams*)cp, radiuses, kmax, nx, ny, nz, mask)) return -1;
+    s.kind = (Kind)kind;  // algebra of the kind, covariances clipped to the box
+    const long long N = s.g.n();
+    if (cdf_size > 0)
+        for (long long i = 0; i < N; ++i)
+            if (mask[i]) {
+                const double P = np_prob(cdf_vals, cdf_probs, cdf_size, values[i]);
+                values[i] = (float)gauss_inverse(P, 0, 1);
+            }
+    PathLcg path;
+    if (!path_init(path, N, (int)seed)) return -1;
+    std::mt19937 gen;
+    gen.seed(static_cast<std::mt19937::result_type>(seed));
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    long long nsim = 0;
+    if (out_order)
+        for (long long i = 0; i < N; ++i) out_order[i] = -1;
+    std::vector<int> nb;
+    for (long long counter = 0; counter < N; ++counter) {
+        const int node = (int)path_next(path);
+        if (mask[node]) continue;
+        if (simmask && simmask[node] != 1) continue;
+        int cx, cy, cz;
+        s.g.loc(node, cx, cy, cz);
+        nb.clear();
+        for (size_t t = 0; t < s.tmpl.size() && (int)nb.size() < s.kmax; ++t) {
+            const Off& o = s.tmpl[t];
+            const int idx = s.g.index(cx + o.x, cy + o.y, cz + o.z);
+            if (s.informed(idx)) nb.push_back(idx);
+        }
+        CellResult cr;
+        if (nb.empty()) {
+            cr.status = ST_NO_NEIGHBOURS;
+        } else {
+            cr = weights(s, node, nb);
+        }
+        const double u = dist(gen);
+        double value;
+        if (cr.status == ST_KRIGED) {
+            float est = 0.0f;  // combine with mean 0 (single_mean_t(0) / no_mean_t)
+            for (size_t i = 0; i < nb.size(); ++i) est += (float)(cr.w[i] * (values[nb[i]] - 0.0f));
+            value = gauss_inverse(u, (double)est, cr.var);
+        } else {
+            value = gauss_inverse(u, 0, 1);
+        }
+        values[node] = (float)value;
+        mask[node] = 1;
+        if (out_order) out_order[nsim] = node;
+        ++nsim;
+    }
+    if (cdf_size > 0)
+        for (long long i = 0; i < N; ++i)
+            if (mask[i]) {
+                const double P = gauss_prob(values[i], 0, 1);
+                values[i] = np_inverse(cdf_vals, cdf_probs, cdf_size, (float)P);
+            }
+    return nsim;
 }
 
 }  // extern "C"

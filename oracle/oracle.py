@@ -482,3 +482,57 @@ def ref_vario_points(t, xs, ys, zs, vs, lag_count=None):
     tp = _ref_template(t)
     ref_lib().calc_variograms_from_point_set(C.byref(tp), C.byref(ps), _p(out, C.c_float), C.c_int(n))
     return out
+
+
+# ------------------------------------------------------------------ SGS
+def mt_uniform(seed, n):
+    """First ``n`` draws of std::uniform_real_distribution<double>(0, 1) over
+    std::mt19937 seeded with (uint32) seed, as bs_random_generator.cpp takes them."""
+    out = np.zeros(n, np.float64)
+    lib().orc_mt_uniform(C.c_long(seed), C.c_int(n), _p(out, C.c_double))
+    return out
+
+
+def sgs_path(size, seed, n=None):
+    """First ``n`` nodes (default all) of the reference's random path."""
+    n = size if n is None else n
+    out = np.zeros(n, np.int64)
+    rc = lib().orc_sgs_path(C.c_longlong(size), C.c_int(seed), C.c_int(n), _p(out, C.c_longlong))
+    if rc != 0:
+        raise ValueError("path size out of range")
+    return out
+
+
+def gauss(x, mean=0.0, var=1.0, inverse=False):
+    """gaussian_cdf_t: the CDF approximation, or its quantile when ``inverse``."""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    out = np.zeros_like(x)
+    lib().orc_gauss(_p(x, C.c_double), C.c_int(x.size), C.c_double(mean), C.c_double(var),
+                    C.c_int(1 if inverse else 0), _p(out, C.c_double))
+    return out
+
+
+def sgs(kind, cov, radiuses, max_nb, shape, values, mask, seed, simmask=None, cdf=None):
+    """Sequential Gaussian simulation along the reference's path.  Returns
+    (values, mask, order): the realization, the final informed flags and the
+    simulated nodes in path order.  ``cdf`` = (values, probs) or None."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    v = np.array(values, np.float32).reshape(-1)
+    m = np.array(mask, np.uint8).reshape(-1)
+    sm = None if simmask is None else np.ascontiguousarray(simmask, np.uint8).reshape(-1)
+    cv = cp = None
+    csize = 0
+    if cdf is not None:
+        cv = np.ascontiguousarray(cdf[0], np.float32)
+        cp = np.ascontiguousarray(cdf[1], np.float32)
+        csize = cv.size
+    order = np.zeros(n, np.int32)
+    f = lib().orc_sgs
+    f.restype = C.c_longlong
+    k = f(C.c_int(kind), C.byref(cov), (C.c_int * 3)(*radiuses), C.c_int(max_nb), C.c_int(nx), C.c_int(ny),
+          C.c_int(nz), _p(v, C.c_float), _p(m, C.c_ubyte), _p(sm, C.c_ubyte), C.c_long(seed),
+          _p(cv, C.c_float), _p(cp, C.c_float), C.c_longlong(csize), _p(order, C.c_int))
+    if k < 0:
+        raise ValueError("oracle rejected the parameters")
+    return v, m, order[:k]
