@@ -18,8 +18,10 @@ from 40 wells, 256x256x64, cluster path), `ik` (indicator kriging over 4
 categories, 200x200x100), `lvm` (kriging with a locally varying mean,
 200x200x40), `io` (INC property file of 4M values: GPU parse and print) and
 `variogram` (experimental semivariogram of the 200x200x100 median-IK hard
-data, 40,000 cells, and of 20,000 scattered points).
-Each is timed beside the reference's own CPU build where it is present.
+data, 40,000 cells, and of 20,000 scattered points) and `sgs` (sequential
+Gaussian simulation of the 100x100x50 grid, 475,000 cells).
+Each is timed beside the reference's own CPU build where it is present (for
+`sgs`, beside the CPU oracle: the reference's simulation cannot be built).
 """
 import argparse
 import ctypes as C
@@ -323,6 +325,50 @@ def side_workload(name, args):
                       sub * sub, "pairs/s", f"reference calc_variograms_from_point_set on {sub} points, 1 thread")
             rp["cores"] = 1
             line["cpu_baseline_points"] = rp
+    elif name == "sgs":
+        # Sequential Gaussian simulation of the 100x100x50 grid (5 % hard data
+        # as normal scores), simple kriging, k = 24, radiuses (20, 20, 4).
+        wl = S.ok_c2()
+        inf = wl.mask == 1
+        z = ((wl.values - wl.values[inf].mean()) / wl.values[inf].std()).astype(np.float32)
+        v0 = np.where(inf, z, np.float32(0)).astype(np.float32)
+        model = CovarianceModel(**wl.cov)
+        seed = 20240611
+        dv0, dm0 = torch.from_numpy(v0).to(dev), torch.from_numpy(wl.mask).to(dev)
+        dv, dm = dv0.clone(), dm0.clone()
+
+        def run():
+            dv.copy_(dv0)
+            dm.copy_(dm0)
+            D.sgs_simulation(dv, dm, wl.shape, wl.radiuses, wl.k, model, seed, "sk")
+
+        t = _timed(run, args.steps, args.warmup, sync)
+        nsim = int((~inf).sum())
+        line.update(metric="simulated cells/s of sequential Gaussian simulation on 1 MI355X", value=nsim / t,
+                    unit="simulated cells/s", dtype="f64", ms_per_step=t * 1e3,
+                    config={"workload": "SGS of a 100x100x50 grid with 5% hard data (normal scores), simple "
+                                        "kriging, exponential (20,20,4), nugget 0.05, radiuses (20,20,4), "
+                                        "max_neighbours 24, the reference's path and draws for seed 20240611",
+                            "simulated_cells": nsim})
+        # agreement with the oracle's one-by-one walk, and its speed, on a
+        # 100x100x10 slab (the reference's own simulation cannot be built:
+        # its path generator needs a prime table it does not ship)
+        if not args.no_cpu_baseline:
+            sub = (100, 100, 10)
+            ns = int(np.prod(sub))
+            sv, smask = v0[:ns].copy(), wl.mask[:ns].copy()
+            cov = O.make_cov(wl.cov["type"], wl.cov["ranges"], wl.cov["angles"], wl.cov["sill"], wl.cov["nugget"])
+            t0 = time.perf_counter()
+            ov, om, order = O.sgs(1, cov, wl.radiuses, wl.k, sub, sv, smask, seed)
+            tc = time.perf_counter() - t0
+            gv, gm = torch.from_numpy(sv).to(dev), torch.from_numpy(smask).to(dev)
+            D.sgs_simulation(gv, gm, sub, wl.radiuses, wl.k, model, seed, "sk")
+            sync()
+            line["max_abs_diff_vs_oracle_slab"] = float(np.abs(gv.cpu().numpy() - ov).max())
+            line["cpu_baseline"] = {"value": order.size / tc, "unit": "simulated cells/s", "cores": 1,
+                                    "kind": "port",
+                                    "sample": f"the oracle's sequential walk over a 100x100x10 slab ({order.size} "
+                                              "cells), 1 thread", "seconds": tc}
     else:
         raise SystemExit(f"unknown workload {name}")
     return line
@@ -342,7 +388,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
-    ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io", "variogram"],
+    ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io", "variogram", "sgs"],
                     help="ok (default): the headline OK 100x100x50 workload; others: see the module docstring")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "

@@ -11,12 +11,13 @@
 
 namespace hpglx {
 
-// Simulates the schedule's cells level by level on `st`: per level a template
-// search over the current informed mask, the FP64 SK (mean 0) or OK solve
-// with variance over the plan's clipped table, and the draw
-// z * sqrt(variance) + estimate written back as data.  values and mask
-// (device, N) are updated in place.  The plan must be built with the
-// covariance clipped to the radius box (the simple-kriging plan).
+// Simulates the schedule's cells level by level on `st`, one launch per
+// level and one wave per cell: a template search over the current informed
+// mask, the FP64 SK (mean 0) or OK solve with variance over the plan's
+// clipped table, and the draw z * sqrt(variance) + estimate written back as
+// data.  values and mask (device, N) are updated in place.  The plan must be
+// built with the covariance clipped to the radius box (the simple-kriging
+// plan).
 int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_values, unsigned char* d_mask,
             const SgsSchedule& s, hipStream_t st, std::string& err);
 

@@ -38,55 +38,6 @@ __global__ void k_sgs_tdelta(GridDims g, int T, const int* tdx, const int* tdy, 
     tpack[t] = pack_off(tdx[t], tdy[t], tdz[t]);
 }
 
-// One wave per listed cell: the template is probed 64 entries at a time and
-// the hits, ranked by a ballot, are the first kmax informed cells in template
-// order (sugarbox_neighbour_lookup.h:37-62).
-__global__ __launch_bounds__(256) void k_sgs_search(GridDims g, int kmax, int T, const int* tdelta, const int* tpack,
-                                                    const unsigned char* pmask, const int* cells, int ncells,
-                                                    int* count, int* nboff) {
-    const int lane = threadIdx.x & 63;
-    const int w = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (w >= ncells) return;  // whole waves leave together
-    const int cell = cells[w];
-    const int x = cell % g.nx, y = (cell / g.nx) % g.ny, z = cell / (g.nx * g.ny);
-    const long long pbase = ((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
-    int cnt = 0;
-    for (int t0 = 0; t0 < T && cnt < kmax; t0 += 64) {
-        const int t = t0 + lane;
-        const bool hit = t < T && pmask[pbase + tdelta[t]] != 0;
-        const unsigned long long b = __ballot(hit);
-        const int rank = cnt + __popcll(b & ((1ull << lane) - 1ull));
-        if (hit && rank < kmax) nboff[(long long)cell * kmax + rank] = tpack[t];
-        cnt += __popcll(b);
-    }
-    if (lane == 0) count[cell] = cnt < kmax ? cnt : kmax;
-}
-
-// The draw of each listed cell (sample.h:14-25): a kriged cell takes
-// gaussian_cdf_t(estimate, variance).inverse(u) = z sqrt(variance) + estimate
-// (products and sums rounded singly, as the reference's host code), any other
-// the standard deviate; a draw outside (0, 1) (z = NaN) gives the mean itself.
-// The value becomes data: mask and padded mask are set.
-__global__ void k_sgs_sample(GridDims g, const int* cells, int ncells, const double* z, const unsigned char* status,
-                             const float* est, const double* var, float* values, unsigned char* mask,
-                             unsigned char* pmask) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncells) return;
-    const int cell = cells[i];
-    const double d = z[i];
-    float v;
-    if (status[cell] == ST_KRIGED) {
-        const double m = (double)est[cell];
-        v = isnan(d) ? (float)m : (float)__dadd_rn(__dmul_rn(d, __dsqrt_rn(var[cell])), m);
-    } else {
-        v = isnan(d) ? 0.0f : (float)d;
-    }
-    values[cell] = v;
-    mask[cell] = 1;
-    const int x = cell % g.nx, y = (cell / g.nx) % g.ny, zc = cell / (g.nx * g.ny);
-    pmask[((long long)(zc + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx)] = 1;
-}
-
 unsigned blocks_of(long long work, int per) {
     return (unsigned)std::max(1LL, (work + per - 1) / per);
 }
@@ -123,7 +74,6 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     const int L = (int)s.level_off.size() - 1;
     if (m == 0) return 0;
     auto* pmask = (unsigned char*)ws.get("sgs_pmask", (size_t)P, err);
-    auto* count = (int*)ws.get("sgs_count", (size_t)n * 4, err);
     auto* nboff = (int*)ws.get("sgs_nboff", (size_t)n * K * 4, err);
     auto* est = (float*)ws.get("sgs_est", (size_t)n * 4, err);
     auto* var = (double*)ws.get("sgs_var", (size_t)n * 8, err);
@@ -132,13 +82,11 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     auto* tpack = (int*)ws.get("sgs_tpack", (size_t)std::max(T, 1) * 4, err);
     auto* cells = (int*)ws.get("sgs_cells", (size_t)m * 4, err);
     auto* zdev = (double*)ws.get("sgs_z", (size_t)m * 8, err);
-    auto* lcount = (int*)ws.get("sgs_level_count", (size_t)L * 4, err);
     if (!err.empty()) return -1;
     std::vector<int> lc((size_t)L);
     for (int l = 0; l < L; ++l) lc[(size_t)l] = s.level_off[(size_t)l + 1] - s.level_off[(size_t)l];
     if (hipMemcpyAsync(cells, s.cells.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(zdev, s.z.data(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(lcount, lc.data(), (size_t)L * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+        hipMemcpyAsync(zdev, s.z.data(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
         err = "host to device copy failed";
         return -1;
     }
@@ -156,7 +104,7 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     so.values = d_values;
     so.mask = d_mask;
     so.mean = 0.0;  // SK: single_mean_t(0) (sequential_gaussian_simulation.cpp:37-43)
-    so.count = count;
+    so.count = nullptr;  // the fused level kernel passes each cell's count directly
     so.nboff = nboff;
     so.ctab = plan.ctab;
     so.ctab32 = plan.ctab32;
@@ -172,26 +120,26 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     so.out_val = est;
     so.out_var = var;
     so.out_status = status;
-    const hipEvent_t no_ev[2] = {nullptr, nullptr};
+    SgsLevelArgs la;
+    la.T = T;
+    la.tdelta = tdelta;
+    la.tpack = tpack;
+    la.pmask = pmask;
+    la.values = d_values;
+    la.mask = d_mask;
     for (int l = 0; l < L; ++l) {
-        const int off = s.level_off[(size_t)l], cnt = lc[(size_t)l];
-        if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_sgs_search, dim3(blocks_of((long long)cnt * 64, 256)), dim3(256), 0, st, g, K, T,
-                           (const int*)tdelta, (const int*)tpack, (const unsigned char*)pmask,
-                           (const int*)(cells + off), cnt, count, nboff);
-        so.cell_list = cells + off;
-        so.n_list = lcount + l;
+        const int off = s.level_off[(size_t)l];
+        la.cells = cells + off;
+        la.ncells = lc[(size_t)l];
+        la.z = zdev + off;
         if (K <= 8)
-            dispatch_solve_8(so, kind, false, cnt, 0, no_ev, st);
+            kind == KIND_OK ? sgs_level_8_0(so, la, st) : sgs_level_8_1(so, la, st);
         else if (K <= 16)
-            dispatch_solve_16(so, kind, false, cnt, 0, no_ev, st);
+            kind == KIND_OK ? sgs_level_16_0(so, la, st) : sgs_level_16_1(so, la, st);
         else if (K <= 24)
-            dispatch_solve_24(so, kind, false, cnt, 0, no_ev, st);
+            kind == KIND_OK ? sgs_level_24_0(so, la, st) : sgs_level_24_1(so, la, st);
         else
-            dispatch_solve_32(so, kind, false, cnt, 0, no_ev, st);
-        hipLaunchKernelGGL(k_sgs_sample, dim3(blocks_of(cnt, 256)), dim3(256), 0, st, g, (const int*)(cells + off),
-                           cnt, (const double*)(zdev + off), (const unsigned char*)status, (const float*)est,
-                           (const double*)var, d_values, d_mask, pmask);
+            kind == KIND_OK ? sgs_level_32_0(so, la, st) : sgs_level_32_1(so, la, st);
     }
     if (hipGetLastError() != hipSuccess) {
         err = "sequential simulation kernel launch failed";

@@ -22,10 +22,27 @@ void dispatch_solve_24(const SolveArgs& sa, int kind, bool mixed, long long n, i
 void dispatch_solve_32(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
                        hipStream_t st);
 
+// One level of sequential Gaussian simulation in one launch: a wave per
+// listed cell runs the template search over the padded mask, the FP64 solve
+// with variance (SolveArgs: kind, tables, values, outputs est/var/status) and
+// the draw z * sqrt(variance) + estimate, written back as data.
+struct SgsLevelArgs {
+    int T;                    // template entries
+    const int* tdelta;        // their deltas in the padded mask
+    const int* tpack;         // their packed offsets
+    unsigned char* pmask;     // padded 0/1 mask (halo r, pitch px)
+    const int* cells;         // the level's cells
+    int ncells;
+    const double* z;          // their standard deviates (NaN: the draw gives the mean)
+    float* values;            // property values (N), updated
+    unsigned char* mask;      // informed mask (N), updated
+};
+
 // One per (bucket, kind): solve_n<NMAX>_k<KIND>.hip.
 #define HPGLX_DECL_SOLVE(NMAX, KIND)                                                                      \
     void dispatch_solve_##NMAX##_##KIND(const SolveArgs& sa, bool mixed, long long n, int nelem,          \
-                                        const hipEvent_t* ev, hipStream_t st);
+                                        const hipEvent_t* ev, hipStream_t st);                            \
+    void sgs_level_##NMAX##_##KIND(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
 HPGLX_DECL_SOLVE(8, 0)
 HPGLX_DECL_SOLVE(8, 1)
 HPGLX_DECL_SOLVE(8, 2)

@@ -128,5 +128,70 @@ void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, cons
     }
 }
 
+// One level of sequential Gaussian simulation (see SgsLevelArgs): one wave
+// (block of 64) per cell.  The search ranks the template's hits with a
+// ballot, 64 entries at a time, and keeps the first kmax in template order
+// (sugarbox_neighbour_lookup.h:37-62); the first G lanes then solve the cell
+// as the list solve does; lane 0 draws (sample.h:14-25,
+// gaussian_distribution.h:56-89, products and sums rounded singly).
+template <int NMAX, int G, int KIND, bool OCT>
+__global__ __launch_bounds__(64) void k_sgs_level(SolveArgs a, SgsLevelArgs la) {
+    const int lane = threadIdx.x;
+    const int cell = la.cells[blockIdx.x];
+    const GridDims& g = a.g;
+    const int x = cell % g.nx, y = (cell / g.nx) % g.ny, zc = cell / (g.nx * g.ny);
+    const long long pcell = ((long long)(zc + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
+    int* nboff = const_cast<int*>(a.nboff) + (long long)cell * a.kmax;
+    int cnt = 0;
+    for (int t0 = 0; t0 < la.T && cnt < a.kmax; t0 += 64) {
+        const int t = t0 + lane;
+        const bool hit = t < la.T && la.pmask[pcell + la.tdelta[t]] != 0;
+        const unsigned long long b = __ballot(hit);
+        const int rank = cnt + __popcll(b & ((1ull << lane) - 1ull));
+        if (hit && rank < a.kmax) nboff[rank] = la.tpack[t];
+        cnt += __popcll(b);
+    }
+    if (cnt > a.kmax) cnt = a.kmax;
+    __threadfence_block();
+    __syncthreads();
+    const unsigned ox = (unsigned)(a.wx + 1), oxy = ox * (unsigned)(a.wy + 1);
+    if (lane < G) solve_cell<NMAX, G, false, KIND, OCT>(a, cell, cnt, lane, a.otab_w, ox, oxy);
+    __threadfence_block();
+    __syncthreads();
+    if (lane == 0) {
+        const double d = la.z[blockIdx.x];
+        float v;
+        if (a.out_status[cell] == ST_KRIGED) {
+            const double m = (double)a.out_val[cell];
+            v = isnan(d) ? (float)m : (float)__dadd_rn(__dmul_rn(d, __dsqrt_rn(a.out_var[cell])), m);
+        } else {
+            v = isnan(d) ? 0.0f : (float)d;
+        }
+        la.values[cell] = v;
+        la.mask[cell] = 1;
+        la.pmask[pcell] = 1;
+    }
+}
+
+template <int NMAX, int G64, int KIND>
+void launch_sgs_level(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    if constexpr (KIND == 0 || KIND == 1) {  // ordinary or simple kriging (median IK is not simulated)
+        // a level holds few cells, so each cell's latency counts: 8 lanes per
+        // cell from 16 neighbours up
+        constexpr int GS = NMAX >= 16 ? 8 : G64;
+        if (la.ncells <= 0) return;
+        if (sa.otab_w)
+            hipLaunchKernelGGL((k_sgs_level<NMAX, GS, KIND, true>), dim3((unsigned)la.ncells), dim3(64), 0, st, sa,
+                               la);
+        else
+            hipLaunchKernelGGL((k_sgs_level<NMAX, GS, KIND, false>), dim3((unsigned)la.ncells), dim3(64), 0, st, sa,
+                               la);
+    } else {
+        (void)sa;
+        (void)la;
+        (void)st;
+    }
+}
+
 }  // namespace
 }  // namespace hpglx

@@ -8,4 +8,8 @@ void dispatch_solve_16_1(const SolveArgs& sa, bool mixed, long long n, int nelem
     dispatch_kind<16, 4, 4, 1>(sa, mixed, n, nelem, ev, st);
 }
 
+void sgs_level_16_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    launch_sgs_level<16, 4, 1>(sa, la, st);
+}
+
 }  // namespace hpglx

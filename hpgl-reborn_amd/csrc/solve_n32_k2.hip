@@ -8,4 +8,8 @@ void dispatch_solve_32_2(const SolveArgs& sa, bool mixed, long long n, int nelem
     dispatch_kind<32, 8, 8, 2>(sa, mixed, n, nelem, ev, st);
 }
 
+void sgs_level_32_2(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    launch_sgs_level<32, 8, 2>(sa, la, st);
+}
+
 }  // namespace hpglx
