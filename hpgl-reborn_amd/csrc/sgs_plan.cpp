@@ -131,9 +131,11 @@ bool sgs_schedule(const int dims[3], const int r[3], const unsigned char* inform
 
     // Levels: a cell's level is one more than the highest level among the
     // cells simulated before it inside its +-r box.  The maximum is taken
-    // over blocks of about r/2 cells per axis (each holding the highest
-    // level placed in it so far), which can only over-separate.
-    const int bx = std::max(1, (r[0] + 1) / 2), by = std::max(1, (r[1] + 1) / 2), bz = std::max(1, (r[2] + 1) / 2);
+    // over blocks of about r/2 cells in x and y and r/4 in z (each holding
+    // the highest level placed in it so far), which can only over-separate;
+    // finer blocks give fewer levels for more host time (100x100x50, r =
+    // (20, 20, 4): 27,965 levels in 0.10 s; exact levels 20,235 in 4 s).
+    const int bx = std::max(1, (r[0] + 1) / 2), by = std::max(1, (r[1] + 1) / 2), bz = std::max(1, (r[2] + 1) / 4);
     const long long mx = (nx + bx - 1) / bx, my = (ny + by - 1) / by, mz = (nz + bz - 1) / bz;
     std::vector<int> top((size_t)(mx * my * mz), 0);
     std::vector<int> lv;
