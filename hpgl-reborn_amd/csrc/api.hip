@@ -568,7 +568,7 @@ bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned ch
 // are updated in place; simmask (host, N) may be null.
 int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vector<float>& v,
             std::vector<unsigned char>& m, const unsigned char* simmask, const hpgl_non_parametric_cdf_t* cdf,
-            hipStream_t st, std::string& err) {
+            const float* lvm_host, hipStream_t st, std::string& err) {
     if (p->m_kriging_kind != KIND_OK && p->m_kriging_kind != KIND_SK) {
         err = "sequential simulation takes simple (1) or ordinary (0) kriging";
         return -1;
@@ -588,6 +588,18 @@ int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vect
     const long long n = (long long)v.size();
     const bool transform = cdf && cdf->m_size > 0 && cdf->m_values && cdf->m_probs;
     if (transform) sgs_to_normal_scores(v.data(), m.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
+    std::vector<float> lvm;
+    float* d_lvm = nullptr;
+    if (lvm_host) {
+        if (p->m_kriging_kind != KIND_SK) {
+            err = "a locally varying mean applies to simple kriging only";
+            return -1;
+        }
+        lvm.assign(lvm_host, lvm_host + n);
+        if (transform) sgs_means_to_normal_scores(lvm.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
+        d_lvm = (float*)L.ws.get("sgs_io_lvm", (size_t)n * 4, err);
+        if (!err.empty()) return -1;
+    }
     SgsSchedule s;
     if (!sgs_schedule(dims, p->m_radiuses, m.data(), simmask, p->m_seed, s, err)) return -1;
     auto* dv = (float*)L.ws.get("sgs_io_val", (size_t)n * 4, err);
@@ -596,11 +608,12 @@ int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vect
     if (!ensure_events(L, err)) return -1;
     if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess ||
         hipMemcpyAsync(dv, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dm, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        hipMemcpyAsync(dm, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (d_lvm && hipMemcpyAsync(d_lvm, lvm.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess)) {
         err = "host to device copy failed";
         return -1;
     }
-    if (sgs_run(*plan, L.ws, p->m_kriging_kind, dims, dv, dm, s, st, err) != 0) return -1;
+    if (sgs_run(*plan, L.ws, p->m_kriging_kind, dims, dv, dm, d_lvm, s, st, err) != 0) return -1;
     if (hipMemcpyAsync(v.data(), dv, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(m.data(), dm, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipEventRecord(L.done, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
@@ -782,8 +795,47 @@ void hpgl_sgs_simulation(hpgl_cont_masked_array_t* data, hpgl_sgs_params_t* para
         std::vector<float> v(data->m_data, data->m_data + n);
         std::vector<unsigned char> m(data->m_mask, data->m_mask + n);
         const int dims[3] = {data->m_shape.m_data[0], data->m_shape.m_data[1], data->m_shape.m_data[2]};
-        if (run_sgs(L, params, dims, v, m, simulation_mask ? simulation_mask->m_data : nullptr, cdf, L.io_stream,
-                    e) != 0)
+        if (run_sgs(L, params, dims, v, m, simulation_mask ? simulation_mask->m_data : nullptr, cdf, nullptr,
+                    L.io_stream, e) != 0)
+            return -1;
+        std::memcpy(data->m_data, v.data(), (size_t)n * 4);
+        std::memcpy(data->m_mask, m.data(), (size_t)n);
+        return 0;
+    });
+}
+
+// api.cpp:519-548: simple kriging with a locally varying mean (one per cell,
+// taken through the normal-score transform with the data).
+void hpgl_sgs_lvm_simulation(hpgl_cont_masked_array_t* data, hpgl_sgs_params_t* params,
+                             hpgl_non_parametric_cdf_t* cdf, hpgl_float_array_t* means,
+                             hpgl_ubyte_array_t* simulation_mask) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    const int n = shape_volume(&data->m_shape, err);
+    if (n < 0) {
+        L.last_error = err;
+        return;
+    }
+    if (!means || !means->m_data || shape_volume(&means->m_shape, err) != n) {
+        L.last_error = err.empty() ? "the mean array and the property sizes differ" : err;
+        return;
+    }
+    if (simulation_mask && shape_volume(&simulation_mask->m_shape, err) != n) {
+        L.last_error = err.empty() ? "simulation mask and property sizes differ" : err;
+        return;
+    }
+    timed_call(L, "Sequential Gaussian Simulation with Local Varying Mean", n, [&](std::string& e) -> int {
+        if (!L.io_stream && hipStreamCreateWithFlags(&L.io_stream, hipStreamNonBlocking) != hipSuccess) {
+            e = "hipStreamCreate failed";
+            return -1;
+        }
+        std::vector<float> v(data->m_data, data->m_data + n);
+        std::vector<unsigned char> m(data->m_mask, data->m_mask + n);
+        const int dims[3] = {data->m_shape.m_data[0], data->m_shape.m_data[1], data->m_shape.m_data[2]};
+        if (run_sgs(L, params, dims, v, m, simulation_mask ? simulation_mask->m_data : nullptr, cdf, means->m_data,
+                    L.io_stream, e) != 0)
             return -1;
         std::memcpy(data->m_data, v.data(), (size_t)n * 4);
         std::memcpy(data->m_mask, m.data(), (size_t)n);
@@ -1197,7 +1249,8 @@ int hpglx_indicator_kriging(const hpgl_ik_params_t* params, int ncat, const int 
 }
 
 int hpglx_sgs_simulation(float* values, unsigned char* mask, const int dims[3], const hpgl_sgs_params_t* params,
-                         const hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask, void* stream) {
+                         const hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask,
+                         const float* mean_field, void* stream) {
     Library& L = lib();
     std::lock_guard<std::mutex> g(L.mu);
     L.last_error.clear();
@@ -1209,13 +1262,17 @@ int hpglx_sgs_simulation(float* values, unsigned char* mask, const int dims[3], 
     const hipStream_t st = (hipStream_t)stream;
     std::vector<float> v((size_t)n);
     std::vector<unsigned char> m((size_t)n), sm(simulation_mask ? (size_t)n : 0);
-    if (hipMemcpyAsync(v.data(), values, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+    std::vector<float> lvm(mean_field ? (size_t)n : 0);
+    if ((mean_field && hipMemcpyAsync(lvm.data(), mean_field, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipMemcpyAsync(v.data(), values, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(m.data(), mask, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
         (simulation_mask &&
          hipMemcpyAsync(sm.data(), simulation_mask, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
         return fail(L, "hpglx_sgs_simulation: device to host copy failed");
-    if (run_sgs(L, params, dims, v, m, simulation_mask ? sm.data() : nullptr, cdf, st, err) != 0) return fail(L, err);
+    if (run_sgs(L, params, dims, v, m, simulation_mask ? sm.data() : nullptr, cdf, mean_field ? lvm.data() : nullptr, st,
+                err) != 0)
+        return fail(L, err);
     if (hipMemcpyAsync(values, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(mask, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)

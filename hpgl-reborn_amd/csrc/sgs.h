@@ -18,7 +18,9 @@ namespace hpglx {
 // data.  values and mask (device, N) are updated in place.  The plan must be
 // built with the covariance clipped to the radius box (the simple-kriging
 // plan).
+// d_lvm (device, N; simple kriging only, may be null): the cells' own means
+// in the combine (sequential_gaussian_simulation.cpp:80-131).
 int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_values, unsigned char* d_mask,
-            const SgsSchedule& s, hipStream_t st, std::string& err);
+            const float* d_lvm, const SgsSchedule& s, hipStream_t st, std::string& err);
 
 }  // namespace hpglx

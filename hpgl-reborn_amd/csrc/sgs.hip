@@ -45,10 +45,14 @@ unsigned blocks_of(long long work, int per) {
 }  // namespace
 
 int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_values, unsigned char* d_mask,
-            const SgsSchedule& s, hipStream_t st, std::string& err) {
+            const float* d_lvm, const SgsSchedule& s, hipStream_t st, std::string& err) {
     const PlanHost& h = plan.h;
     if (kind != KIND_OK && kind != KIND_SK) {
         err = "sequential simulation takes simple or ordinary kriging";
+        return -1;
+    }
+    if (d_lvm && kind != KIND_SK) {
+        err = "a locally varying mean applies to simple kriging only";
         return -1;
     }
     if (h.kmax > 32) {
@@ -104,6 +108,7 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     so.values = d_values;
     so.mask = d_mask;
     so.mean = 0.0;  // SK: single_mean_t(0) (sequential_gaussian_simulation.cpp:37-43)
+    so.lvm_mean = d_lvm;
     so.count = nullptr;  // the fused level kernel passes each cell's count directly
     so.nboff = nboff;
     so.ctab = plan.ctab;

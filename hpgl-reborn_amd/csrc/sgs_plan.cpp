@@ -200,6 +200,14 @@ void sgs_to_normal_scores(float* v, const unsigned char* informed, long long n, 
         }
 }
 
+void sgs_means_to_normal_scores(float* m, long long n, const float* cdf_values, const float* cdf_probs,
+                                long long cdf_size) {
+    for (long long i = 0; i < n; ++i) {
+        const double x = standard_deviate(table_prob(cdf_values, cdf_probs, cdf_size, m[i]));
+        m[i] = (float)(std::isnan(x) ? 0.0 : x);
+    }
+}
+
 void sgs_from_normal_scores(float* v, const unsigned char* informed, long long n, const float* cdf_values,
                             const float* cdf_probs, long long cdf_size) {
     for (long long i = 0; i < n; ++i)

@@ -38,5 +38,9 @@ void sgs_to_normal_scores(float* v, const unsigned char* informed, long long n, 
                           const float* cdf_probs, long long cdf_size);
 void sgs_from_normal_scores(float* v, const unsigned char* informed, long long n, const float* cdf_values,
                             const float* cdf_probs, long long cdf_size);
+// The locally varying mean of every cell through the forward transform
+// (cdf_utils.h:44-50).
+void sgs_means_to_normal_scores(float* m, long long n, const float* cdf_values, const float* cdf_probs,
+                                long long cdf_size);
 
 }  // namespace hpglx

@@ -214,14 +214,16 @@ def indicator_kriging(data, mask, grid, categories, marginal_probs, precision="f
 
 
 def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed, kriging_type="sk", cdf=None,
-                   simulation_mask=None, stream=None):
+                   simulation_mask=None, mean_field=None, stream=None):
     """Sequential Gaussian simulation in place on device tensors (work of
     hpgl_sgs_simulation, api.cpp:485-516): float32 values and uint8 mask of
     N cells are overwritten with the realization; the model is used as
     given (geo_bsd.sgs_simulation scales it to unit sill first).  cdf =
     (values, probs) host arrays of the normal-score transform, or None;
-    simulation_mask: uint8 tensor (1 = simulate) or None.  Returns the
-    number of cells informed afterwards."""
+    simulation_mask: uint8 tensor (1 = simulate) or None; mean_field:
+    float32 tensor of one mean per cell (simple kriging with a locally
+    varying mean) or None.  Returns the number of cells informed
+    afterwards."""
     torch = _torch()
     import numpy as np
     from .hpgl_wrap import _HPGL_SGS_PARAMS, hpgl_non_parametric_cdf_t
@@ -231,6 +233,8 @@ def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed
     _check_dev(mask, torch.uint8, n, "mask")
     if simulation_mask is not None:
         _check_dev(simulation_mask, torch.uint8, n, "simulation_mask")
+    if mean_field is not None:
+        _check_dev(mean_field, torch.float32, n, "mean_field")
     _require(kriging_type in ("sk", "ok"), 'kriging_type must be "sk" or "ok"')
     p = _HPGL_SGS_PARAMS(covariance_type=int(cov_model.type), ranges=tuple(float(r) for r in cov_model.ranges),
                          angles=tuple(float(a) for a in cov_model.angles), sill=float(cov_model.sill),
@@ -246,7 +250,8 @@ def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed
     if stream is None:
         stream = torch.cuda.current_stream(values.device).cuda_stream
     rc = _hpgl_so.hpglx_sgs_simulation(_ptr(values), _ptr(mask), (C.c_int * 3)(nx, ny, nz), C.byref(p), cdf_arg,
-                                       _ptr(simulation_mask) if simulation_mask is not None else None, stream)
+                                       _ptr(simulation_mask) if simulation_mask is not None else None,
+                                       _ptr(mean_field) if mean_field is not None else None, stream)
     if rc != 0:
         raise RuntimeError("sgs_simulation: " + _hpgl_so.hpglx_last_error().decode())
     return int(mask.sum().item())

@@ -60,6 +60,10 @@ class __hpgl_cov_params_t(C.Structure):
 _HPGL_COV_PARAMS = __hpgl_cov_params_t
 
 
+class _HPGL_FLOAT_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("shape", _HPGL_SHAPE)]
+
+
 class _HPGL_UBYTE_ARRAY(C.Structure):
     _fields_ = [("data", C.POINTER(C.c_ubyte)), ("shape", _HPGL_SHAPE)]
 
@@ -194,9 +198,14 @@ _hpgl_so.hpgl_sgs_simulation.restype = None
 _hpgl_so.hpgl_sgs_simulation.argtypes = [C.POINTER(_HPGL_CONT_MASKED_ARRAY), C.POINTER(_HPGL_SGS_PARAMS),
                                          C.POINTER(hpgl_non_parametric_cdf_t), C.POINTER(C.c_double),
                                          C.POINTER(_HPGL_UBYTE_ARRAY)]
+_hpgl_so.hpgl_sgs_lvm_simulation.restype = None
+_hpgl_so.hpgl_sgs_lvm_simulation.argtypes = [C.POINTER(_HPGL_CONT_MASKED_ARRAY), C.POINTER(_HPGL_SGS_PARAMS),
+                                             C.POINTER(hpgl_non_parametric_cdf_t), C.POINTER(_HPGL_FLOAT_ARRAY),
+                                             C.POINTER(_HPGL_UBYTE_ARRAY)]
 _hpgl_so.hpglx_sgs_simulation.restype = C.c_int
 _hpgl_so.hpglx_sgs_simulation.argtypes = [C.c_void_p, C.c_void_p, C.c_int * 3, C.POINTER(_HPGL_SGS_PARAMS),
-                                          C.POINTER(hpgl_non_parametric_cdf_t), C.c_void_p, C.c_void_p]
+                                          C.POINTER(hpgl_non_parametric_cdf_t), C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
 _hpgl_so.hpglx_template.restype = C.c_int
 _hpgl_so.hpglx_template.argtypes = [C.POINTER(__hpgl_cov_params_t), C.c_int * 3, C.c_void_p, C.c_int]
 _hpgl_so.hpglx_set_profiling.restype = None

@@ -6,8 +6,8 @@ import numpy
 
 from .cdf import CdfData
 from .geo import ContProperty, CovarianceModel, _check_native, _clone_prop, _shape, accepts_tuple
-from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_KRIGING_KIND, _HPGL_SGS_PARAMS, _HPGL_UBYTE_ARRAY,
-                        _hpgl_so, hpgl_non_parametric_cdf_t)
+from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_FLOAT_ARRAY, _HPGL_KRIGING_KIND, _HPGL_SGS_PARAMS,
+                        _HPGL_UBYTE_ARRAY, _hpgl_so, hpgl_non_parametric_cdf_t)
 from .validation import GridValidator, ParameterValidator
 
 
@@ -33,8 +33,11 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
     cdf_data: None (no transform) or CdfData of the normal-score transform.
     radiuses, max_neighbours, cov_model: the kriging search and model (the
     model is scaled to unit sill).  seed: path and draws.  kriging_type:
-    "sk" (mean 0 in normal-score space) or "ok".  mean: accepted and unused,
-    as in the reference.  use_harddata: condition on the property's informed
+    "sk" (mean 0 in normal-score space) or "ok".  mean: a scalar is accepted
+    and unused, as in the reference; an array of one mean per cell selects
+    simple kriging with a locally varying mean (hpgl_sgs_lvm_simulation;
+    the means go through the normal-score transform with the data).
+    use_harddata: condition on the property's informed
     cells.  mask: 1 where cells are simulated (None: every cell).
     use_regions / force_* / min_neighbours: accepted for compatibility; the
     reference's one-path algorithm ignores them as well.
@@ -49,8 +52,6 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
     ParameterValidator.validate_min_neighbors(min_neighbours, max_neighbours)
     if kriging_type not in ("sk", "ok"):
         raise ValueError('kriging_type must be "sk" or "ok"')
-    if mean is not None and not numpy.isscalar(mean):
-        raise NotImplementedError("sgs_simulation: a locally varying mean is not supported")
     prop.fix_shape(grid)
     cov_model = normed_cov_model(cov_model)
     out_prop = _clone_prop(prop) if use_harddata else _empty_clone(prop)
@@ -78,7 +79,15 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
         sim = C.byref(_HPGL_UBYTE_ARRAY(data=m.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh))
     data = _HPGL_CONT_MASKED_ARRAY(data=out_prop.data.ctypes.data_as(C.POINTER(C.c_float)),
                                    mask=out_prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
-    mean_arg = C.byref(C.c_double(mean)) if mean is not None else None
-    _hpgl_so.hpgl_sgs_simulation(C.byref(data), C.byref(sgsp), cdf, mean_arg, sim)
+    if mean is None or numpy.isscalar(mean):
+        mean_arg = C.byref(C.c_double(mean)) if mean is not None else None
+        _hpgl_so.hpgl_sgs_simulation(C.byref(data), C.byref(sgsp), cdf, mean_arg, sim)
+    else:
+        mf = numpy.require(mean, "float32", "F")
+        if mf.size != grid.x * grid.y * grid.z:
+            raise ValueError("sgs_simulation: the mean array does not match the grid")
+        keep.append(mf)
+        means = _HPGL_FLOAT_ARRAY(data=mf.ctypes.data_as(C.POINTER(C.c_float)), shape=sh)
+        _hpgl_so.hpgl_sgs_lvm_simulation(C.byref(data), C.byref(sgsp), cdf, C.byref(means), sim)
     _check_native("sgs_simulation")
     return out_prop

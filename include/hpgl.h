@@ -93,6 +93,12 @@ struct hpgl_ik_params_t {
     double m_marginal_prob;
 };
 
+/* src/geo_bsd/hpgl/api.h:39-43 */
+struct hpgl_float_array_t {
+    float* m_data;
+    struct hpgl_shape_t m_shape;
+};
+
 /* src/geo_bsd/hpgl/api.h:45-49 */
 struct hpgl_ubyte_array_t {
     unsigned char* m_data;
@@ -167,6 +173,13 @@ void hpgl_median_ik(struct hpgl_ind_masked_array_t* in_data, struct hpgl_median_
 void hpgl_sgs_simulation(struct hpgl_cont_masked_array_t* data, struct hpgl_sgs_params_t* params,
                          struct hpgl_non_parametric_cdf_t* cdf, double* mean,
                          struct hpgl_ubyte_array_t* simulation_mask);
+
+/* Replaces api.cpp:519-548 (geo_bsd.sgs_simulation with a mean array):
+ * simple kriging with one mean per cell (taken through the normal-score
+ * transform with the data when cdf is given). */
+void hpgl_sgs_lvm_simulation(struct hpgl_cont_masked_array_t* data, struct hpgl_sgs_params_t* params,
+                             struct hpgl_non_parametric_cdf_t* cdf, struct hpgl_float_array_t* means,
+                             struct hpgl_ubyte_array_t* simulation_mask);
 
 /* Replaces api.cpp:412-455 (geo.lvm_kriging, lvm_kriging.cpp:16-39): simple
  * kriging where every cell has its own mean, mean_data (N floats, the

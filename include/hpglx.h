@@ -103,12 +103,13 @@ int hpglx_indicator_kriging(const struct hpgl_ik_params_t* params, int ncat, con
                             unsigned char* out_mask, float* out_prob, int precision, void* stream);
 
 /* Sequential Gaussian simulation on device arrays (values, mask: N each;
- * simulation_mask: N or null), as hpgl_sgs_simulation; the CDF tables are
- * host arrays.  Returns 0, or -1 with the message in
+ * simulation_mask: N or null; mean_field: N or null, simple kriging with a
+ * locally varying mean as hpgl_sgs_lvm_simulation), as hpgl_sgs_simulation;
+ * the CDF tables are host arrays.  Returns 0, or -1 with the message in
  * hpgl_get_last_exception_message(). */
 int hpglx_sgs_simulation(float* values, unsigned char* mask, const int dims[3], const struct hpgl_sgs_params_t* params,
                          const struct hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask,
-                         void* stream);
+                         const float* mean_field, void* stream);
 
 /* Property files straight to and from device buffers (the work of
  * hpgl_read_inc_file_float / _byte and hpgl_write_inc_file_float / _byte,

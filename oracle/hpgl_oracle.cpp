@@ -1082,15 +1082,26 @@ void orc_gauss(const double* x, int n, double mean, double var, int inverse, dou
 // to simulate (== 1); cdf_size > 0 applies the normal-score transform before
 // and the back transform after.  out_order (may be null, N ints) receives the
 // simulated nodes in path order, -1 padded.  Returns the number simulated, or
-// -1 on invalid parameters.
+// -1 on invalid parameters.  lvm_mean (simple kriging only, may be null):
+// one mean per cell (sequential_gaussian_simulation.cpp:80-131), taken
+// through the normal-score transform with the values (cdf_utils.h:44-50);
+// the centre and every neighbour then use their own mean.
 long long orc_sgs(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax, int nx, int ny, int nz,
                   float* values, unsigned char* mask, const unsigned char* simmask, long seed,
-                  const float* cdf_vals, const float* cdf_probs, long long cdf_size, int* out_order) {
-    if (kmax < 1 || (kind != OK && kind != SK)) return -1;
+                  const float* cdf_vals, const float* cdf_probs, long long cdf_size, int* out_order,
+                  const float* lvm_mean) {
+    if (kmax < 1 || (kind != OK && kind != SK) || (lvm_mean && kind != SK)) return -1;
     Setup s;
     if (!setup(s, SK, (const CovParams*)cp, radiuses, kmax, nx, ny, nz, mask)) return -1;
     s.kind = (Kind)kind;  // algebra of the kind, covariances clipped to the box
     const long long N = s.g.n();
+    std::vector<float> lm;
+    if (lvm_mean) {
+        lm.assign(lvm_mean, lvm_mean + N);
+        if (cdf_size > 0)
+            for (long long i = 0; i < N; ++i)
+                lm[(size_t)i] = (float)gauss_inverse(np_prob(cdf_vals, cdf_probs, cdf_size, lm[(size_t)i]), 0, 1);
+    }
     if (cdf_size > 0)
         for (long long i = 0; i < N; ++i)
             if (mask[i]) {
@@ -1127,8 +1138,11 @@ long long orc_sgs(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax
         const double u = dist(gen);
         double value;
         if (cr.status == ST_KRIGED) {
-            float est = 0.0f;  // combine with mean 0 (single_mean_t(0) / no_mean_t)
-            for (size_t i = 0; i < nb.size(); ++i) est += (float)(cr.w[i] * (values[nb[i]] - 0.0f));
+            // combine (combiner.h:10-25): mean 0 (single_mean_t(0) / no_mean_t)
+            // or the cells' own means
+            float est = lvm_mean ? lm[(size_t)node] : 0.0f;
+            for (size_t i = 0; i < nb.size(); ++i)
+                est += (float)(cr.w[i] * (values[nb[i]] - (lvm_mean ? lm[(size_t)nb[i]] : 0.0f)));
             value = gauss_inverse(u, (double)est, cr.var);
         } else {
             value = gauss_inverse(u, 0, 1);

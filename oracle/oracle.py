@@ -512,10 +512,11 @@ def gauss(x, mean=0.0, var=1.0, inverse=False):
     return out
 
 
-def sgs(kind, cov, radiuses, max_nb, shape, values, mask, seed, simmask=None, cdf=None):
+def sgs(kind, cov, radiuses, max_nb, shape, values, mask, seed, simmask=None, cdf=None, lvm_mean=None):
     """Sequential Gaussian simulation along the reference's path.  Returns
     (values, mask, order): the realization, the final informed flags and the
-    simulated nodes in path order.  ``cdf`` = (values, probs) or None."""
+    simulated nodes in path order.  ``cdf`` = (values, probs) or None;
+    ``lvm_mean`` (simple kriging): one mean per cell, or None."""
     nx, ny, nz = shape
     n = nx * ny * nz
     v = np.array(values, np.float32).reshape(-1)
@@ -528,11 +529,12 @@ def sgs(kind, cov, radiuses, max_nb, shape, values, mask, seed, simmask=None, cd
         cp = np.ascontiguousarray(cdf[1], np.float32)
         csize = cv.size
     order = np.zeros(n, np.int32)
+    lm = None if lvm_mean is None else np.ascontiguousarray(lvm_mean, np.float32).reshape(-1)
     f = lib().orc_sgs
     f.restype = C.c_longlong
     k = f(C.c_int(kind), C.byref(cov), (C.c_int * 3)(*radiuses), C.c_int(max_nb), C.c_int(nx), C.c_int(ny),
           C.c_int(nz), _p(v, C.c_float), _p(m, C.c_ubyte), _p(sm, C.c_ubyte), C.c_long(seed),
-          _p(cv, C.c_float), _p(cp, C.c_float), C.c_longlong(csize), _p(order, C.c_int))
+          _p(cv, C.c_float), _p(cp, C.c_float), C.c_longlong(csize), _p(order, C.c_int), _p(lm, C.c_float))
     if k < 0:
         raise ValueError("oracle rejected the parameters")
     return v, m, order[:k]

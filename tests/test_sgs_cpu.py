@@ -147,3 +147,17 @@ def test_calc_cdf_hand_case():
     # distinct informed values 1, 2, 3 with counts 2, 2, 1 of 5; the last dropped
     assert cdf.values.tolist() == [1.0, 2.0]
     assert cdf.probs.tolist() == [np.float32(0.4), np.float32(0.8)]
+
+
+def test_sgs_lvm_constant_mean_field_is_simple_kriging(orc):
+    """A mean field of zeros is the simple-kriging mean 0: the realization is
+    the plain simulation's bit for bit; a nonzero field changes it."""
+    shape, values, mask = _sgs_case(seed=9, shape=(10, 9, 3), frac=0.25)
+    cov = orc.make_cov(1, (5, 5, 2), sill=1.0, nugget=0.1)
+    base, bm, _ = orc.sgs(1, cov, (3, 3, 1), 8, shape, values, mask, 5)
+    z, zm, _ = orc.sgs(1, cov, (3, 3, 1), 8, shape, values, mask, 5, lvm_mean=np.zeros(values.size, np.float32))
+    assert np.array_equal(base, z) and np.array_equal(bm, zm)
+    lv, _, _ = orc.sgs(1, cov, (3, 3, 1), 8, shape, values, mask, 5, lvm_mean=np.full(values.size, 0.5, np.float32))
+    assert not np.array_equal(lv[mask == 0], base[mask == 0])
+    with pytest.raises(ValueError):
+        orc.sgs(0, cov, (3, 3, 1), 8, shape, values, mask, 5, lvm_mean=np.zeros(values.size, np.float32))

@@ -37,7 +37,7 @@ def _models(orc, c):
     return orc.make_cov(t, ranges, sill=1.0, nugget=nugget), CovarianceModel(t, ranges, (0.0, 0.0, 0.0), 1.0, nugget)
 
 
-def _device_sgs(values, mask, c, model, simmask=None, cdf=None):
+def _device_sgs(values, mask, c, model, simmask=None, cdf=None, lvm=None):
     import torch
     from geo_bsd import device as D
     print(f"[gpu] sgs {c['name']} shape={c['shape']} k={c['k']} kind={c['kind']}", flush=True)
@@ -45,8 +45,9 @@ def _device_sgs(values, mask, c, model, simmask=None, cdf=None):
     v = torch.from_numpy(values.copy()).to(dev)
     m = torch.from_numpy(mask.copy()).to(dev)
     sm = None if simmask is None else torch.from_numpy(simmask).to(dev)
+    lm = None if lvm is None else torch.from_numpy(lvm).to(dev)
     D.sgs_simulation(v, m, c["shape"], c["r"], c["k"], model, c["seed"], "sk" if c["kind"] == 1 else "ok", cdf=cdf,
-                     simulation_mask=sm)
+                     simulation_mask=sm, mean_field=lm)
     torch.cuda.synchronize()
     return v.cpu().numpy(), m.cpu().numpy()
 
@@ -139,3 +140,39 @@ def test_sgs_invalid_parameters_raise():
         D.sgs_simulation(v, m, (8, 8, 2), (2, 2, 1), 40, model, 1)  # max_neighbours above 32
     with pytest.raises(RuntimeError):
         D.sgs_simulation(v, m, (8, 8, 2), (2, 2, 1), 8, model, -3)  # negative seed
+
+
+def test_sgs_locally_varying_mean(orc):
+    """Simple kriging with one mean per cell, with and without the
+    normal-score transform (which also takes the means)."""
+    c = CASES[2]
+    values, mask = _data(c["shape"], c["frac"], 31)
+    cov, model = _models(orc, c)
+    rng = np.random.default_rng(32)
+    lvm = (0.3 * rng.standard_normal(values.size)).astype(np.float32)
+    ov, om, _ = orc.sgs(1, cov, c["r"], c["k"], c["shape"], values, mask, c["seed"], lvm_mean=lvm)
+    gv, gm = _device_sgs(values, mask, c, model, lvm=lvm)
+    _compare(gv, gm, ov, om)
+    pos = np.where(mask == 1, np.exp(values), 0.0).astype(np.float32)
+    vals = np.unique(pos[mask == 1])
+    cnt = np.array([(pos[mask == 1] == x).sum() for x in vals], np.float64)
+    cdf = (vals[:-1].astype(np.float32), np.add.accumulate(cnt[:-1] / mask.sum()).astype(np.float32))
+    lvm2 = np.exp(lvm).astype(np.float32)
+    ov, om, _ = orc.sgs(1, cov, c["r"], c["k"], c["shape"], pos, mask, c["seed"], cdf=cdf, lvm_mean=lvm2)
+    gv, gm = _device_sgs(pos, mask, c, model, cdf=cdf, lvm=lvm2)
+    _compare(gv, gm, ov, om)
+
+
+def test_geo_sgs_lvm_matches_oracle(orc):
+    from geo_bsd import ContProperty, CovarianceModel, SugarboxGrid, sgs_simulation
+    c = CASES[0]
+    values, mask = _data(c["shape"], c["frac"], 41)
+    prop = ContProperty(values.reshape(c["shape"], order="F"), mask.reshape(c["shape"], order="F"))
+    t, ranges, nugget = c["cov"]
+    model = CovarianceModel(t, ranges, (0.0, 0.0, 0.0), 1.0, nugget)
+    lvm = np.linspace(-0.5, 0.5, values.size).astype(np.float32)
+    out = sgs_simulation(prop, SugarboxGrid(*c["shape"]), None, c["r"], c["k"], model, 17,
+                         mean=lvm.reshape(c["shape"], order="F"))
+    cov = orc.make_cov(t, ranges, sill=1.0, nugget=nugget)
+    ov, om, _ = orc.sgs(1, cov, c["r"], c["k"], c["shape"], values, mask, 17, lvm_mean=lvm)
+    _compare(out.data.reshape(-1, order="F"), out.mask.reshape(-1, order="F"), ov, om)
