@@ -19,7 +19,8 @@ categories, 200x200x100), `lvm` (kriging with a locally varying mean,
 200x200x40), `io` (INC property file of 4M values: GPU parse and print) and
 `variogram` (experimental semivariogram of the 200x200x100 median-IK hard
 data, 40,000 cells, and of 20,000 scattered points) and `sgs` (sequential
-Gaussian simulation of the 100x100x50 grid, 475,000 cells).
+Gaussian simulation of the 100x100x50 grid, 475,000 cells; one realization,
+and `--realizations` of them advancing together).
 Each is timed beside the reference's own CPU build where it is present (for
 `sgs`, beside the CPU oracle: the reference's simulation cannot be built).
 """
@@ -350,6 +351,24 @@ def side_workload(name, args):
                                         "kriging, exponential (20,20,4), nugget 0.05, radiuses (20,20,4), "
                                         "max_neighbours 24, the reference's path and draws for seed 20240611",
                             "simulated_cells": nsim})
+        # several realizations advancing together (one launch per level
+        # index over all of them): seeds seed .. seed + R - 1
+        R = args.realizations
+        if R > 1:
+            seeds = [seed + i for i in range(R)]
+
+            def run_batch():
+                return D.sgs_realizations(dv0, dm0, wl.shape, wl.radiuses, wl.k, model, seeds, "sk")
+
+            tb = _timed(run_batch, max(2, args.steps // 4), 1, sync)
+            bv, _ = run_batch()
+            dv.copy_(dv0)
+            dm.copy_(dm0)
+            D.sgs_simulation(dv, dm, wl.shape, wl.radiuses, wl.k, model, seeds[-1], "sk")
+            sync()
+            line["realizations"] = {"count": R, "value": R * nsim / tb, "unit": "simulated cells/s",
+                                    "ms_per_call": tb * 1e3,
+                                    "last_equals_single_call": bool(torch.equal(bv[-1], dv))}
         # agreement with the oracle's one-by-one walk, and its speed, on a
         # 100x100x10 slab (the reference's own simulation cannot be built:
         # its path generator needs a prime table it does not ship)
@@ -390,6 +409,8 @@ def main():
     ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
     ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io", "variogram", "sgs"],
                     help="ok (default): the headline OK 100x100x50 workload; others: see the module docstring")
+    ap.add_argument("--realizations", type=int, default=8,
+                    help="sgs workload: also time this many realizations run together (1 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "
                          "scaling memory counters)")

@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -562,13 +563,21 @@ bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned ch
     return true;
 }
 
-// Sequential Gaussian simulation over host copies of the property
-// (sequential_gaussian_simulation.cpp:18-78): the normal-score transforms and
-// the path, draws and levels on the host, the levels on the GPU.  v and m (N)
-// are updated in place; simmask (host, N) may be null.
-int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vector<float>& v,
-            std::vector<unsigned char>& m, const unsigned char* simmask, const hpgl_non_parametric_cdf_t* cdf,
-            const float* lvm_host, hipStream_t st, std::string& err) {
+bool sgs_transforms(const hpgl_non_parametric_cdf_t* cdf) {
+    return cdf && cdf->m_size > 0 && cdf->m_values && cdf->m_probs;
+}
+
+// Sequential Gaussian simulation (sequential_gaussian_simulation.cpp:18-78):
+// the normal-score transform and each realization's path, draws and levels
+// on the host, the levels on the GPU.  v and m (host, N) are the data; v is
+// transformed in place when the CDF is given.  Realization r (seed seeds[r])
+// is written, still in normal scores, to d_v + r N and d_m + r N (device);
+// all of them advance together, one launch per level index.  simmask (host,
+// N) and lvm_host (host, N) may be null.
+int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vector<float>& v,
+             std::vector<unsigned char>& m, const unsigned char* simmask, const hpgl_non_parametric_cdf_t* cdf,
+             const float* lvm_host, const long* seeds, int nreal, float* d_v, unsigned char* d_m, hipStream_t st,
+             std::string& err) {
     if (p->m_kriging_kind != KIND_OK && p->m_kriging_kind != KIND_SK) {
         err = "sequential simulation takes simple (1) or ordinary (0) kriging";
         return -1;
@@ -586,7 +595,7 @@ int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vect
     DevPlan* plan = get_plan(L, KIND_SK, c, p->m_radiuses, p->m_max_neighbours, err);
     if (!plan) return -1;
     const long long n = (long long)v.size();
-    const bool transform = cdf && cdf->m_size > 0 && cdf->m_values && cdf->m_probs;
+    const bool transform = sgs_transforms(cdf);
     if (transform) sgs_to_normal_scores(v.data(), m.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
     std::vector<float> lvm;
     float* d_lvm = nullptr;
@@ -600,27 +609,63 @@ int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vect
         d_lvm = (float*)L.ws.get("sgs_io_lvm", (size_t)n * 4, err);
         if (!err.empty()) return -1;
     }
-    SgsSchedule s;
-    if (!sgs_schedule(dims, p->m_radiuses, m.data(), simmask, p->m_seed, s, err)) return -1;
-    auto* dv = (float*)L.ws.get("sgs_io_val", (size_t)n * 4, err);
-    auto* dm = (unsigned char*)L.ws.get("sgs_io_mask", (size_t)n, err);
-    if (!err.empty()) return -1;
+    // The schedules are independent host work: one thread per realization,
+    // at most 16 at a time.
+    std::vector<SgsSchedule> sched((size_t)nreal);
+    std::vector<std::string> errs((size_t)nreal);
+    std::vector<char> ok((size_t)nreal, 0);
+    const int workers = std::max(1, std::min(nreal, 16));
+    auto work = [&](int w) {
+        for (int r = w; r < nreal; r += workers)
+            ok[(size_t)r] = sgs_schedule(dims, p->m_radiuses, m.data(), simmask, seeds[r], sched[(size_t)r],
+                                         errs[(size_t)r]);
+    };
+    if (workers == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < workers; ++w) pool.emplace_back(work, w);
+        for (auto& t : pool) t.join();
+    }
+    for (int r = 0; r < nreal; ++r)
+        if (!ok[(size_t)r]) {
+            err = errs[(size_t)r];
+            return -1;
+        }
     if (!ensure_events(L, err)) return -1;
     if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess ||
-        hipMemcpyAsync(dv, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dm, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
         (d_lvm && hipMemcpyAsync(d_lvm, lvm.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess)) {
         err = "host to device copy failed";
         return -1;
     }
-    if (sgs_run(*plan, L.ws, p->m_kriging_kind, dims, dv, dm, d_lvm, s, st, err) != 0) return -1;
+    for (int r = 0; r < nreal; ++r)
+        if (hipMemcpyAsync(d_v + (long long)r * n, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_m + (long long)r * n, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+            err = "host to device copy failed";
+            return -1;
+        }
+    // sgs_run synchronises st before returning, so the host arrays above
+    // outlive their copies.
+    return sgs_run(*plan, L.ws, p->m_kriging_kind, dims, d_v, d_m, d_lvm, sched, st, err);
+}
+
+// One realization over host copies of the property: v and m (N) are updated
+// in place, back in the data's units when the CDF is given.
+int run_sgs(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vector<float>& v,
+            std::vector<unsigned char>& m, const unsigned char* simmask, const hpgl_non_parametric_cdf_t* cdf,
+            const float* lvm_host, hipStream_t st, std::string& err) {
+    const long long n = (long long)v.size();
+    auto* dv = (float*)L.ws.get("sgs_io_val", (size_t)n * 4, err);
+    auto* dm = (unsigned char*)L.ws.get("sgs_io_mask", (size_t)n, err);
+    if (!err.empty()) return -1;
+    if (sgs_core(L, p, dims, v, m, simmask, cdf, lvm_host, &p->m_seed, 1, dv, dm, st, err) != 0) return -1;
     if (hipMemcpyAsync(v.data(), dv, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(m.data(), dm, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipEventRecord(L.done, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
         err = "device to host copy failed";
         return -1;
     }
-    if (transform) sgs_from_normal_scores(v.data(), m.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
+    if (sgs_transforms(cdf)) sgs_from_normal_scores(v.data(), m.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
     return 0;
 }
 
@@ -1277,6 +1322,53 @@ int hpglx_sgs_simulation(float* values, unsigned char* mask, const int dims[3], 
         hipMemcpyAsync(mask, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return fail(L, "hpglx_sgs_simulation: host to device copy failed");
+    return 0;
+}
+
+int hpglx_sgs_realizations(const float* values, const unsigned char* mask, const int dims[3],
+                           const hpgl_sgs_params_t* params, const long* seeds, int nreal,
+                           const hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask,
+                           const float* mean_field, float* out_values, unsigned char* out_mask, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!values || !mask || !dims || !params || !seeds || !out_values || !out_mask)
+        return fail(L, "hpglx_sgs_realizations: null argument");
+    if (nreal < 1 || nreal > 4096) return fail(L, "hpglx_sgs_realizations: nreal must be 1..4096");
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || n > 2147483647LL || n * nreal > 2147483647LL)
+        return fail(L, "hpglx_sgs_realizations: invalid grid dimensions (nreal x cells must stay below 2^31)");
+    const hipStream_t st = (hipStream_t)stream;
+    std::vector<float> v((size_t)n);
+    std::vector<unsigned char> m((size_t)n), sm(simulation_mask ? (size_t)n : 0);
+    std::vector<float> lvm(mean_field ? (size_t)n : 0);
+    if ((mean_field && hipMemcpyAsync(lvm.data(), mean_field, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipMemcpyAsync(v.data(), values, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(m.data(), mask, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (simulation_mask &&
+         hipMemcpyAsync(sm.data(), simulation_mask, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(L, "hpglx_sgs_realizations: device to host copy failed");
+    if (sgs_core(L, params, dims, v, m, simulation_mask ? sm.data() : nullptr, cdf, mean_field ? lvm.data() : nullptr,
+                 seeds, nreal, out_values, out_mask, st, err) != 0)
+        return fail(L, err);
+    if (sgs_transforms(cdf)) {
+        // back transform on the host, one realization at a time
+        std::vector<unsigned char> rm((size_t)n);
+        for (int r = 0; r < nreal; ++r) {
+            float* dv = out_values + (long long)r * n;
+            const unsigned char* dm = out_mask + (long long)r * n;
+            if (hipMemcpyAsync(v.data(), dv, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(rm.data(), dm, (size_t)n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return fail(L, "hpglx_sgs_realizations: device to host copy failed");
+            sgs_from_normal_scores(v.data(), rm.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
+            if (hipMemcpyAsync(dv, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return fail(L, "hpglx_sgs_realizations: host to device copy failed");
+        }
+    }
     return 0;
 }
 

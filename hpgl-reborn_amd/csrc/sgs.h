@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "krige_launch.h"
 #include "sgs_plan.h"
@@ -19,8 +20,11 @@ namespace hpglx {
 // built with the covariance clipped to the radius box (the simple-kriging
 // plan).
 // d_lvm (device, N; simple kriging only, may be null): the cells' own means
-// in the combine (sequential_gaussian_simulation.cpp:80-131).
+// in the combine (sequential_gaussian_simulation.cpp:80-131).  Several
+// realizations (one schedule each) run together, step l taking level l of
+// every one: values and mask then hold one slice of N per realization, each
+// starting from its conditioning data.
 int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_values, unsigned char* d_mask,
-            const float* d_lvm, const SgsSchedule& s, hipStream_t st, std::string& err);
+            const float* d_lvm, const std::vector<SgsSchedule>& sched, hipStream_t st, std::string& err);
 
 }  // namespace hpglx

@@ -45,7 +45,7 @@ unsigned blocks_of(long long work, int per) {
 }  // namespace
 
 int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_values, unsigned char* d_mask,
-            const float* d_lvm, const SgsSchedule& s, hipStream_t st, std::string& err) {
+            const float* d_lvm, const std::vector<SgsSchedule>& sched, hipStream_t st, std::string& err) {
     const PlanHost& h = plan.h;
     if (kind != KIND_OK && kind != KIND_SK) {
         err = "sequential simulation takes simple or ordinary kriging";
@@ -74,28 +74,53 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     g.cx = g.cy = g.cz = 0;
     const long long n = g.n, P = (long long)g.px * g.py * g.pz;
     const int K = h.kmax, T = plan.tsize;
-    const long long m = (long long)s.cells.size();
-    const int L = (int)s.level_off.size() - 1;
+    const int R = (int)sched.size();
+    // Step l runs level l of every realization.
+    int steps = 0;
+    long long m = 0;
+    for (const SgsSchedule& s : sched) {
+        steps = std::max(steps, (int)s.level_off.size() - 1);
+        m += (long long)s.cells.size();
+    }
     if (m == 0) return 0;
-    auto* pmask = (unsigned char*)ws.get("sgs_pmask", (size_t)P, err);
-    auto* nboff = (int*)ws.get("sgs_nboff", (size_t)n * K * 4, err);
-    auto* est = (float*)ws.get("sgs_est", (size_t)n * 4, err);
-    auto* var = (double*)ws.get("sgs_var", (size_t)n * 8, err);
-    auto* status = (unsigned char*)ws.get("sgs_status", (size_t)n, err);
+    std::vector<int> ecell((size_t)m), ereal((size_t)m), step_off((size_t)steps + 1, 0);
+    std::vector<double> ez((size_t)m);
+    {
+        size_t e = 0;
+        for (int l = 0; l < steps; ++l) {
+            step_off[(size_t)l] = (int)e;
+            for (int r = 0; r < R; ++r) {
+                const SgsSchedule& s = sched[(size_t)r];
+                if (l + 1 >= (int)s.level_off.size()) continue;
+                for (int i = s.level_off[(size_t)l]; i < s.level_off[(size_t)l + 1]; ++i, ++e) {
+                    ecell[e] = s.cells[(size_t)i];
+                    ereal[e] = r;
+                    ez[e] = s.z[(size_t)i];
+                }
+            }
+        }
+        step_off[(size_t)steps] = (int)e;
+    }
+    auto* pmask = (unsigned char*)ws.get("sgs_pmask", (size_t)(R * P), err);
+    auto* nboff = (int*)ws.get("sgs_nboff", (size_t)R * n * K * 4, err);
+    auto* est = (float*)ws.get("sgs_est", (size_t)R * n * 4, err);
+    auto* var = (double*)ws.get("sgs_var", (size_t)R * n * 8, err);
+    auto* status = (unsigned char*)ws.get("sgs_status", (size_t)R * n, err);
     auto* tdelta = (int*)ws.get("sgs_tdelta", (size_t)std::max(T, 1) * 4, err);
     auto* tpack = (int*)ws.get("sgs_tpack", (size_t)std::max(T, 1) * 4, err);
     auto* cells = (int*)ws.get("sgs_cells", (size_t)m * 4, err);
+    auto* reals = (int*)ws.get("sgs_reals", (size_t)m * 4, err);
     auto* zdev = (double*)ws.get("sgs_z", (size_t)m * 8, err);
     if (!err.empty()) return -1;
-    std::vector<int> lc((size_t)L);
-    for (int l = 0; l < L; ++l) lc[(size_t)l] = s.level_off[(size_t)l + 1] - s.level_off[(size_t)l];
-    if (hipMemcpyAsync(cells, s.cells.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(zdev, s.z.data(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+    if (hipMemcpyAsync(cells, ecell.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(reals, ereal.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(zdev, ez.data(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
         err = "host to device copy failed";
         return -1;
     }
-    hipLaunchKernelGGL(k_sgs_pmask, dim3((unsigned)std::min<long long>(4096, blocks_of(P, 256))), dim3(256), 0, st, g,
-                       (const unsigned char*)d_mask, pmask);
+    for (int r = 0; r < R; ++r)
+        hipLaunchKernelGGL(k_sgs_pmask, dim3((unsigned)std::min<long long>(4096, blocks_of(P, 256))), dim3(256), 0, st,
+                           g, (const unsigned char*)(d_mask + (long long)r * n), pmask + (long long)r * P);
     if (T > 0)
         hipLaunchKernelGGL(k_sgs_tdelta, dim3(blocks_of(T, 256)), dim3(256), 0, st, g, T, (const int*)plan.tdx,
                            (const int*)plan.tdy, (const int*)plan.tdz, tdelta, tpack);
@@ -130,12 +155,15 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     la.tdelta = tdelta;
     la.tpack = tpack;
     la.pmask = pmask;
+    la.n = n;
+    la.pn = P;
     la.values = d_values;
     la.mask = d_mask;
-    for (int l = 0; l < L; ++l) {
-        const int off = s.level_off[(size_t)l];
+    for (int l = 0; l < steps; ++l) {
+        const int off = step_off[(size_t)l];
         la.cells = cells + off;
-        la.ncells = lc[(size_t)l];
+        la.real = reals + off;
+        la.ncells = step_off[(size_t)l + 1] - off;
         la.z = zdev + off;
         if (K <= 8)
             kind == KIND_OK ? sgs_level_8_0(so, la, st) : sgs_level_8_1(so, la, st);
@@ -150,7 +178,7 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
         err = "sequential simulation kernel launch failed";
         return -1;
     }
-    // the schedule's host vectors are read by the copies above
+    // the host vectors above are read by the copies
     if (hipStreamSynchronize(st) != hipSuccess) {
         err = "sequential simulation failed on the device";
         return -1;

@@ -34,8 +34,11 @@ struct SgsLevelArgs {
     const int* cells;         // the level's cells
     int ncells;
     const double* z;          // their standard deviates (NaN: the draw gives the mean)
-    float* values;            // property values (N), updated
-    unsigned char* mask;      // informed mask (N), updated
+    const int* real;          // each cell's realization (several run in one launch)
+    long long n;              // cells per realization: values, mask, results, neighbours
+    long long pn;             // padded cells per realization
+    float* values;            // property values (realizations x N), updated
+    unsigned char* mask;      // informed masks (realizations x N), updated
 };
 
 // One per (bucket, kind): solve_n<NMAX>_k<KIND>.hip.

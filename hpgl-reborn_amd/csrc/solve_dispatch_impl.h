@@ -133,11 +133,21 @@ void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, cons
 // ballot, 64 entries at a time, and keeps the first kmax in template order
 // (sugarbox_neighbour_lookup.h:37-62); the first G lanes then solve the cell
 // as the list solve does; lane 0 draws (sample.h:14-25,
-// gaussian_distribution.h:56-89, products and sums rounded singly).
+// gaussian_distribution.h:56-89, products and sums rounded singly).  Each
+// cell works in its realization's slices of the arrays.
 template <int NMAX, int G, int KIND, bool OCT>
 __global__ __launch_bounds__(64) void k_sgs_level(SolveArgs a, SgsLevelArgs la) {
     const int lane = threadIdx.x;
     const int cell = la.cells[blockIdx.x];
+    const long long r = la.real[blockIdx.x];
+    a.values += r * la.n;
+    a.nboff += r * la.n * a.kmax;
+    a.out_val += r * la.n;
+    a.out_var += r * la.n;
+    a.out_status += r * la.n;
+    float* values = la.values + r * la.n;
+    unsigned char* mask = la.mask + r * la.n;
+    unsigned char* pmask = la.pmask + r * la.pn;
     const GridDims& g = a.g;
     const int x = cell % g.nx, y = (cell / g.nx) % g.ny, zc = cell / (g.nx * g.ny);
     const long long pcell = ((long long)(zc + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
@@ -145,7 +155,7 @@ __global__ __launch_bounds__(64) void k_sgs_level(SolveArgs a, SgsLevelArgs la) 
     int cnt = 0;
     for (int t0 = 0; t0 < la.T && cnt < a.kmax; t0 += 64) {
         const int t = t0 + lane;
-        const bool hit = t < la.T && la.pmask[pcell + la.tdelta[t]] != 0;
+        const bool hit = t < la.T && pmask[pcell + la.tdelta[t]] != 0;
         const unsigned long long b = __ballot(hit);
         const int rank = cnt + __popcll(b & ((1ull << lane) - 1ull));
         if (hit && rank < a.kmax) nboff[rank] = la.tpack[t];
@@ -167,9 +177,9 @@ __global__ __launch_bounds__(64) void k_sgs_level(SolveArgs a, SgsLevelArgs la) 
         } else {
             v = isnan(d) ? 0.0f : (float)d;
         }
-        la.values[cell] = v;
-        la.mask[cell] = 1;
-        la.pmask[pcell] = 1;
+        values[cell] = v;
+        mask[cell] = 1;
+        pmask[pcell] = 1;
     }
 }
 

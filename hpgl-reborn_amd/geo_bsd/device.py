@@ -213,17 +213,8 @@ def indicator_kriging(data, mask, grid, categories, marginal_probs, precision="f
     return out
 
 
-def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed, kriging_type="sk", cdf=None,
-                   simulation_mask=None, mean_field=None, stream=None):
-    """Sequential Gaussian simulation in place on device tensors (work of
-    hpgl_sgs_simulation, api.cpp:485-516): float32 values and uint8 mask of
-    N cells are overwritten with the realization; the model is used as
-    given (geo_bsd.sgs_simulation scales it to unit sill first).  cdf =
-    (values, probs) host arrays of the normal-score transform, or None;
-    simulation_mask: uint8 tensor (1 = simulate) or None; mean_field:
-    float32 tensor of one mean per cell (simple kriging with a locally
-    varying mean) or None.  Returns the number of cells informed
-    afterwards."""
+def _sgs_args(values, mask, grid, radiuses, max_neighbours, cov_model, seed, kriging_type, cdf, simulation_mask,
+              mean_field):
     torch = _torch()
     import numpy as np
     from .hpgl_wrap import _HPGL_SGS_PARAMS, hpgl_non_parametric_cdf_t
@@ -241,20 +232,68 @@ def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed
                          nugget=float(cov_model.nugget), radiuses=tuple(int(r) for r in radiuses),
                          max_neighbours=int(max_neighbours), kriging_kind=1 if kriging_type == "sk" else 0,
                          seed=int(seed), min_neighbours=0)
-    cdf_arg = None
+    cdf_arg, keep = None, ()
     if cdf is not None:
         cv = np.ascontiguousarray(cdf[0], np.float32)
         cp = np.ascontiguousarray(cdf[1], np.float32)
+        keep = (cv, cp)
         cdf_arg = C.byref(hpgl_non_parametric_cdf_t(values=cv.ctypes.data_as(C.POINTER(C.c_float)),
                                                     probs=cp.ctypes.data_as(C.POINTER(C.c_float)), size=cv.size))
+    return (nx, ny, nz), p, cdf_arg, keep
+
+
+def sgs_simulation(values, mask, grid, radiuses, max_neighbours, cov_model, seed, kriging_type="sk", cdf=None,
+                   simulation_mask=None, mean_field=None, stream=None):
+    """Sequential Gaussian simulation in place on device tensors (work of
+    hpgl_sgs_simulation, api.cpp:485-516): float32 values and uint8 mask of
+    N cells are overwritten with the realization; the model is used as
+    given (geo_bsd.sgs_simulation scales it to unit sill first).  cdf =
+    (values, probs) host arrays of the normal-score transform, or None;
+    simulation_mask: uint8 tensor (1 = simulate) or None; mean_field:
+    float32 tensor of one mean per cell (simple kriging with a locally
+    varying mean) or None.  Returns the number of cells informed
+    afterwards."""
+    torch = _torch()
+    dims, p, cdf_arg, keep = _sgs_args(values, mask, grid, radiuses, max_neighbours, cov_model, seed, kriging_type,
+                                       cdf, simulation_mask, mean_field)
     if stream is None:
         stream = torch.cuda.current_stream(values.device).cuda_stream
-    rc = _hpgl_so.hpglx_sgs_simulation(_ptr(values), _ptr(mask), (C.c_int * 3)(nx, ny, nz), C.byref(p), cdf_arg,
+    rc = _hpgl_so.hpglx_sgs_simulation(_ptr(values), _ptr(mask), (C.c_int * 3)(*dims), C.byref(p), cdf_arg,
                                        _ptr(simulation_mask) if simulation_mask is not None else None,
                                        _ptr(mean_field) if mean_field is not None else None, stream)
+    del keep
     if rc != 0:
         raise RuntimeError("sgs_simulation: " + _hpgl_so.hpglx_last_error().decode())
     return int(mask.sum().item())
+
+
+def sgs_realizations(values, mask, grid, radiuses, max_neighbours, cov_model, seeds, kriging_type="sk", cdf=None,
+                     simulation_mask=None, mean_field=None, stream=None):
+    """Several realizations from one conditioning data set, one per seed:
+    returns (values, mask) tensors of shape (len(seeds), N), row r equal to
+    what sgs_simulation gives with seed seeds[r].  The inputs are left as
+    they are.  The realizations advance together on the GPU, one launch per
+    level index, so a few cells per level from each of them fill the chip."""
+    torch = _torch()
+    seeds = [int(s) for s in seeds]
+    _require(1 <= len(seeds) <= 4096, "seeds must hold 1..4096 values")
+    dims, p, cdf_arg, keep = _sgs_args(values, mask, grid, radiuses, max_neighbours, cov_model, seeds[0],
+                                       kriging_type, cdf, simulation_mask, mean_field)
+    n = dims[0] * dims[1] * dims[2]
+    out_v = torch.empty((len(seeds), n), dtype=torch.float32, device=values.device)
+    out_m = torch.empty((len(seeds), n), dtype=torch.uint8, device=values.device)
+    sarr = (C.c_long * len(seeds))(*seeds)
+    if stream is None:
+        stream = torch.cuda.current_stream(values.device).cuda_stream
+    rc = _hpgl_so.hpglx_sgs_realizations(_ptr(values), _ptr(mask), (C.c_int * 3)(*dims), C.byref(p), sarr,
+                                         len(seeds), cdf_arg,
+                                         _ptr(simulation_mask) if simulation_mask is not None else None,
+                                         _ptr(mean_field) if mean_field is not None else None, _ptr(out_v),
+                                         _ptr(out_m), stream)
+    del keep
+    if rc != 0:
+        raise RuntimeError("sgs_realizations: " + _hpgl_so.hpglx_last_error().decode())
+    return out_v, out_m
 
 
 def indicator_cdf(zfield, mask, grid, thresholds, marginal, radiuses, max_neighbours, cov_model, precision="fp64",

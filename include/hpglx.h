@@ -111,6 +111,19 @@ int hpglx_sgs_simulation(float* values, unsigned char* mask, const int dims[3], 
                          const struct hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask,
                          const float* mean_field, void* stream);
 
+/* Several realizations of hpglx_sgs_simulation from one conditioning data
+ * set (values, mask: device, N; untouched), realization r with seed
+ * seeds[r] (host array of nreal, 1..4096): each equals what
+ * hpglx_sgs_simulation gives with that seed, written to out_values +
+ * r N and out_mask + r N (device, nreal x N).  The realizations advance
+ * together, one launch per level index, which fills the GPU where one
+ * realization's levels are small.  The reference runs realizations one call
+ * at a time (sgs_simulation in geo_bsd); this has no single counterpart. */
+int hpglx_sgs_realizations(const float* values, const unsigned char* mask, const int dims[3],
+                           const struct hpgl_sgs_params_t* params, const long* seeds, int nreal,
+                           const struct hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask,
+                           const float* mean_field, float* out_values, unsigned char* out_mask, void* stream);
+
 /* Property files straight to and from device buffers (the work of
  * hpgl_read_inc_file_float / _byte and hpgl_write_inc_file_float / _byte,
  * api.cpp:53-251): the text is parsed and printed on the GPU.  These

@@ -176,3 +176,75 @@ def test_geo_sgs_lvm_matches_oracle(orc):
     cov = orc.make_cov(t, ranges, sill=1.0, nugget=nugget)
     ov, om, _ = orc.sgs(1, cov, c["r"], c["k"], c["shape"], values, mask, 17, lvm_mean=lvm)
     _compare(out.data.reshape(-1, order="F"), out.mask.reshape(-1, order="F"), ov, om)
+
+
+def _realizations(values, mask, c, model, seeds, **kw):
+    import torch
+    from geo_bsd import device as D
+    print(f"[gpu] sgs_realizations {c['name']} shape={c['shape']} seeds={seeds}", flush=True)
+    dev = torch.device("cuda:0")
+    v = torch.from_numpy(values.copy()).to(dev)
+    m = torch.from_numpy(mask.copy()).to(dev)
+    ov, om = D.sgs_realizations(v, m, c["shape"], c["r"], c["k"], model, seeds,
+                                "sk" if c["kind"] == 1 else "ok", **kw)
+    torch.cuda.synchronize()
+    # the conditioning data are left as they were
+    assert np.array_equal(v.cpu().numpy(), values) and np.array_equal(m.cpu().numpy(), mask)
+    return ov.cpu().numpy(), om.cpu().numpy()
+
+
+@pytest.mark.parametrize("ci", [0, 3])
+def test_sgs_realizations_match_oracle(orc, ci):
+    """Realizations run together equal the oracle's one-by-one walk for each
+    seed, and the one-realization device call for the same seed bit for bit."""
+    c = CASES[ci]
+    values, mask = _data(c["shape"], c["frac"], c["seed"] + 200)
+    cov, model = _models(orc, c)
+    seeds = [c["seed"], 1, 77, 123456, 0]
+    rv, rm = _realizations(values, mask, c, model, seeds)
+    assert rv.shape == (len(seeds), values.size)
+    for i, s in enumerate(seeds):
+        ov, om, _ = orc.sgs(c["kind"], cov, c["r"], c["k"], c["shape"], values, mask, s)
+        _compare(rv[i], rm[i], ov, om)
+    one = dict(c, seed=seeds[2])
+    gv, gm = _device_sgs(values, mask, one, model)
+    assert np.array_equal(rv[2], gv) and np.array_equal(rm[2], gm)
+    # different seeds give different realizations
+    assert not np.array_equal(rv[0], rv[1])
+
+
+def test_sgs_realizations_transform_mask_and_lvm(orc):
+    """The batched call with a simulation mask, a CDF and a mean field: each
+    row equals the single call with its seed."""
+    c = CASES[2]
+    values, mask = _data(c["shape"], c["frac"], 51)
+    pos = np.where(mask == 1, np.exp(values), 0.0).astype(np.float32)
+    _, model = _models(orc, c)
+    rng = np.random.default_rng(52)
+    simmask = (rng.random(values.size) < 0.7).astype(np.uint8)
+    vals = np.unique(pos[mask == 1])
+    cnt = np.array([(pos[mask == 1] == x).sum() for x in vals], np.float64)
+    cdf = (vals[:-1].astype(np.float32), np.add.accumulate(cnt[:-1] / mask.sum()).astype(np.float32))
+    lvm = np.exp(0.2 * rng.standard_normal(values.size)).astype(np.float32)
+    import torch
+    dev = torch.device("cuda:0")
+    seeds = [5, 6, 7]
+    rv, rm = _realizations(pos, mask, c, model, seeds, cdf=cdf, simulation_mask=torch.from_numpy(simmask).to(dev),
+                           mean_field=torch.from_numpy(lvm).to(dev))
+    for i, s in enumerate(seeds):
+        gv, gm = _device_sgs(pos, mask, dict(c, seed=s), model, simmask=simmask, cdf=cdf, lvm=lvm)
+        assert np.array_equal(rv[i], gv) and np.array_equal(rm[i], gm)
+
+
+def test_sgs_realizations_invalid():
+    import torch
+    from geo_bsd import CovarianceModel
+    from geo_bsd import device as D
+    dev = torch.device("cuda:0")
+    v = torch.zeros(8 * 8 * 2, dtype=torch.float32, device=dev)
+    m = torch.zeros(8 * 8 * 2, dtype=torch.uint8, device=dev)
+    model = CovarianceModel(0, (4, 4, 2), (0.0, 0.0, 0.0), 1.0, 0.0)
+    with pytest.raises(RuntimeError):
+        D.sgs_realizations(v, m, (8, 8, 2), (2, 2, 1), 8, model, [1, -2])  # a negative seed
+    with pytest.raises(ValueError):
+        D.sgs_realizations(v, m, (8, 8, 2), (2, 2, 1), 8, model, [])
