@@ -1372,6 +1372,32 @@ int hpglx_sgs_realizations(const float* values, const unsigned char* mask, const
     return 0;
 }
 
+int hpglx_sgs_schedule(const int dims[3], const int radiuses[3], const unsigned char* informed,
+                       const unsigned char* simulation_mask, long seed, int* out_cells, double* out_deviates,
+                       int* out_level, int capacity) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    if (!dims || !radiuses || !informed || capacity < 0 || (capacity > 0 && (!out_cells || !out_deviates || !out_level)))
+        return fail(L, "hpglx_sgs_schedule: null argument");
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || (long long)dims[0] * dims[1] * dims[2] > 2147483647LL)
+        return fail(L, "hpglx_sgs_schedule: invalid grid dimensions");
+    if (radiuses[0] < 0 || radiuses[1] < 0 || radiuses[2] < 0)
+        return fail(L, "hpglx_sgs_schedule: negative radius");
+    SgsSchedule s;
+    std::string err;
+    if (!sgs_schedule(dims, radiuses, informed, simulation_mask, seed, s, err)) return fail(L, err);
+    const int m = (int)s.cells.size();
+    if (m > capacity) return m;
+    for (size_t l = 0; l + 1 < s.level_off.size(); ++l)
+        for (int i = s.level_off[l]; i < s.level_off[l + 1]; ++i) {
+            out_cells[i] = s.cells[(size_t)i];
+            out_deviates[i] = s.z[(size_t)i];
+            out_level[i] = (int)l;
+        }
+    return m;
+}
+
 int hpglx_read_inc_float(const char* filename, float undefined_value, int size, float* d_data, unsigned char* d_mask,
                          void* stream) {
     Library& L = lib();

@@ -210,6 +210,9 @@ _hpgl_so.hpglx_sgs_realizations.restype = C.c_int
 _hpgl_so.hpglx_sgs_realizations.argtypes = [C.c_void_p, C.c_void_p, C.c_int * 3, C.POINTER(_HPGL_SGS_PARAMS),
                                             C.POINTER(C.c_long), C.c_int, C.POINTER(hpgl_non_parametric_cdf_t),
                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+_hpgl_so.hpglx_sgs_schedule.restype = C.c_int
+_hpgl_so.hpglx_sgs_schedule.argtypes = [C.c_int * 3, C.c_int * 3, C.c_void_p, C.c_void_p, C.c_long, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int]
 _hpgl_so.hpglx_template.restype = C.c_int
 _hpgl_so.hpglx_template.argtypes = [C.POINTER(__hpgl_cov_params_t), C.c_int * 3, C.c_void_p, C.c_int]
 _hpgl_so.hpglx_set_profiling.restype = None

@@ -91,3 +91,34 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
         _hpgl_so.hpgl_sgs_lvm_simulation(C.byref(data), C.byref(sgsp), cdf, C.byref(means), sim)
     _check_native("sgs_simulation")
     return out_prop
+
+
+def sgs_schedule(shape, radiuses, informed, seed, simulation_mask=None):
+    """The host schedule of sgs_simulation for a grid of `shape` (nx, ny, nz):
+    the cells to simulate along the reference's random path, grouped into
+    levels that the GPU runs in parallel.  informed and simulation_mask are
+    flat uint8 arrays of nx*ny*nz cells (x fastest).  Returns (cells,
+    deviates, levels) in level order, path order within a level; deviates
+    are the standard normal quantiles of the cells' draws.  Needs no GPU."""
+    nx, ny, nz = (int(v) for v in shape)
+    n = nx * ny * nz
+    inf = numpy.ascontiguousarray(informed, dtype=numpy.uint8).reshape(-1)
+    if inf.size != n:
+        raise ValueError("informed must hold nx*ny*nz values")
+    sm = None
+    if simulation_mask is not None:
+        sm = numpy.ascontiguousarray(simulation_mask, dtype=numpy.uint8).reshape(-1)
+        if sm.size != n:
+            raise ValueError("simulation_mask must hold nx*ny*nz values")
+    dims = (C.c_int * 3)(nx, ny, nz)
+    rad = (C.c_int * 3)(*(int(r) for r in radiuses))
+    args = (dims, rad, inf.ctypes.data, None if sm is None else sm.ctypes.data, int(seed))
+    m = _hpgl_so.hpglx_sgs_schedule(*args, None, None, None, 0)
+    if m < 0:
+        raise ValueError("sgs_schedule: " + _hpgl_so.hpglx_last_error().decode())
+    cells = numpy.empty(m, numpy.int32)
+    dev = numpy.empty(m, numpy.float64)
+    lev = numpy.empty(m, numpy.int32)
+    if m:
+        _hpgl_so.hpglx_sgs_schedule(*args, cells.ctypes.data, dev.ctypes.data, lev.ctypes.data, m)
+    return cells, dev, lev

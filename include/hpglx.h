@@ -124,6 +124,19 @@ int hpglx_sgs_realizations(const float* values, const unsigned char* mask, const
                            const struct hpgl_non_parametric_cdf_t* cdf, const unsigned char* simulation_mask,
                            const float* mean_field, float* out_values, unsigned char* out_mask, void* stream);
 
+/* The host schedule of a sequential simulation, no GPU needed: the cells to
+ * simulate (informed = 0 and simulation_mask null or 1) along the
+ * reference's random path for `seed` (path_random_generator.cpp:20-94),
+ * grouped into levels that run in parallel -- two cells inside each other's
+ * +-radiuses box never share a level, and the earlier on the path has the
+ * lower one.  Writes, level by level and in path order within a level, each
+ * cell, its standard normal deviate (the reference's quantile of the cell's
+ * mt19937 draw; NaN where the draw is 0 or 1) and its level.  Returns the
+ * number of cells (nothing written when it exceeds capacity), or -1. */
+int hpglx_sgs_schedule(const int dims[3], const int radiuses[3], const unsigned char* informed,
+                       const unsigned char* simulation_mask, long seed, int* out_cells, double* out_deviates,
+                       int* out_level, int capacity);
+
 /* Property files straight to and from device buffers (the work of
  * hpgl_read_inc_file_float / _byte and hpgl_write_inc_file_float / _byte,
  * api.cpp:53-251): the text is parsed and printed on the GPU.  These

@@ -161,3 +161,56 @@ def test_sgs_lvm_constant_mean_field_is_simple_kriging(orc):
     assert not np.array_equal(lv[mask == 0], base[mask == 0])
     with pytest.raises(ValueError):
         orc.sgs(0, cov, (3, 3, 1), 8, shape, values, mask, 5, lvm_mean=np.zeros(values.size, np.float32))
+
+
+@pytest.mark.parametrize("shape,r,seed", [((12, 10, 4), (3, 3, 1), 7), ((9, 7, 5), (2, 4, 2), 123),
+                                          ((30, 1, 1), (4, 0, 0), 5), ((16, 16, 6), (5, 5, 1), 20240611)])
+def test_sgs_schedule_levels_keep_the_walk(orc, shape, r, seed):
+    """The product's host schedule (what the GPU simulation runs): the cells
+    and deviates of the oracle's walk, and levels under which running a
+    level's cells at once gives the one-by-one walk -- two cells inside each
+    other's +-r box never share a level and the earlier on the path comes
+    first; cells of one level keep path order."""
+    from geo_bsd.sgs import sgs_schedule
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(seed)
+    informed = (rng.random(n) < 0.15).astype(np.uint8)
+    simmask = (rng.random(n) < 0.8).astype(np.uint8)
+    cells, dev, lev = sgs_schedule(shape, r, informed, seed, simulation_mask=simmask)
+    path = orc.sgs_path(n, seed)
+    seen, expect = set(), []
+    for c in path:
+        c = int(c)
+        if informed[c] or simmask[c] != 1 or c in seen:
+            continue
+        seen.add(c)
+        expect.append(c)
+    pos = {c: i for i, c in enumerate(expect)}
+    assert sorted(cells.tolist()) == sorted(expect)
+    # deviates of the draws in path order
+    z = orc.gauss(orc.mt_uniform(seed, len(expect)), inverse=True)
+    assert np.array_equal(dev, z[[pos[int(c)] for c in cells]])
+    # levels ascend through the output, path order within a level
+    assert np.all(np.diff(lev) >= 0)
+    for l in np.unique(lev):
+        p = [pos[int(c)] for c in cells[lev == l]]
+        assert p == sorted(p)
+    nx, ny = shape[0], shape[1]
+    x, y, zc = cells % nx, (cells // nx) % ny, cells // (nx * ny)
+    near = ((np.abs(x[:, None] - x[None, :]) <= r[0]) & (np.abs(y[:, None] - y[None, :]) <= r[1]) &
+            (np.abs(zc[:, None] - zc[None, :]) <= r[2]))
+    np.fill_diagonal(near, False)
+    order = np.array([pos[int(c)] for c in cells])
+    earlier = order[:, None] < order[None, :]
+    # every pair inside a common box: the earlier cell's level is lower
+    i, j = np.nonzero(near & earlier)
+    assert np.all(lev[i] < lev[j])
+    assert lev.max() + 1 < len(cells) or len(cells) <= 1  # some cells do share levels
+
+
+def test_sgs_schedule_rejects_bad_input():
+    from geo_bsd.sgs import sgs_schedule
+    with pytest.raises(ValueError):
+        sgs_schedule((4, 4, 2), (1, 1, 1), np.zeros(32, np.uint8), -1)
+    with pytest.raises(ValueError):
+        sgs_schedule((4, 4, 2), (1, 1, 1), np.zeros(31, np.uint8), 1)
