@@ -409,7 +409,7 @@ def main():
     ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
     ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io", "variogram", "sgs"],
                     help="ok (default): the headline OK 100x100x50 workload; others: see the module docstring")
-    ap.add_argument("--realizations", type=int, default=8,
+    ap.add_argument("--realizations", type=int, default=32,
                     help="sgs workload: also time this many realizations run together (1 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, one device-to-device copy of MIB MiB (a kernel of known traffic for "

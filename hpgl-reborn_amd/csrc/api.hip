@@ -638,12 +638,17 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
         err = "host to device copy failed";
         return -1;
     }
-    for (int r = 0; r < nreal; ++r)
-        if (hipMemcpyAsync(d_v + (long long)r * n, v.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(d_m + (long long)r * n, m.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+    // the data once from the host, then on the device into every other slice
+    for (int r = 0; r < nreal; ++r) {
+        const void* sv = r == 0 ? (const void*)v.data() : (const void*)d_v;
+        const void* sm = r == 0 ? (const void*)m.data() : (const void*)d_m;
+        const hipMemcpyKind kind = r == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        if (hipMemcpyAsync(d_v + (long long)r * n, sv, (size_t)n * 4, kind, st) != hipSuccess ||
+            hipMemcpyAsync(d_m + (long long)r * n, sm, (size_t)n, kind, st) != hipSuccess) {
             err = "host to device copy failed";
             return -1;
         }
+    }
     // sgs_run synchronises st before returning, so the host arrays above
     // outlive their copies.
     return sgs_run(*plan, L.ws, p->m_kriging_kind, dims, d_v, d_m, d_lvm, sched, st, err);
