@@ -47,7 +47,7 @@ def build(verbose=True):
         objs.append(o)
     # the HIP translation units compile in parallel
     procs = []
-    units = ["krige.hip", "loo.hip", "api.hip", "inc_io.hip", "variogram.hip", "sgs.hip", "loo_m0.hip",
+    units = ["krige.hip", "solve_wide.hip", "loo.hip", "api.hip", "inc_io.hip", "variogram.hip", "sgs.hip", "loo_m0.hip",
              "loo_m1.hip", "loo_m2.hip"]
     units += [f"solve_n{n}_k{k}.hip" for n in (32, 24, 16, 8) for k in (0, 1, 2)]
     for src in units:

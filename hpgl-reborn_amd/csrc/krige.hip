@@ -902,10 +902,6 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         err = "grid volume exceeds INT_MAX";
         return -1;
     }
-    if (h.kmax > 32) {
-        err = "max_neighbours above 32 is not supported by the GPU solver";
-        return -1;
-    }
     if (c.kind == KIND_MIK && (c.nind < 1 || c.nind > 16)) {
         err = "median IK needs between 1 and 16 thresholds";
         return -1;
@@ -935,7 +931,13 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     g.cz = (int)(c.nz / (double)g.rz + 2);
     const long long P = (long long)g.px * g.py * g.pz;
     const int ncl = g.cx * g.cy * g.cz;
-    const int K = h.kmax;
+    // A cell has at most n neighbours (a caller's neighbour buffer keeps the
+    // stride it was given); above 32 the wide solver runs.
+    const int K = (int)(c.out_nboff ? (long long)h.kmax : std::min<long long>(h.kmax, n));
+    if (K > 65535) {
+        err = "max_neighbours above 65535 is not supported";
+        return -1;
+    }
     const int T = plan.tsize;
 
     auto* pmask = (unsigned char*)ws.get("pmask", (size_t)P, err);
@@ -1158,7 +1160,9 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
     const int nelem = (plan.otab && c.solve_variant == 0) ? (int)h.otab.size() : 0;
-    if (K <= 8)
+    if (K > 32) {
+        if (dispatch_solve_wide(so, c.kind, ws, ev_t + 2, st, err) != 0) return -1;
+    } else if (K <= 8)
         dispatch_solve_8(so, c.kind, mixed, n, nelem, ev_t + 2, st);
     else if (K <= 16)
         dispatch_solve_16(so, c.kind, mixed, n, nelem, ev_t + 2, st);

@@ -4,7 +4,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "krige_kernels.h"
+#include "krige_launch.h"
 
 namespace hpglx {
 
@@ -21,6 +24,11 @@ void dispatch_solve_24(const SolveArgs& sa, int kind, bool mixed, long long n, i
                        hipStream_t st);
 void dispatch_solve_32(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
                        hipStream_t st);
+
+// Systems above 32 neighbours (sa.kmax > 32): one workgroup per cell, FP64,
+// every cell of the grid; scratch from ws.  Returns 0, or -1 with err.
+int dispatch_solve_wide(const SolveArgs& sa, int kind, Workspace& ws, const hipEvent_t* ev, hipStream_t st,
+                        std::string& err);
 
 // One level of sequential Gaussian simulation in one launch: a wave per
 // listed cell runs the template search over the padded mask, the FP64 solve

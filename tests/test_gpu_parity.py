@@ -458,3 +458,78 @@ def test_geo_indicator_kriging_matches_oracle(orc):
     k = orc.krige(2, cov, c0["radiuses"], c0["k"], wl.shape, wl.mask, indicators=d2[None], means=[0.5])
     cls = (k["prob"][0].astype(np.float32) > 0.5).astype(np.uint8)
     assert np.array_equal(out2.data.reshape(-1, order="F"), cls)
+
+
+def _wide_case(seed=1):
+    """The reference's edge case (test_edge_cases.py:828-851): 10x10x5 grid,
+    every tenth cell uninformed, radiuses spanning the grid, max_neighbours
+    far above the informed count."""
+    rng = np.random.default_rng(seed)
+    shape = (10, 10, 5)
+    n = 500
+    values = (rng.random(n) * 100).astype(np.float32)
+    mask = np.ones(n, np.uint8)
+    mask[::10] = 0
+    return shape, values, mask
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("k", [33, 64, 200, 1000])
+def test_wide_systems_match_oracle(orc, kind, k):
+    """Above 32 neighbours the wide solver (one workgroup per cell, FP64)
+    runs: its neighbour sets, statuses and estimates equal the oracle's.
+    k = 200 and 1000 exceed the LDS working set (the global scratch form)."""
+    import torch
+    from geo_bsd import CovarianceModel
+    from geo_bsd import device as D
+    shape, values, mask = _wide_case()
+    cov = orc.make_cov(0, (20.0, 20.0, 10.0), sill=1.0, nugget=0.1)
+    o = orc.krige(kind, cov, (20, 20, 10), k, shape, mask, values=values, means=[50.0], auto_mean=False)
+    model = CovarianceModel(0, (20.0, 20.0, 10.0), (0.0, 0.0, 0.0), 1.0, 0.1)
+    dev = torch.device("cuda:0")
+    print(f"[gpu] wide kind={kind} k={k}", flush=True)
+    out = D.krige(kind, shape, (20, 20, 10), k, model, torch.from_numpy(mask).to(dev),
+                  values=torch.from_numpy(values).to(dev), mean=50.0 if kind == 1 else None, est64=True,
+                  status=True, neighbours=True)
+    torch.cuda.synchronize()
+    g = {key: t.cpu().numpy() for key, t in out.items()}
+    searched = o["status"] != orc.ST_INFORMED
+    assert np.array_equal(g["count"][searched], o["count"][searched])
+    assert g["count"][searched].max() == min(k, 450)
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    assert np.array_equal(g["mask"], o["mask"])
+    kr = o["status"] == orc.ST_KRIGED
+    # larger systems: FP64 agreement scaled by their conditioning
+    assert np.allclose(g["est64"][kr], o["est64"][kr], rtol=1e-7, atol=1e-7)
+    assert np.all(np.abs(g["value"] - o["value"]) <= 1e-4 * np.maximum(1.0, np.abs(o["value"])))
+    assert np.allclose(g["variance"][kr], o["var"][kr], rtol=1e-7, atol=1e-9)
+
+
+def test_wide_median_ik_matches_oracle(orc):
+    """Median IK with 40 neighbours and 3 thresholds through the wide
+    solver."""
+    wl = _wl("mik", 0.1)
+    wl.k = 40
+    o = _oracle(orc, wl)
+    g = _device(wl)
+    kr = o["status"] == orc.ST_KRIGED
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    assert np.allclose(g["est64"][:, kr], o["est64"][:, kr], rtol=0, atol=1e-8)
+    assert np.allclose(g["prob"][:, kr], o["prob"][:, kr].astype(np.float32), rtol=1e-5, atol=1e-5)
+
+
+def test_geo_ordinary_kriging_max_neighbours_1000():
+    """geo_bsd.ordinary_kriging with the reference edge test's arguments
+    (test_edge_cases.py:828-851) returns the grid's shape with every
+    uninformed cell estimated."""
+    import geo_bsd
+    shape, values, mask = _wide_case()
+    prop = geo_bsd.ContProperty(values, mask)
+    model = geo_bsd.CovarianceModel(type=geo_bsd.covariance.spherical, ranges=(20.0, 20.0, 10.0),
+                                    angles=(0.0, 0.0, 0.0), sill=1.0, nugget=0.1)
+    grid = geo_bsd.SugarboxGrid(x=10, y=10, z=5)
+    result = geo_bsd.ordinary_kriging(prop=prop, grid=grid, radiuses=(20, 20, 10), max_neighbours=1000,
+                                      cov_model=model)
+    result.fix_shape(grid)
+    assert result.data.shape == (10, 10, 5)
+    assert result.mask.all()
