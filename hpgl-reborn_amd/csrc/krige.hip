@@ -902,6 +902,12 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         err = "grid volume exceeds INT_MAX";
         return -1;
     }
+    if (h.r[0] < 1 || h.r[1] < 1 || h.r[2] < 1) {
+        // the reference's clusterizer divides the grid by them
+        // (clusterizer.cpp:87-89)
+        err = "search radiuses must be >= 1 for kriging";
+        return -1;
+    }
     if (c.kind == KIND_MIK && (c.nind < 1 || c.nind > 16)) {
         err = "median IK needs between 1 and 16 thresholds";
         return -1;

@@ -102,8 +102,8 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
         return false;
     }
     for (int i = 0; i < 3; ++i) {
-        if (radiuses[i] < 1) {
-            err = "search radiuses must be >= 1 (the reference divides by them)";
+        if (radiuses[i] < 0) {
+            err = "search radiuses must be >= 0";
             return false;
         }
         if (radiuses[i] > 255) {

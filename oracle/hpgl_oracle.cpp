@@ -260,8 +260,11 @@ struct Setup {
     }
 };
 
+// clusters = false: the template walk alone (sequential simulation), where a
+// radius of 0 is allowed; kriging clusters the grid by the radiuses
+// (clusterizer.cpp:87-89), which must then be at least 1.
 bool setup(Setup& s, int kind, const CovParams* cp, const int radiuses[3], int kmax, int nx, int ny,
-           int nz, const unsigned char* mask) {
+           int nz, const unsigned char* mask, bool clusters = true) {
     s.kind = (Kind)kind;
     s.cov.kind = s.kind;
     s.cov.m.p = *cp;
@@ -269,12 +272,13 @@ bool setup(Setup& s, int kind, const CovParams* cp, const int radiuses[3], int k
     for (int i = 0; i < 3; ++i) {
         s.r[i] = radiuses[i];
         s.cov.r[i] = radiuses[i];
-        if (radiuses[i] < 1) return false;
+        if (radiuses[i] < (clusters ? 1 : 0)) return false;
     }
     s.g = Grid{nx, ny, nz};
     s.kmax = kmax;
     s.mask = mask;
     s.tmpl = build_template(s.cov, s.r);
+    if (!clusters) return true;
     Clusters& c = s.cl;
     for (int i = 0; i < 3; ++i) c.r[i] = s.r[i];
     c.mx = (int)(nx / (double)s.r[0] + 2);
@@ -1092,7 +1096,7 @@ long long orc_sgs(int kind, const orc_cov_t* cp, const int radiuses[3], int kmax
                   const float* lvm_mean) {
     if (kmax < 1 || (kind != OK && kind != SK) || (lvm_mean && kind != SK)) return -1;
     Setup s;
-    if (!setup(s, SK, (const CovParams*)cp, radiuses, kmax, nx, ny, nz, mask)) return -1;
+    if (!setup(s, SK, (const CovParams*)cp, radiuses, kmax, nx, ny, nz, mask, false)) return -1;
     s.kind = (Kind)kind;  // algebra of the kind, covariances clipped to the box
     const long long N = s.g.n();
     std::vector<float> lm;

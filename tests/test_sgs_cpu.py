@@ -214,3 +214,19 @@ def test_sgs_schedule_rejects_bad_input():
         sgs_schedule((4, 4, 2), (1, 1, 1), np.zeros(32, np.uint8), -1)
     with pytest.raises(ValueError):
         sgs_schedule((4, 4, 2), (1, 1, 1), np.zeros(31, np.uint8), 1)
+
+
+def test_sgs_zero_radius_is_the_path_of_draws(orc):
+    """Search radiuses (0, 0, 0): the template is the cell itself, never
+    informed when it is simulated, so every simulated cell takes the deviate
+    of its own draw, as with a pure nugget; kriging still rejects a zero
+    radius (the reference's clusters divide by it)."""
+    shape, values, mask = _sgs_case(seed=4)
+    seed = 99
+    cov = orc.make_cov(0, (1.0, 1.0, 1.0), sill=1.0, nugget=0.0)
+    v, m, order = orc.sgs(1, cov, (0, 0, 0), 1, shape, values, mask, seed)
+    u = orc.mt_uniform(seed, order.size)
+    assert np.array_equal(v[order], orc.gauss(u, inverse=True).astype(np.float32))
+    assert m.all()
+    with pytest.raises(ValueError):
+        orc.krige(0, cov, (0, 0, 0), 4, shape, mask, values=values)
