@@ -19,7 +19,8 @@ import numpy
 
 from . import validation
 from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_IK_PARAMS, _HPGL_MEDIAN_IK_PARAMS,
-                        _HPGL_OK_PARAMS, _HPGL_SHAPE, _HPGL_SK_PARAMS, _HPGL_COV_PARAMS, _hpgl_so,
+                        _HPGL_FLOAT_ARRAY, _HPGL_OK_PARAMS, _HPGL_SHAPE, _HPGL_SK_PARAMS, _HPGL_COV_PARAMS,
+                        _HPGL_UBYTE_ARRAY, _hpgl_so,
                         hpgl_output_handler, hpgl_progress_handler, last_error)
 from .validation import (CriticalValidationError, GridValidator, ParameterValidator, PathValidator, ValidationError,
                          ValidationWarning)
@@ -125,6 +126,18 @@ class CovarianceModel:
         ParameterValidator.validate_covariance_parameters(sill, nugget, ranges, angles)
 
 
+def _empty_clone(prop):
+    """geo.py:360-370: a property of the same kind and shape, zero data,
+    nothing informed (the indicator count kept)."""
+    data2 = prop.data.copy("F")
+    data2.fill(0)
+    mask2 = prop.mask.copy("F")
+    mask2.fill(0)
+    if isinstance(prop, IndProperty):
+        return IndProperty(data2, mask2, prop.indicator_count)
+    return ContProperty(data2, mask2)
+
+
 def _clone_prop(prop):
     """geo.py:372-380: F-order copies of data and mask."""
     data2 = prop.data.copy("F")
@@ -169,6 +182,41 @@ def _shape(dims):
     sh.strides[1] = int(dims[0])
     sh.strides[2] = int(dims[0]) * int(dims[1])
     return sh
+
+
+def _require_cont_data(data):
+    """geo.py:382-385: float32, Fortran order (None passes through)."""
+    return None if data is None else numpy.require(data, dtype="float32", requirements="F")
+
+
+def _requite_ind_data(data):
+    """geo.py:387-390 (the reference's spelling): uint8, Fortran order."""
+    return None if data is None else numpy.require(data, dtype="uint8", requirements="F")
+
+
+def _array_shape(array, grid):
+    if grid is not None:
+        return _shape((grid.x, grid.y, grid.z))
+    dims = tuple(array.shape) + (1,) * (3 - array.ndim)
+    return _shape(dims)
+
+
+def _create_hpgl_float_array(array, grid):
+    """geo.py:203-214: an hpgl_float_array_t over a Fortran-order float32
+    array (the array is kept alive by the struct)."""
+    checkFWA(array)
+    res = _HPGL_FLOAT_ARRAY(data=array.ctypes.data_as(C.POINTER(C.c_float)), shape=_array_shape(array, grid))
+    res._array_ref = array
+    return res
+
+
+def _create_hpgl_ubyte_array(array, grid):
+    """geo.py:191-201: an hpgl_ubyte_array_t over a Fortran-order uint8
+    array."""
+    checkFWA(array)
+    res = _HPGL_UBYTE_ARRAY(data=array.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=_array_shape(array, grid))
+    res._array_ref = array
+    return res
 
 
 def _grid_arrays(prop, grid):

@@ -5,16 +5,10 @@ import ctypes as C
 import numpy
 
 from .cdf import CdfData
-from .geo import ContProperty, CovarianceModel, _check_native, _clone_prop, _shape, accepts_tuple
+from .geo import ContProperty, CovarianceModel, _check_native, _clone_prop, _empty_clone, _shape, accepts_tuple
 from .hpgl_wrap import (_HPGL_CONT_MASKED_ARRAY, _HPGL_FLOAT_ARRAY, _HPGL_KRIGING_KIND, _HPGL_SGS_PARAMS,
                         _HPGL_UBYTE_ARRAY, _hpgl_so, hpgl_non_parametric_cdf_t)
 from .validation import GridValidator, ParameterValidator
-
-
-def _empty_clone(prop):
-    """geo.py _empty_clone: zero data, nothing informed."""
-    return ContProperty(numpy.zeros(prop.data.shape, dtype="float32", order="F"),
-                        numpy.zeros(prop.mask.shape, dtype="uint8", order="F"))
 
 
 def normed_cov_model(cov_model):

@@ -108,3 +108,33 @@ def test_thread_num_roundtrip():
     import geo_bsd
     geo_bsd.set_thread_num(3)
     assert geo_bsd.get_thread_num() == 3
+
+
+def test_empty_clone_keeps_kind_and_indicator_count():
+    """geo.py:360-370 (imported from geo_bsd.geo by the reference's tests)."""
+    import numpy as np
+    from geo_bsd.geo import ContProperty, IndProperty, _empty_clone
+    ind = IndProperty(np.array([0, 2, 1, 2], np.uint8), np.ones(4, np.uint8), 5)
+    c = _empty_clone(ind)
+    assert isinstance(c, IndProperty) and c.indicator_count == 5
+    assert not c.data.any() and not c.mask.any()
+    cont = ContProperty(np.array([1.5, -2.0], np.float32), np.array([1, 0], np.uint8))
+    c = _empty_clone(cont)
+    assert isinstance(c, ContProperty) and c.data.dtype == np.float32
+    assert not c.data.any() and not c.mask.any() and cont.data[0] == 1.5
+
+
+def test_reference_array_helpers():
+    """geo.py:191-214, 382-390 helpers the reference's tests import."""
+    import numpy as np
+    from geo_bsd.geo import (SugarboxGrid, _create_hpgl_float_array, _create_hpgl_ubyte_array, _require_cont_data,
+                             _requite_ind_data)
+    a = _require_cont_data(np.arange(24, dtype=np.float64).reshape(2, 3, 4))
+    assert a.dtype == np.float32 and a.flags["F"]
+    b = _requite_ind_data(np.ones((2, 3, 4), np.int64))
+    assert b.dtype == np.uint8 and b.flags["F"]
+    assert _require_cont_data(None) is None and _requite_ind_data(None) is None
+    fa = _create_hpgl_float_array(a, None)
+    assert list(fa.shape.data) == [2, 3, 4] and fa.data[5] == a.reshape(-1, order="F")[5]
+    ua = _create_hpgl_ubyte_array(b, SugarboxGrid(2, 3, 4))
+    assert list(ua.shape.data) == [2, 3, 4] and ua.data[0] == 1
