@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -616,9 +617,14 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
     std::vector<char> ok((size_t)nreal, 0);
     const int workers = std::max(1, std::min(nreal, 16));
     auto work = [&](int w) {
-        for (int r = w; r < nreal; r += workers)
-            ok[(size_t)r] = sgs_schedule(dims, p->m_radiuses, m.data(), simmask, seeds[r], sched[(size_t)r],
-                                         errs[(size_t)r]);
+        for (int r = w; r < nreal; r += workers) {
+            try {
+                ok[(size_t)r] = sgs_schedule(dims, p->m_radiuses, m.data(), simmask, seeds[r], sched[(size_t)r],
+                                             errs[(size_t)r]);
+            } catch (const std::exception& e) {  // out of host memory: reported, not fatal
+                errs[(size_t)r] = std::string("sequential simulation schedule: ") + e.what();
+            }
+        }
     };
     if (workers == 1) {
         work(0);
