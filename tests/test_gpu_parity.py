@@ -533,3 +533,39 @@ def test_geo_ordinary_kriging_max_neighbours_1000():
     result.fix_shape(grid)
     assert result.data.shape == (10, 10, 5)
     assert result.mask.all()
+
+
+def test_wide_lvm_and_categories_match_oracle(orc):
+    """The wide solver's locally varying mean and indicator-category forms:
+    LVM kriging and 4-category indicator kriging with max_neighbours 40."""
+    import torch
+    from geo_bsd import device as D, synthetic as S
+    dev = torch.device("cuda:0")
+    wl = S.lvm_c1(0.6)
+    wl.k = 40
+    o = orc.krige(1, _cov(orc, wl), wl.radiuses, wl.k, wl.shape, wl.mask, values=wl.values, lvm_mean=wl.lvm_mean)
+    print(f"[gpu] wide lvm {wl.shape} k=40", flush=True)
+    out = D.lvm_kriging(torch.from_numpy(wl.values).to(dev), torch.from_numpy(wl.mask).to(dev),
+                        torch.from_numpy(wl.lvm_mean).to(dev), wl.shape, wl.radiuses, wl.k, _model(wl), est64=True,
+                        status=True, neighbours=True)
+    torch.cuda.synchronize()
+    r = {k: t.cpu().numpy() for k, t in out.items()}
+    assert np.array_equal(r["status"], o["status"].astype(np.uint8))
+    un = wl.mask == 0
+    assert np.array_equal(r["count"][un], o["count"][un])
+    assert r["count"][un].max() > 32
+    kr = o["status"] == orc.ST_KRIGED
+    assert np.all(np.abs(r["value"] - o["value"]) <= 1e-5 * np.maximum(1.0, np.abs(o["value"])))
+    assert np.all(np.abs(r["est64"][kr] - o["est64"][kr]) <= 1e-8)
+    wl = S.ik_c3(0.12)
+    for c in wl.categories:
+        c["k"] = 40
+    o = _ik_oracle(orc, wl)
+    print(f"[gpu] wide ik {wl.shape} k=40", flush=True)
+    out = D.indicator_kriging(torch.from_numpy(wl.values).to(dev), torch.from_numpy(wl.mask).to(dev), wl.shape,
+                              _ik_categories(wl), wl.marginal)
+    torch.cuda.synchronize()
+    r = {k: t.cpu().numpy() for k, t in out.items()}
+    assert r["mask"].all()
+    assert np.all(np.abs(r["prob"] - o["prob"]) <= 2e-6)
+    _check_categories(r["category"], o)
