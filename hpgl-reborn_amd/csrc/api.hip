@@ -620,7 +620,7 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
         for (int r = w; r < nreal; r += workers) {
             try {
                 ok[(size_t)r] = sgs_schedule(dims, p->m_radiuses, m.data(), simmask, seeds[r], sched[(size_t)r],
-                                             errs[(size_t)r]);
+                                             errs[(size_t)r], nreal == 1);
             } catch (const std::exception& e) {  // out of host memory: reported, not fatal
                 errs[(size_t)r] = std::string("sequential simulation schedule: ") + e.what();
             }
@@ -1397,7 +1397,7 @@ int hpglx_sgs_schedule(const int dims[3], const int radiuses[3], const unsigned 
         return fail(L, "hpglx_sgs_schedule: negative radius");
     SgsSchedule s;
     std::string err;
-    if (!sgs_schedule(dims, radiuses, informed, simulation_mask, seed, s, err)) return fail(L, err);
+    if (!sgs_schedule(dims, radiuses, informed, simulation_mask, seed, s, err, true)) return fail(L, err);
     const int m = (int)s.cells.size();
     if (m > capacity) return m;
     for (size_t l = 0; l + 1 < s.level_off.size(); ++l)

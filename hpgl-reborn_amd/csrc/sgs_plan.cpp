@@ -11,6 +11,7 @@
 #include <cmath>
 #include <limits>
 #include <random>
+#include <thread>
 
 namespace hpglx {
 namespace {
@@ -119,15 +120,35 @@ float table_value(const float* xv, const float* pv, long long m, float p) {
 }  // namespace
 
 bool sgs_schedule(const int dims[3], const int r[3], const unsigned char* informed, const unsigned char* simmask,
-                  long seed, SgsSchedule& s, std::string& err) {
+                  long seed, SgsSchedule& s, std::string& err, bool draw_thread) {
     const long long nx = dims[0], ny = dims[1], nz = dims[2];
     const long long n = nx * ny * nz;
     Walk walk;
     if (!make_walk(n, (int)seed, walk, err)) return false;
-    // The uniform draws: one per simulated cell, in path order.
-    std::mt19937 gen;
-    gen.seed(static_cast<std::mt19937::result_type>(seed));
-    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    // The uniform draws, one per simulated cell in path order, and their
+    // deviates: independent of the walk, so optionally on a thread of their
+    // own.  Every cell the walk can simulate is counted; a walk that repeats
+    // nodes simulates fewer and leaves the last deviates unused.
+    long long eligible = 0;
+    for (long long i = 0; i < n; ++i) eligible += !informed[i] && (!simmask || simmask[i] == 1);
+    std::vector<double> dev((size_t)eligible);
+    auto draws = [&dev, seed]() {
+        std::mt19937 gen;
+        gen.seed(static_cast<std::mt19937::result_type>(seed));
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        for (double& d : dev) d = standard_deviate(uni(gen));
+    };
+    std::thread drawer;
+    struct JoinOnExit {  // also when an allocation below throws
+        std::thread& t;
+        ~JoinOnExit() {
+            if (t.joinable()) t.join();
+        }
+    } join_on_exit{drawer};
+    if (draw_thread && eligible > 0)
+        drawer = std::thread(draws);
+    else
+        draws();
 
     // Levels: a cell's level is one more than the highest level among the
     // cells simulated before it inside its +-r box.  The maximum is taken
@@ -146,10 +167,8 @@ bool sgs_schedule(const int dims[3], const int r[3], const unsigned char* inform
     std::vector<int> win((size_t)(mx * my * mz), 0);
     std::vector<int> lv;
     std::vector<int> order;
-    std::vector<double> dev;
-    lv.reserve((size_t)n);
-    order.reserve((size_t)n);
-    dev.reserve((size_t)n);
+    lv.reserve((size_t)eligible);
+    order.reserve((size_t)eligible);
     int levels = 0;
     // Where every prime up to size / 3 divides the size (2, 4, 6, 8, 12, ...)
     // the reference's walk repeats nodes and misses others; a repeat finds
@@ -172,9 +191,9 @@ bool sgs_schedule(const int dims[3], const int r[3], const unsigned char* inform
         for (long long a = std::max(0LL, a0 - hx); a <= std::min(mx - 1, a0 + hx); ++a) row[a] = std::max(row[a], v);
         lv.push_back(m);
         order.push_back((int)node);
-        dev.push_back(standard_deviate(uni(gen)));
         levels = std::max(levels, m + 1);
     }
+    if (drawer.joinable()) drawer.join();
     // Group by level, path order kept within each (a counting sort).
     s.level_off.assign((size_t)levels + 1, 0);
     for (int l : lv) s.level_off[(size_t)l + 1]++;

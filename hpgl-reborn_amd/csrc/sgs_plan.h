@@ -27,9 +27,10 @@ struct SgsSchedule {
 // simulated only where it is exactly 1) are host arrays.  Two cells whose
 // offset lies inside the +-r box never share a level, and the earlier one on
 // the path comes first, so running a level's cells in parallel gives the
-// results of the one-by-one walk.
+// results of the one-by-one walk.  draw_thread: compute the draws and their
+// deviates on a second thread while the walk assigns levels.
 bool sgs_schedule(const int dims[3], const int r[3], const unsigned char* informed, const unsigned char* simmask,
-                  long seed, SgsSchedule& s, std::string& err);
+                  long seed, SgsSchedule& s, std::string& err, bool draw_thread = false);
 
 // Normal-score transforms of the informed values in place (cdf_utils.h:14-25
 // with non_parametric_cdf_2_t and gaussian_cdf_t): forward before the
