@@ -431,8 +431,8 @@ def set_progress_handler(handler, param):
         _hpgl_so.hpgl_set_progress_handler(h, param)
 
 
-def set_thread_num(n):
-    _hpgl_so.hpgl_set_thread_num(int(n))
+def set_thread_num(num):
+    _hpgl_so.hpgl_set_thread_num(int(num))
 
 
 def get_thread_num():
@@ -566,6 +566,15 @@ def write_property(prop, filename, prop_name, undefined_value, indicator_values=
                                                len(indicator_values))
     if rc != 0:
         raise _native_error("write_property failed: ")
+
+
+def get_gslib_property(prop_dict, prop_name, undefined_value):
+    """geo.py:937-945: (values, informed) for one column of a dict of GSLIB
+    columns; a cell is informed where its value differs from
+    undefined_value."""
+    prop = prop_dict[prop_name]
+    informed = (numpy.asarray(prop) != undefined_value).astype(numpy.uint8)
+    return (prop, informed)
 
 
 @accepts_tuple("prop", 0)

@@ -180,3 +180,14 @@ def test_writer_bytes_match_reference(ref, tmp_path, gslib):
     head = b"HPGL saved GSLIB file\n1\nFACIES\n" if gslib else b"FACIES\n"
     tail = b"" if gslib else b"/\n"
     assert path.read_bytes() == head + body + tail
+
+
+def test_get_gslib_property_marks_undefined():
+    """geo.py:937-945: informed where the value differs from the undefined
+    marker; the column itself is returned as given."""
+    import numpy as np
+    from geo_bsd.geo import get_gslib_property
+    col = np.array([1.5, -99.0, 0.0, -99.0, 7.0], np.float32)
+    vals, informed = get_gslib_property({"poro": col, "perm": col * 2}, "poro", -99.0)
+    assert vals is col
+    assert informed.dtype == np.uint8 and informed.tolist() == [1, 0, 1, 0, 1]
