@@ -45,7 +45,8 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
     ParameterValidator.validate_seed(seed)
     ParameterValidator.validate_min_neighbors(min_neighbours, max_neighbours)
     if kriging_type not in ("sk", "ok"):
-        raise ValueError('kriging_type must be "sk" or "ok"')
+        # the reference's lookup of the kind (sgs.py:122) raises KeyError
+        raise KeyError(kriging_type)
     prop.fix_shape(grid)
     cov_model = normed_cov_model(cov_model)
     out_prop = _clone_prop(prop) if use_harddata else _empty_clone(prop)

@@ -230,3 +230,13 @@ def test_sgs_zero_radius_is_the_path_of_draws(orc):
     assert m.all()
     with pytest.raises(ValueError):
         orc.krige(0, cov, (0, 0, 0), 4, shape, mask, values=values)
+
+
+def test_sgs_invalid_kriging_type_raises_key_error():
+    """sgs.py:122 looks the kind up in a dict: an unknown kriging_type is a
+    KeyError, raised before anything runs."""
+    from geo_bsd import ContProperty, CovarianceModel, SugarboxGrid, sgs_simulation
+    prop = ContProperty(np.zeros(8, np.float32), np.zeros(8, np.uint8))
+    model = CovarianceModel(0, (2.0, 2.0, 1.0), (0.0, 0.0, 0.0), 1.0, 0.0)
+    with pytest.raises(KeyError):
+        sgs_simulation(prop, SugarboxGrid(2, 2, 2), None, (1, 1, 1), 4, model, 1, kriging_type="invalid")
