@@ -7,6 +7,9 @@
 // gaussian_distribution.h:56-89 (the draw).
 #include "sgs.h"
 
+#include <memory>
+#include <thread>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -83,12 +86,21 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
         m += (long long)s.cells.size();
     }
     if (m == 0) return 0;
-    std::vector<int> ecell((size_t)m), ereal((size_t)m), step_off((size_t)steps + 1, 0);
-    std::vector<double> ez((size_t)m);
-    {
-        size_t e = 0;
-        for (int l = 0; l < steps; ++l) {
-            step_off[(size_t)l] = (int)e;
+    // Entries of step l: level l of realization 0, then of 1, ...  Sizes and
+    // offsets first, then the copies on up to 16 threads over disjoint step
+    // ranges (the buffers are left uninitialised: each thread touches its own).
+    std::vector<int> step_off((size_t)steps + 1, 0);
+    for (int l = 0; l < steps; ++l) {
+        long long c = 0;
+        for (const SgsSchedule& s : sched)
+            if (l + 1 < (int)s.level_off.size()) c += s.level_off[(size_t)l + 1] - s.level_off[(size_t)l];
+        step_off[(size_t)l + 1] = step_off[(size_t)l] + (int)c;
+    }
+    std::unique_ptr<int[]> ecell(new int[(size_t)m]), ereal(new int[(size_t)m]);
+    std::unique_ptr<double[]> ez(new double[(size_t)m]);
+    auto fill = [&](int l0, int l1) {
+        for (int l = l0; l < l1; ++l) {
+            size_t e = (size_t)step_off[(size_t)l];
             for (int r = 0; r < R; ++r) {
                 const SgsSchedule& s = sched[(size_t)r];
                 if (l + 1 >= (int)s.level_off.size()) continue;
@@ -99,7 +111,22 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
                 }
             }
         }
-        step_off[(size_t)steps] = (int)e;
+    };
+    const int nthr = (int)std::min<long long>(16, std::max<long long>(1, m / 200000));
+    if (nthr == 1) {
+        fill(0, steps);
+    } else {
+        // step ranges of about equal entry counts
+        std::vector<std::thread> pool;
+        int l0 = 0;
+        for (int t = 0; t < nthr; ++t) {
+            const long long target = m * (t + 1) / nthr;
+            int l1 = l0;
+            while (l1 < steps && (t == nthr - 1 || step_off[(size_t)l1] < target)) ++l1;
+            pool.emplace_back(fill, l0, l1);
+            l0 = l1;
+        }
+        for (auto& th : pool) th.join();
     }
     auto* pmask = (unsigned char*)ws.get("sgs_pmask", (size_t)(R * P), err);
     auto* nboff = (int*)ws.get("sgs_nboff", (size_t)R * n * K * 4, err);
@@ -112,10 +139,11 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     auto* reals = (int*)ws.get("sgs_reals", (size_t)m * 4, err);
     auto* zdev = (double*)ws.get("sgs_z", (size_t)m * 8, err);
     if (!err.empty()) return -1;
-    if (hipMemcpyAsync(cells, ecell.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(reals, ereal.data(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(zdev, ez.data(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+    if (hipMemcpyAsync(cells, ecell.get(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(reals, ereal.get(), (size_t)m * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(zdev, ez.get(), (size_t)m * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
         err = "host to device copy failed";
+        (void)hipStreamSynchronize(st);
         return -1;
     }
     for (int r = 0; r < R; ++r)
@@ -176,6 +204,7 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     }
     if (hipGetLastError() != hipSuccess) {
         err = "sequential simulation kernel launch failed";
+        (void)hipStreamSynchronize(st);  // the copies above read host arrays freed on return
         return -1;
     }
     // the host vectors above are read by the copies
