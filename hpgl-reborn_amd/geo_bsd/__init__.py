@@ -14,3 +14,4 @@ from .geo import *  # noqa: F401,F403
 from . import validation  # noqa: F401
 from .cdf import CdfData, calc_cdf  # noqa: F401
 from .sgs import sgs_simulation  # noqa: F401
+from . import cvariogram  # noqa: F401  (geo_bsd.cvariogram after "import geo_bsd", as the reference)
