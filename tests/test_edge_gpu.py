@@ -134,3 +134,17 @@ def test_sgs_simulation_mask_selecting_nothing_and_everything(orc):
                             simmask=np.ones(500, np.uint8), cdf=(cdf3.values, cdf3.probs))
     assert order.size == 500
     assert np.abs(got.astype(np.float64) - ov).max() <= 1e-3
+
+
+def test_kriging_rejects_zero_radius():
+    """Kriging clusters the grid by the search radiuses
+    (clusterizer.cpp:87-89), so a zero radius is refused with an error,
+    where the reference would divide by zero; simulation accepts it (see
+    test_sgs_zero_radius_matches_oracle)."""
+    import geo_bsd
+    values = np.arange(125, dtype=np.float32)
+    mask = np.ones(125, np.uint8)
+    mask[::3] = 0
+    print("[gpu] edge kriging radius 0", flush=True)
+    with pytest.raises(RuntimeError, match="radiuses"):
+        _ok(geo_bsd.ContProperty(values, mask), geo_bsd.SugarboxGrid(5, 5, 5), (0, 2, 2), 4, _model((2.0, 2.0, 2.0)))
