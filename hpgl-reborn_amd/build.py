@@ -9,6 +9,11 @@
   reference's own g++-built code does.
 * The .hip sources are compiled with hipcc for gfx950 only, with packed FP32
   instructions disabled.
+
+    python hpgl-reborn_amd/build.py --incremental
+
+recompiles only the objects older than their source or than any header of
+csrc/ and include/.
 """
 import os
 import shutil
@@ -38,12 +43,24 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True):
+def _stale(src, obj, newest_header):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return os.path.getmtime(src) > t or newest_header > t
+
+
+def build(verbose=True, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    headers += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    newest_header = max(os.path.getmtime(h) for h in headers)
     objs = []
     for cpp in ("plan.cpp", "variogram_plan.cpp", "sgs_plan.cpp"):
         o = os.path.join(BUILD, cpp.replace(".cpp", ".o"))
-        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, cpp), "-o", o])
+        if not incremental or _stale(os.path.join(CSRC, cpp), o, newest_header):
+            _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, cpp), "-o", o])
         objs.append(o)
     # the HIP translation units compile in parallel
     procs = []
@@ -52,10 +69,12 @@ def build(verbose=True):
     units += [f"solve_n{n}_k{k}.hip" for n in (32, 24, 16, 8) for k in (0, 1, 2)]
     for src in units:
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        objs.append(o)
+        if incremental and not _stale(os.path.join(CSRC, src), o, newest_header):
+            continue
         cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o]
         print("+", " ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd)))
-        objs.append(o)
     failed = [src for src, pr in procs if pr.wait() != 0]
     if failed:
         raise RuntimeError("hipcc failed for " + ", ".join(failed))
@@ -66,5 +85,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv[1:])
     sys.exit(0)
