@@ -116,7 +116,29 @@ __global__ __launch_bounds__(256) void k_sk_weights(DevModel m, const float* cen
     }
 }
 
+// ------------------------------------------- scratch of hpglx_krige_noalloc
+// Every device buffer of an allocation-free call beyond the caller's inputs
+// and outputs (plan tables, padded mask, neighbour lists, cluster lists,
+// counters) is carved from this array.  Each GPU the library runs on holds
+// its own copy until the process ends.  In host code the name `scratch` is
+// not a device address: the address is read once per device with
+// hipGetSymbolAddress (NoAllocDevice::base) and used everywhere instead.
+constexpr size_t kNoAllocScratchBytes = (size_t)HPGLX_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+// Lanes of the full-table cluster search in that mode (its per-lane candidate
+// buffers are the largest scratch it would otherwise take).
+constexpr int kNoAllocClusterLanes = 512;
+constexpr size_t kNoAllocGuard = 64 << 10;
+
 namespace {
+
+// State of the allocation-free entry point on one device.
+struct NoAllocDevice {
+    char* base = nullptr;       // this device's copy of `scratch` (hipGetSymbolAddress)
+    Workspace ws;               // arena over the scratch after the resident plan's tables
+    std::string resident;       // key of the plan whose tables start the scratch ("" = none)
+    size_t plan_bytes = 0;
+};
 
 struct Library {
     std::mutex mu;
@@ -137,6 +159,12 @@ struct Library {
     std::vector<CallTiming> tpool;
     size_t tcalls = 0;
     hipEvent_t done = nullptr;   // completion of the last call's work
+    const int* last_counters = nullptr;  // device counters of the last grid call (fallbacks, cluster cells)
+    // hpglx_krige_noalloc: per-device scratch state and host-built plans
+    // (placed into the scratch on demand; keys include the device).
+    std::map<int, NoAllocDevice> noalloc;
+    std::map<std::string, std::unique_ptr<DevPlan>> na_plans;
+    std::map<std::string, unsigned long long> na_use;
     hipStream_t io_stream = nullptr;
     IncText text;                // pinned host copy of the last property file read or written
 };
@@ -256,17 +284,8 @@ bool read_timing(const CallTiming& t, float ms[2]) {
            hipEventElapsedTime(&ms[1], t.ev[2], t.ev[3]) == hipSuccess;
 }
 
-int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err,
-                const int* cats_host = nullptr) {
-    if (!q) {
-        err = "null request";
-        return -1;
-    }
-    if (!ensure_events(L, err)) return -1;
-    const CovParams c = to_cov(q->cov);
-    DevPlan* plan = get_plan(L, q->kind, c, q->radiuses, q->max_neighbours, err);
-    if (!plan) return -1;
-    GridCall call;
+// The launcher call a request describes (plan and workspace aside).
+bool request_call(const hpglx_grid_request_t* q, GridCall& call, const int* cats_host, std::string& err) {
     call.kind = q->kind;
     call.precision = q->precision;
     call.nx = q->dims[0];
@@ -301,15 +320,30 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     if (const char* sv = std::getenv("HPGLX_CLUSTER_CAP")) call.cluster_cap = std::atoi(sv);
     if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
         err = "values and mask are required";
-        return -1;
+        return false;
     }
     if (q->kind == KIND_MIK) {
         if (!q->marginal || (!q->ind && !(q->zfield && q->thresholds))) {
             err = "median IK needs indicator data (or a field and thresholds) and marginal probabilities";
-            return -1;
+            return false;
         }
         if (q->zfield) call.values = q->zfield;
     }
+    return true;
+}
+
+int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err,
+                const int* cats_host = nullptr) {
+    if (!q) {
+        err = "null request";
+        return -1;
+    }
+    if (!ensure_events(L, err)) return -1;
+    GridCall call;
+    if (!request_call(q, call, cats_host, err)) return -1;
+    const CovParams c = to_cov(q->cov);
+    DevPlan* plan = get_plan(L, q->kind, c, q->radiuses, q->max_neighbours, err);
+    if (!plan) return -1;
     // The previous call's work must be finished before its workspace is reused.
     if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) {
         err = "hipStreamWaitEvent failed";
@@ -322,6 +356,7 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
         err = "hipEventRecord failed";
         return -1;
     }
+    if (rc == 0) L.last_counters = (const int*)L.ws.bufs["counters"].first;
     return rc;
 }
 
@@ -1194,6 +1229,83 @@ int hpglx_krige(const hpglx_grid_request_t* req, void* stream) {
     return 0;
 }
 
+int hpglx_krige_noalloc(const hpglx_grid_request_t* req, void* stream) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!req) return fail(L, "null request");
+    const hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(L, "hipGetDevice failed");
+    NoAllocDevice& na = L.noalloc[dev];
+    if (!na.base) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(L, std::string("hipGetSymbolAddress of the allocation-free scratch failed: ") +
+                               hipGetErrorString(e));
+        na.base = (char*)p;
+    }
+    if (!ensure_events(L, err)) return fail(L, err);
+    GridCall call;
+    if (!request_call(req, call, nullptr, err)) return fail(L, err);
+    call.cluster_lanes = kNoAllocClusterLanes;
+    // The plan's host tables: built once per parameter set and device, kept
+    // (at most kMaxPlans) for the copies into the scratch.
+    const CovParams c = to_cov(req->cov);
+    const std::string key = plan_key(req->kind, c, req->radiuses, req->max_neighbours);
+    ++L.plan_clock;
+    DevPlan* plan = nullptr;
+    auto it = L.na_plans.find(key);
+    if (it != L.na_plans.end()) {
+        plan = it->second.get();
+    } else {
+        std::unique_ptr<DevPlan> p(new DevPlan());
+        if (!build_plan(p->h, (Kind)req->kind, c, req->radiuses, req->max_neighbours, err)) return fail(L, err);
+        if (L.na_plans.size() >= kMaxPlans) {
+            auto lru = L.na_use.begin();
+            for (auto u = L.na_use.begin(); u != L.na_use.end(); ++u)
+                if (u->second < lru->second) lru = u;
+            // its host tables may still feed a pending copy
+            if (L.done && hipEventSynchronize(L.done) != hipSuccess) return fail(L, "hipEventSynchronize failed");
+            for (auto& d : L.noalloc)
+                if (d.second.resident == lru->first) d.second.resident.clear();
+            L.na_plans.erase(lru->first);
+            L.na_use.erase(lru);
+        }
+        plan = p.get();
+        L.na_plans[key] = std::move(p);
+    }
+    L.na_use[key] = L.plan_clock;
+    // After the previous call's work (of any entry point): the scratch, its
+    // resident tables and its cleared halo are shared by every call.
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) return fail(L, "hipStreamWaitEvent failed");
+    int rc = 0;
+    if (na.resident != key) {
+        na.resident.clear();
+        size_t used = 0;
+        if (!plan->place(na.base, kNoAllocScratchBytes - kNoAllocGuard, used, st, err)) rc = -1;
+        if (rc == 0) {
+            na.resident = key;
+            na.plan_bytes = used;
+        }
+    }
+    if (rc == 0) {
+        // The last 64 KiB stay unused: a margin past the last buffer, like
+        // the page rounding of an allocation.
+        na.ws.arena = na.base + na.plan_bytes;
+        na.ws.arena_cap = kNoAllocScratchBytes - na.plan_bytes - kNoAllocGuard;
+        na.ws.reset_arena();
+        rc = launch_grid(*plan, na.ws, call, st, nullptr, err);
+    }
+    // Whatever was enqueued (table copies, kernels) orders the next call.
+    if (hipEventRecord(L.done, st) != hipSuccess) return fail(L, "hipEventRecord failed");
+    if (rc != 0) return fail(L, err);
+    L.last_counters = (const int*)na.ws.bufs["counters"].first;
+    return 0;
+}
+
 int hpglx_ordinary_kriging(const hpgl_cov_params_t* cov, const int radiuses[3], int max_nb, const int dims[3],
                            const float* values, const unsigned char* mask, float* out_val, unsigned char* out_mask,
                            double* out_var, int precision, void* stream) {
@@ -1640,12 +1752,10 @@ int hpglx_profile_summary(float* ms2, int* ncalls) {
 int hpglx_last_call_stats(unsigned int* fallbacks, int* cluster_cells) {
     Library& L = lib();
     std::lock_guard<std::mutex> g(L.mu);
-    std::string err;
-    int* counters = (int*)L.ws.get("counters", 64, err);
-    if (!counters) return fail(L, err);
     int h[4] = {0, 0, 0, 0};
     if (L.done && hipEventSynchronize(L.done) != hipSuccess) return fail(L, "hipEventSynchronize failed");
-    if (hipMemcpy(h, counters, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) return fail(L, "copy failed");
+    if (L.last_counters && hipMemcpy(h, L.last_counters, sizeof h, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(L, "copy failed");
     if (fallbacks) *fallbacks = (unsigned)h[3];
     if (cluster_cells) *cluster_cells = h[0];
     return 0;

@@ -820,6 +820,57 @@ bool DevPlan::upload(std::string& err) {
     return true;
 }
 
+bool DevPlan::place(char* base, size_t cap, size_t& used, hipStream_t st, std::string& err) {
+    const size_t nt = h.ctab.size();
+    ctab32_host.resize(nt);
+    for (size_t i = 0; i < nt; ++i) ctab32_host[i] = (float)h.ctab[i];
+    tsize = (int)h.tdx.size();
+    const size_t tb = (size_t)(tsize > 0 ? tsize : 1) * sizeof(int);
+    tpad = (tsize + TMPL_U - 1) / TMPL_U * TMPL_U + TMPL_U;
+    const size_t nw = h.octant ? h.otab_w.size() : 0;
+    const size_t no = h.octant ? h.otab.size() : 0;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base + off;
+        off = (off + bytes + 255) / 256 * 256;
+        return p;
+    };
+    ctab = (double*)take(nt * sizeof(double));
+    ctab32 = (float*)take(nt * sizeof(float));
+    tdx = (int*)take(tb);
+    tdy = (int*)take(tb);
+    tdz = (int*)take(tb);
+    tdelta = (int*)take((size_t)tpad * 4);
+    tdelta_lds = (int*)take((size_t)tpad * 4);
+    tpack = (int*)take((size_t)tpad * 4);
+    otab_w = nw ? (double*)take(nw * sizeof(double)) : nullptr;
+    rank_w = nw ? (unsigned*)take(nw * sizeof(unsigned)) : nullptr;
+    otab = no ? (double*)take(no * sizeof(double)) : nullptr;
+    tkey[0] = tkey[1] = tkey[2] = tkey[3] = -1;
+    if (off > cap) {
+        err = "the allocation-free scratch is too small for this parameter set's tables (" + std::to_string(off) +
+              " bytes, scratch " + std::to_string(cap) + ")";
+        ctab = nullptr;
+        ctab32 = nullptr;
+        tdx = tdy = tdz = tdelta = tdelta_lds = tpack = nullptr;
+        otab_w = otab = nullptr;
+        rank_w = nullptr;
+        return false;
+    }
+    used = off;
+    auto put = [&](void* dst, const void* src, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    if (!put(ctab, h.ctab.data(), nt * sizeof(double)) || !put(ctab32, ctab32_host.data(), nt * sizeof(float)) ||
+        (tsize > 0 && (!put(tdx, h.tdx.data(), tb) || !put(tdy, h.tdy.data(), tb) || !put(tdz, h.tdz.data(), tb))) ||
+        !put(otab_w, h.otab_w.data(), nw * sizeof(double)) || !put(rank_w, h.rank_w.data(), nw * sizeof(unsigned)) ||
+        !put(otab, h.otab.data(), no * sizeof(double))) {
+        err = "hipMemcpyAsync failed for the plan tables";
+        return false;
+    }
+    return true;
+}
+
 void DevPlan::release() {
     if (ctab) (void)hipFree(ctab);
     if (ctab32) (void)hipFree(ctab32);
@@ -845,6 +896,21 @@ void* Workspace::get(const std::string& name, size_t bytes, std::string& err) {
     if (bytes == 0) bytes = 256;
     auto it = bufs.find(name);
     if (it != bufs.end() && it->second.second >= bytes) return it->second.first;
+    if (arena) {
+        // Buffers keep their place between calls with the same layout, so
+        // state recorded in tags (keyed by address and geometry) stays valid.
+        const size_t off = (arena_used + 255) / 256 * 256;
+        if (off + bytes > arena_cap) {
+            if (err.empty())
+                err = "the allocation-free scratch is too small for this call: buffer '" + name + "' needs " +
+                      std::to_string(bytes) + " bytes at offset " + std::to_string(off) + " of " +
+                      std::to_string(arena_cap);
+            return nullptr;
+        }
+        arena_used = off + bytes;
+        bufs[name] = std::make_pair((void*)(arena + off), bytes);
+        return arena + off;
+    }
     if (it != bufs.end()) {
         // The previous contents are not needed; callers wait for prior work
         // before the launcher runs, so freeing here is safe.
@@ -982,7 +1048,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
                              cl_lds <= kSearchLdsMax;
     // The full-table search takes what the octant form leaves (its overflow
     // list), or every cluster-path cell without octant tables.
-    const int nlanes = oct_cluster ? 8192 : 32768;
+    const int nlanes = c.cluster_lanes > 0 ? c.cluster_lanes : (oct_cluster ? 8192 : 32768);
     auto* skey = (double*)ws.get("skey", (size_t)cap * nlanes * 8, err);
     auto* soff = (int*)ws.get("soff", (size_t)cap * nlanes * 4, err);
     if (!err.empty()) return -1;

@@ -5,6 +5,7 @@
 
 #include <map>
 #include <string>
+#include <vector>
 
 #include "plan.h"
 
@@ -29,15 +30,31 @@ struct DevPlan {
     int* tpack = nullptr;
     int tpad = 0;
     int tkey[4] = {-1, -1, -1, -1};
+    std::vector<float> ctab32_host;  // host copy of ctab32 while place()'s copies may be pending
     bool upload(std::string& err);
+    // Lays the device tables out in the region [base, base + cap) instead of
+    // allocating them and enqueues their copies from the host tables on st;
+    // `used` receives the bytes taken.  The host tables must outlive the
+    // copies.  release() must not be called on a placed plan.
+    bool place(char* base, size_t cap, size_t& used, hipStream_t st, std::string& err);
     void release();
 };
 
-// Grow-only named device buffers reused across calls.
+// Named device buffers reused across calls.  Heap mode (arena == null):
+// grow-only allocations.  Arena mode: every buffer of a call is carved, in
+// request order, from the fixed region [arena, arena + arena_cap), nothing
+// is allocated or freed; reset_arena() starts the next call's layout.  A
+// request beyond the region fails with a message naming the bytes needed.
 struct Workspace {
     std::map<std::string, std::pair<void*, size_t>> bufs;
     std::map<std::string, std::string> tags;  // state kept in a buffer between calls
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     void* get(const std::string& name, size_t bytes, std::string& err);
+    void reset_arena() {
+        bufs.clear();
+        arena_used = 0;
+    }
     void release();
 };
 
@@ -75,6 +92,7 @@ struct GridCall {
     int solve_variant = 0;                  // 0 = LDS / octant tables, 1 = full table only, 2 = no LDS table
     int search_head = 0;                    // packed offsets staged in LDS by the search (0 = default)
     int cluster_cap = 0;                    // kept candidates per lane of the LDS cluster search (0 = default)
+    int cluster_lanes = 0;                  // lanes of the full-table cluster search (multiple of 256; 0 = default)
     bool profile = false;
 };
 

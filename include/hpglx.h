@@ -70,6 +70,28 @@ typedef struct hpglx_grid_request_t {
  * `stream` (a hipStream_t; null = the default stream). */
 int hpglx_krige(const hpglx_grid_request_t* req, void* stream);
 
+/* Size of the allocation-free entry point's scratch, per GPU. */
+#define HPGLX_NOALLOC_SCRATCH_MIB 128
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the work of hpglx_krige, with the same request and
+ * results, calling no hipMalloc or hipFree.  Every device buffer the call
+ * needs beyond the caller's inputs and outputs (the parameter set's tables,
+ * padded mask, neighbour lists, cluster lists, counters) comes from one
+ * static device array of HPGLX_NOALLOC_SCRATCH_MIB MiB that each GPU holds
+ * from the library's first such call until the process ends.  A call whose
+ * buffers do not fit (the 100x100x50, 24-neighbour grid takes about 80 MiB;
+ * the neighbour lists, 4 * cells * max_neighbours bytes, are most of it)
+ * returns -1 with a message naming the bytes needed, before any kernel runs.
+ * All calls on a GPU share its array, which keeps the last call's contents.
+ * The library orders each call after the previous call's work (a stream wait
+ * on its completion event), so calls made through it do not overlap; calls
+ * that overlap on one GPU anyway (for example one captured into a graph and
+ * replayed beside another) race.  Host memory is still used for the tables
+ * on the first call with a parameter set. */
+int hpglx_krige_noalloc(const hpglx_grid_request_t* req, void* stream);
+
 /* Convenience forms of hpglx_krige. */
 int hpglx_ordinary_kriging(const struct hpgl_cov_params_t* cov, const int radiuses[3], int max_neighbours,
                            const int dims[3], const float* values, const unsigned char* mask, float* out_val,
