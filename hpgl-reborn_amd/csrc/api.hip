@@ -2,6 +2,7 @@
 // over host buffers and the device-pointer extensions (include/hpglx.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -167,6 +168,26 @@ struct Library {
     std::map<std::string, unsigned long long> na_use;
     hipStream_t io_stream = nullptr;
     IncText text;                // pinned host copy of the last property file read or written
+    unsigned long long plan_serial = 0;
+    // Graphs of repeated grid calls: a call equal to the previous one (plan,
+    // geometry, buffers, parameters) is captured into a hipGraph once and
+    // then replayed while the workspace and the state the capture relied on
+    // (the padded mask's cleared halo, the plan's template deltas) stay as
+    // they were.  Off unless HPGLX_GRAPH=1 or hpglx_set_graph_mode(1): on
+    // MI355X a replay measured no faster than the same launches from the
+    // stream.
+    struct CallGraph {
+        hipGraphExec_t exec = nullptr;
+        unsigned long long ws_gen = 0;
+        std::string pm_tag;
+        int tkey[4] = {0, 0, 0, 0};
+        unsigned long long use = 0;
+    };
+    std::map<std::string, CallGraph> graphs;
+    std::string last_key;
+    unsigned long long graph_clock = 0;
+    hipStream_t cap_stream = nullptr;
+    int graph_mode = -1;  // -1: not read yet
 };
 
 Library& lib() {
@@ -218,6 +239,7 @@ CovParams to_cov(const hpgl_cov_params_t& p) {
 // the least recently used one is dropped beyond kMaxPlans, after the last
 // call's work has finished with it.
 constexpr size_t kMaxPlans = 8;
+constexpr size_t kMaxGraphs = 8;
 
 DevPlan* get_plan(Library& L, int kind, const CovParams& c, const int r[3], int kmax, std::string& err) {
     const std::string key = plan_key(kind, c, r, kmax);
@@ -229,6 +251,7 @@ DevPlan* get_plan(Library& L, int kind, const CovParams& c, const int r[3], int 
     }
     std::unique_ptr<DevPlan> p(new DevPlan());
     if (!build_plan(p->h, (Kind)kind, c, r, kmax, err)) return nullptr;
+    p->serial = ++L.plan_serial;
     if (L.plans.size() >= kMaxPlans) {
         auto lru = L.plan_use.begin();
         for (auto u = L.plan_use.begin(); u != L.plan_use.end(); ++u)
@@ -332,6 +355,120 @@ bool request_call(const hpglx_grid_request_t* q, GridCall& call, const int* cats
     return true;
 }
 
+// ------------------------------------------------------------ call graphs
+template <class T>
+void key_put(std::string& k, const T& v) {
+    k.append((const char*)&v, sizeof v);
+}
+
+// Everything a grid call's launches take: two calls with equal keys enqueue
+// the same kernels with the same arguments (given the same workspace).
+std::string call_key(const GridCall& c, const DevPlan& plan) {
+    std::string k;
+    k.reserve(320);
+    key_put(k, plan.serial);
+    const int ints[] = {c.kind, c.precision, c.nx, c.ny, c.nz, c.nind, c.auto_mean, c.order_relations,
+                        c.refine_steps, c.search_variant, c.solve_variant, c.search_head, c.cluster_cap,
+                        c.cluster_lanes, c.blocks_per_cu, c.cats_host ? 1 : 0};
+    key_put(k, ints);
+    const void* ptrs[] = {c.values, c.mask, c.ind, c.zfield, c.thresholds, c.lvm_mean, c.out_val, c.out_mask,
+                          c.out_class, c.out_prob, c.out_var, c.out_est64, c.out_status, c.out_count, c.out_nboff};
+    key_put(k, ptrs);
+    key_put(k, c.mean);
+    key_put(k, c.pivot_tol);
+    for (int i = 0; i < c.nind && i < 16; ++i) {
+        if (c.marg_host) key_put(k, c.marg_host[i]);
+        if (c.cats_host) key_put(k, c.cats_host[i]);
+    }
+    return k;
+}
+
+bool graphs_enabled(Library& L) {
+    if (L.graph_mode < 0) {
+        const char* v = std::getenv("HPGLX_GRAPH");
+        L.graph_mode = (v && v[0] == '1') ? 1 : 0;
+    }
+    return L.graph_mode == 1;
+}
+
+// Captures the call into a graph (on a library stream, so a caller's legacy
+// default stream can still replay it).  False when the capture did not give
+// a graph that can stand for the call; then the call runs as usual.
+bool capture_call(Library& L, DevPlan& plan, const GridCall& call, const std::string& key) {
+    if (!L.cap_stream && hipStreamCreateWithFlags(&L.cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        L.cap_stream = nullptr;
+        return false;
+    }
+    const unsigned long long gen0 = L.ws.gen;
+    const std::string tag0 = L.ws.tags["pmask"];
+    int tk0[4];
+    std::copy(plan.tkey, plan.tkey + 4, tk0);
+    if (hipStreamBeginCapture(L.cap_stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    std::string cerr;
+    const int rc = launch_grid(plan, L.ws, call, L.cap_stream, nullptr, cerr);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(L.cap_stream, &graph);
+    const bool steady = L.ws.gen == gen0 && L.ws.tags["pmask"] == tag0 && std::equal(tk0, tk0 + 4, plan.tkey);
+    if (rc != 0 || e != hipSuccess || !graph || !steady) {
+        if (graph) (void)hipGraphDestroy(graph);
+        // set-up work the capture recorded never ran: make the next call redo it
+        if (L.ws.tags["pmask"] != tag0) L.ws.tags.erase("pmask");
+        if (!std::equal(tk0, tk0 + 4, plan.tkey)) std::fill(plan.tkey, plan.tkey + 4, -1);
+        (void)hipGetLastError();
+        return false;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess || !exec) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (L.graphs.size() >= kMaxGraphs) {
+        auto lru = L.graphs.begin();
+        for (auto g = L.graphs.begin(); g != L.graphs.end(); ++g)
+            if (g->second.use < lru->second.use) lru = g;
+        (void)hipGraphExecDestroy(lru->second.exec);
+        L.graphs.erase(lru);
+    }
+    Library::CallGraph& g = L.graphs[key];
+    g.exec = exec;
+    g.ws_gen = gen0;
+    g.pm_tag = tag0;
+    std::copy(tk0, tk0 + 4, g.tkey);
+    g.use = ++L.graph_clock;
+    return true;
+}
+
+// One grid call through the graph cache: replay when a valid graph exists,
+// capture when the call repeats the previous one, else launch as usual.
+int launch_call(Library& L, DevPlan& plan, const GridCall& call, hipStream_t st, std::string& err) {
+    const std::string key = call_key(call, plan);
+    auto it = L.graphs.find(key);
+    if (it == L.graphs.end() && key == L.last_key) {
+        if (capture_call(L, plan, call, key)) it = L.graphs.find(key);
+    }
+    if (it != L.graphs.end()) {
+        Library::CallGraph& g = it->second;
+        if (g.ws_gen == L.ws.gen && L.ws.tags["pmask"] == g.pm_tag && std::equal(g.tkey, g.tkey + 4, plan.tkey)) {
+            g.use = ++L.graph_clock;
+            L.last_key = key;
+            if (hipGraphLaunch(g.exec, st) != hipSuccess) {
+                err = "hipGraphLaunch failed";
+                return -1;
+            }
+            return 0;
+        }
+        (void)hipGraphExecDestroy(g.exec);
+        L.graphs.erase(it);
+    }
+    L.last_key = key;
+    return launch_grid(plan, L.ws, call, st, nullptr, err);
+}
+
 int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err,
                 const int* cats_host = nullptr) {
     if (!q) {
@@ -351,7 +488,9 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     }
     CallTiming* t = nullptr;
     if (L.profiling && !(t = next_timing(L, err))) return -1;
-    int rc = launch_grid(*plan, L.ws, call, st, t, err);
+    // profiled calls record events on their dispatches, so they run eagerly
+    int rc = (!t && graphs_enabled(L)) ? launch_call(L, *plan, call, st, err)
+                                       : launch_grid(*plan, L.ws, call, st, t, err);
     if (rc == 0 && hipEventRecord(L.done, st) != hipSuccess) {
         err = "hipEventRecord failed";
         return -1;
@@ -1759,6 +1898,12 @@ int hpglx_last_call_stats(unsigned int* fallbacks, int* cluster_cells) {
     if (fallbacks) *fallbacks = (unsigned)h[3];
     if (cluster_cells) *cluster_cells = h[0];
     return 0;
+}
+
+void hpglx_set_graph_mode(int on) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.graph_mode = on ? 1 : 0;
 }
 
 void hpglx_set_default_precision(int precision) {

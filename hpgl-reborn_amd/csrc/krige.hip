@@ -742,22 +742,6 @@ __global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(Clust
 
 // ------------------------------------------------------------------ launcher
 
-// Bucket entry points: route by kind to the per-kind translation units.
-#define HPGLX_BUCKET(NMAX)                                                                                   \
-    void dispatch_solve_##NMAX(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem,            \
-                               const hipEvent_t* ev, hipStream_t st) {                                       \
-        if (kind == 0)                                                                                        \
-            dispatch_solve_##NMAX##_0(sa, mixed, n, nelem, ev, st);                                           \
-        else if (kind == 1)                                                                                   \
-            dispatch_solve_##NMAX##_1(sa, mixed, n, nelem, ev, st);                                           \
-        else                                                                                                  \
-            dispatch_solve_##NMAX##_2(sa, mixed, n, nelem, ev, st);                                           \
-    }
-HPGLX_BUCKET(8)
-HPGLX_BUCKET(16)
-HPGLX_BUCKET(24)
-HPGLX_BUCKET(32)
-#undef HPGLX_BUCKET
 
 
 #define HPGLX_CHECK(expr)                                              \
@@ -892,6 +876,204 @@ void DevPlan::release() {
     tdx = tdy = tdz = nullptr;
 }
 
+// ----------------------------------------------------------- work list
+// The solve's work list (solve.h): the cells to krige ordered by search path
+// (0 template walk, 1 cluster search; every cell is path 1 when the solve
+// has no LDS table), count bucket (25-32, 17-24, 9-16, 0-8 neighbours) and
+// count, descending.  Bin = 33 path + 32 - n.  Informed OK/SK cells are not
+// listed: the counting pass writes their outputs (the data value, mask 1,
+// variance 0).
+__device__ __forceinline__ int work_bin(int cnt, int all_global) {
+    const bool cl = all_global || cnt < CNT_INFORMED || (cnt >= 0 && (cnt & CNT_CLUSTER));
+    return (cl ? 33 : 0) + 32 - count_n(cnt);
+}
+
+// Wave-aggregated add of one to h[bin] (bin < 0: none): one atomic per
+// distinct bin of the wave, which also returns each lane's rank among the
+// wave's lanes of its bin added before it (base from the atomic + lanes of
+// lower index).  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_bin_add(int* h, int bin) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(bin >= 0);
+    int rank = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int lb = __shfl(bin, leader);
+        const unsigned long long same = __ballot(bin == lb) & todo;
+        int wbase = 0;
+        if (lane == leader) wbase = atomicAdd(&h[lb], __popcll(same));
+        wbase = __shfl(wbase, leader);
+        if ((same >> lane) & 1ull) rank = wbase + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return rank;
+}
+
+// The count and scatter passes run BUCKET_BLOCKS blocks, block b over the
+// same contiguous range of cells, so the list is a counting sort without
+// global atomics: the count pass writes each block's bin counts, the scan
+// turns them into each block's place in each bin, the scatter writes.
+constexpr int BUCKET_BLOCKS = 1024;
+constexpr int BUCKET_UNROLL = 4;  // tiles of 256 cells whose loads issue together
+
+__device__ __forceinline__ void bucket_range(long long n, long long& lo, long long& hi) {
+    const long long per = ((n + BUCKET_BLOCKS - 1) / BUCKET_BLOCKS + 255) / 256 * 256;
+    lo = (long long)blockIdx.x * per;
+    hi = lo + per < n ? lo + per : n;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_count(SolveArgs a, int all_global, int* hist) {
+    __shared__ int h[NBINS];
+    for (int i = threadIdx.x; i < NBINS; i += 256) h[i] = 0;
+    __syncthreads();
+    long long lo, hi;
+    bucket_range(a.g.n, lo, hi);
+    for (long long base = lo; base < hi; base += 256 * BUCKET_UNROLL) {
+        int cnt[BUCKET_UNROLL];
+#pragma unroll
+        for (int k = 0; k < BUCKET_UNROLL; ++k) {
+            const long long cell = base + 256 * k + threadIdx.x;
+            cnt[k] = cell < hi ? a.count[cell] : CNT_INFORMED;
+        }
+#pragma unroll
+        for (int k = 0; k < BUCKET_UNROLL; ++k) {
+            const long long cell = base + 256 * k + threadIdx.x;
+            int bin = -1;
+            if (cell < hi) {
+                if (a.kind != 2 && cnt[k] == CNT_INFORMED) {
+                    const float v = a.values[cell];
+                    if (a.out_val) a.out_val[cell] = v;
+                    if (a.out_mask) a.out_mask[cell] = 1;
+                    if (a.out_var) a.out_var[cell] = 0.0;
+                    if (a.out_est64) a.out_est64[cell] = v;
+                    if (a.out_status) a.out_status[cell] = ST_INFORMED;
+                } else {
+                    bin = work_bin(cnt[k], all_global);
+                }
+            }
+            (void)wave_bin_add(h, bin);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NBINS; i += 256) hist[blockIdx.x * NBINS + i] = h[i];
+}
+
+// One block per bin: exclusive prefix of the bin's column of block counts,
+// written back in place, and the bin's total.
+__global__ __launch_bounds__(256) void k_bucket_scan(int* hist, int* tot) {
+    __shared__ int wsum[4];
+    constexpr int PER = BUCKET_BLOCKS / 256;
+    const int bin = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int v[PER];
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        v[k] = hist[(tid * PER + k) * NBINS + bin];
+        local += v[k];
+    }
+    int x = local;  // inclusive scan over the block
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int before = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) before += k < w ? wsum[k] : 0;
+    int run = before + x - local;  // exclusive
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        hist[(tid * PER + k) * NBINS + bin] = run;
+        run += v[k];
+    }
+    if (tid == 255) tot[bin] = run;
+}
+
+// Lays the bins out from their totals (each bucket starting at a multiple of
+// SOLVE_CHUNK; block 0 also writes the segment table and the gaps, -1) and
+// writes the block's cells at its places.
+__global__ __launch_bounds__(256) void k_bucket_scatter(SolveArgs a, int all_global, const int* hist, const int* tot,
+                                                        int* seg, int* sorted) {
+    __shared__ int tot_s[NBINS], off[NBINS], loc[NBINS];
+    const int tid = threadIdx.x;
+    if (tid < NBINS) {
+        tot_s[tid] = tot[tid];
+        off[tid] = hist[blockIdx.x * NBINS + tid];
+        loc[tid] = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int pos = 0;
+        for (int p = 0; p < 2; ++p)
+            for (int b = 3; b >= 0; --b) {
+                const int begin = (pos + SOLVE_CHUNK - 1) / SOLVE_CHUNK * SOLVE_CHUNK;
+                if (blockIdx.x == 0)
+                    for (int i = pos; i < begin; ++i) sorted[i] = -1;
+                pos = begin;
+                const int nhi = 8 * b + 8, nlo = b == 0 ? 0 : 8 * b + 1;
+                for (int n = nhi; n >= nlo; --n) {
+                    const int bin = 33 * p + 32 - n;
+                    off[bin] += pos;
+                    pos += tot_s[bin];
+                }
+                if (blockIdx.x == 0) {
+                    seg[2 * (4 * p + b)] = begin;
+                    seg[2 * (4 * p + b) + 1] = pos;
+                }
+            }
+    }
+    __syncthreads();
+    long long lo, hi;
+    bucket_range(a.g.n, lo, hi);
+    for (long long base = lo; base < hi; base += 256 * BUCKET_UNROLL) {
+        int cnt[BUCKET_UNROLL];
+#pragma unroll
+        for (int k = 0; k < BUCKET_UNROLL; ++k) {
+            const long long cell = base + 256 * k + tid;
+            cnt[k] = cell < hi ? a.count[cell] : CNT_INFORMED;
+        }
+#pragma unroll
+        for (int k = 0; k < BUCKET_UNROLL; ++k) {
+            const long long cell = base + 256 * k + tid;
+            int bin = -1;
+            if (cell < hi && !(a.kind != 2 && cnt[k] == CNT_INFORMED)) bin = work_bin(cnt[k], all_global);
+            const int rank = wave_bin_add(loc, bin);
+            if (bin >= 0) sorted[off[bin] + rank] = (int)cell;
+        }
+    }
+}
+
+bool solve_sorted(int top, int kind, const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
+    switch (top * 4 + kind) {
+        case 8 * 4 + 0: return solve_sorted_8_0(sa, l, st);
+        case 8 * 4 + 1: return solve_sorted_8_1(sa, l, st);
+        case 8 * 4 + 2: return solve_sorted_8_2(sa, l, st);
+        case 16 * 4 + 0: return solve_sorted_16_0(sa, l, st);
+        case 16 * 4 + 1: return solve_sorted_16_1(sa, l, st);
+        case 16 * 4 + 2: return solve_sorted_16_2(sa, l, st);
+        case 24 * 4 + 0: return solve_sorted_24_0(sa, l, st);
+        case 24 * 4 + 1: return solve_sorted_24_1(sa, l, st);
+        case 24 * 4 + 2: return solve_sorted_24_2(sa, l, st);
+        case 32 * 4 + 0: return solve_sorted_32_0(sa, l, st);
+        case 32 * 4 + 1: return solve_sorted_32_1(sa, l, st);
+        default: return solve_sorted_32_2(sa, l, st);
+    }
+}
+
+void solve_fallback(bool g8, int kind, const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    if (g8) {
+        if (kind == 0) solve_fallback_g8_0(sa, max_cells, st);
+        else if (kind == 1) solve_fallback_g8_1(sa, max_cells, st);
+        else solve_fallback_g8_2(sa, max_cells, st);
+    } else {
+        if (kind == 0) solve_fallback_g4_0(sa, max_cells, st);
+        else if (kind == 1) solve_fallback_g4_1(sa, max_cells, st);
+        else solve_fallback_g4_2(sa, max_cells, st);
+    }
+}
+
 void* Workspace::get(const std::string& name, size_t bytes, std::string& err) {
     if (bytes == 0) bytes = 256;
     auto it = bufs.find(name);
@@ -923,12 +1105,14 @@ void* Workspace::get(const std::string& name, size_t bytes, std::string& err) {
         err = "hipMalloc failed for workspace buffer '" + name + "' (" + std::to_string(bytes) + " bytes)";
         return nullptr;
     }
+    ++gen;
     bufs[name] = std::make_pair(p, bytes);
     tags.erase(name);
     return p;
 }
 
 void Workspace::release() {
+    ++gen;
     for (auto& kv : bufs) (void)hipFree(kv.second.first);
     bufs.clear();
     tags.clear();
@@ -1026,13 +1210,13 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
     auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
     auto* cl_cells = (int*)ws.get("cl_cells", (size_t)n * 4, err);
-    auto* counters = (int*)ws.get("counters", 64, err);
+    auto* counters = (int*)ws.get("counters", CTR_COUNT * 4, err);
+    auto* sorted = (int*)ws.get("sorted", ((size_t)n + 16 * SOLVE_CHUNK) * 4, err);
+    auto* bucket_hist = (int*)ws.get("bucket_hist", (size_t)BUCKET_BLOCKS * NBINS * 4, err);
     const int mean_blocks = 1024;
     auto* psum = (double*)ws.get("mean_psum", mean_blocks * 8, err);
     auto* pcnt = (int*)ws.get("mean_pcnt", mean_blocks * 4, err);
     auto* dmean = (double*)ws.get("mean", 8, err);
-    auto* dmarg = (double*)ws.get("marg", 16 * 8, err);
-    auto* dcats = (int*)ws.get("cats", 16 * 4, err);
     // Cluster-path scratch: every candidate comes from one of 27 clusters
     // holding at most `limit` nodes each.
     const int cap = (int)(27 * (h.limit > 0 ? h.limit : 1));
@@ -1052,11 +1236,11 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     auto* skey = (double*)ws.get("skey", (size_t)cap * nlanes * 8, err);
     auto* soff = (int*)ws.get("soff", (size_t)cap * nlanes * 4, err);
     if (!err.empty()) return -1;
-    int* n_cl_cells = counters + 0;
-    int* overflow = counters + 2;
-    int* n_fallback = counters + 3;
-    int* n_ovf = counters + 4;
-    int* n_ovf2 = counters + 5;
+    int* n_cl_cells = counters + CTR_N_CL_CELLS;
+    int* overflow = counters + CTR_OVERFLOW;
+    int* n_fallback = counters + CTR_N_FALLBACK;
+    int* n_ovf = counters + CTR_N_OVF;
+    int* n_ovf2 = counters + CTR_N_OVF2;
     auto* ovf_cells = (int*)ws.get("ovf_cells", (size_t)n * 4, err);
     auto* ovf2_cells = (int*)ws.get("ovf2_cells", (size_t)n * 4, err);
     auto* fb_cells = (int*)ws.get("fb_cells", (size_t)n * 4, err);
@@ -1079,7 +1263,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         ws.tags["pmask"] = pm_sig;
     }
     hipLaunchKernelGGL(k_cluster_build, dim3(ncl), dim3(256), 0, st, g, c.mask, pmask, cl_count, cl_nodes, node_cap,
-                       counters, 16);
+                       counters, (int)CTR_COUNT);
     hipLaunchKernelGGL(k_cluster_flags, dim3(blocks_for(ncl, B)), dim3(B), 0, st, g, (const int*)cl_count,
                        (long long)h.limit, nbhd);
     // Template deltas for this grid's padding and tile box, kept by the plan.
@@ -1096,16 +1280,6 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         hipLaunchKernelGGL(k_mean_partial, dim3(mean_blocks), dim3(256), 0, st, n, c.values, c.mask, psum, pcnt);
         hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, st, mean_blocks, (const double*)psum,
                            (const int*)pcnt, dmean);
-    }
-    if (c.kind == KIND_MIK) {
-        double hm[16] = {0};
-        for (int k = 0; k < c.nind; ++k) hm[k] = c.marg_host[k];
-        HPGLX_CHECK(hipMemcpyAsync(dmarg, hm, sizeof(hm), hipMemcpyHostToDevice, st));
-        if (c.cats_host) {
-            int hc[16] = {0};
-            for (int k = 0; k < c.nind; ++k) hc[k] = c.cats_host[k];
-            HPGLX_CHECK(hipMemcpyAsync(dcats, hc, sizeof(hc), hipMemcpyHostToDevice, st));
-        }
     }
 
     SearchArgs sa;
@@ -1130,10 +1304,10 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         if (tile_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_tile, (int)tile_lds));
         const dim3 grid((c.nx + TILE_X - 1) / TILE_X, (c.ny + TILE_Y - 1) / TILE_Y, c.nz);
         const int head = c.search_head > 0 && c.search_head < SEARCH_HEAD ? c.search_head : SEARCH_HEAD;
-        hipExtLaunchKernelGGL(k_search_tile, grid, dim3(TILE_X * TILE_Y), (uint32_t)tile_lds, st, ev_t[0], ev_t[1],
-                              0, sa, head);
+        HPGLX_LAUNCH_EV(k_search_tile, grid, dim3(TILE_X * TILE_Y), (uint32_t)tile_lds, st, ev_t[0], ev_t[1], sa,
+                        head);
     } else {
-        hipExtLaunchKernelGGL(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, ev_t[0], ev_t[1], 0, sa);
+        HPGLX_LAUNCH_EV(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, ev_t[0], ev_t[1], sa);
     }
 
     ClusterSearchArgs ca;
@@ -1186,7 +1360,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     }
     hipLaunchKernelGGL(k_search_cluster, dim3(nlanes / B), dim3(B), 0, st, ca);
 
-    SolveArgs so;
+    SolveArgs so{};
     so.g = g;
     so.kind = c.kind;
     so.kmax = K;
@@ -1198,8 +1372,12 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.nind = c.nind;
     so.mean_dev = (c.kind == KIND_SK && c.auto_mean) ? dmean : nullptr;
     so.mean = c.mean;
-    so.marg = dmarg;
-    so.cats = c.cats_host ? dcats : nullptr;
+    // marginal probabilities and categories travel in the kernel arguments
+    for (int k = 0; k < 16; ++k) {
+        so.marg[k] = (c.kind == KIND_MIK && k < c.nind) ? c.marg_host[k] : 0.0;
+        so.cats[k] = (c.cats_host && k < c.nind) ? c.cats_host[k] : 0;
+    }
+    so.has_cats = c.cats_host ? 1 : 0;
     so.lvm_mean = c.lvm_mean;
     so.count = count;
     so.nboff = nboff;
@@ -1231,17 +1409,60 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     so.fallback_cells = fb_cells;
     so.n_fallback = n_fallback;
     const bool mixed = c.precision == PREC_MIXED;
-    const int nelem = (plan.otab && c.solve_variant == 0) ? (int)h.otab.size() : 0;
     if (K > 32) {
         if (dispatch_solve_wide(so, c.kind, ws, ev_t + 2, st, err) != 0) return -1;
-    } else if (K <= 8)
-        dispatch_solve_8(so, c.kind, mixed, n, nelem, ev_t + 2, st);
-    else if (K <= 16)
-        dispatch_solve_16(so, c.kind, mixed, n, nelem, ev_t + 2, st);
-    else if (K <= 24)
-        dispatch_solve_24(so, c.kind, mixed, n, nelem, ev_t + 2, st);
-    else
-        dispatch_solve_32(so, c.kind, mixed, n, nelem, ev_t + 2, st);
+        HPGLX_CHECK(hipGetLastError());
+        return 0;
+    }
+    // The LDS table form needs the packed x/y table addresses (both 16-bit
+    // fields below 32768: x up to E_x * 8, y up to E_y * (E_x + 1) * 8).
+    const bool pack = (long long)h.E[0] * 8 < 32768 && (long long)h.E[1] * (h.E[0] + 1) * 8 < 32768;
+    const int nelem = (plan.otab && c.solve_variant == 0 && pack) ? (int)h.otab.size() : 0;
+    // (the table must fit one block's LDS)
+    const bool lds = nelem > 0 && (size_t)nelem * sizeof(double) <= kSearchLdsMax;
+    const int all_global = lds ? 0 : 1;
+    so.sorted = sorted;
+    so.seg = counters + CTR_SEG;
+    hipLaunchKernelGGL(k_bucket_count, dim3(BUCKET_BLOCKS), dim3(256), 0, st, so, all_global, bucket_hist);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(NBINS), dim3(256), 0, st, bucket_hist, counters + CTR_BIN_TOTAL);
+    hipLaunchKernelGGL(k_bucket_scatter, dim3(BUCKET_BLOCKS), dim3(256), 0, st, so, all_global,
+                       (const int*)bucket_hist, (const int*)(counters + CTR_BIN_TOTAL), counters + CTR_SEG, sorted);
+    // Buckets up to `top` run with 4 lanes per cell in one launch per path;
+    // the 25-32 bucket with 8 lanes in its own.  The main kernel (events) is
+    // the largest bucket's launch on the main path.
+    const int top = K <= 8 ? 8 : (K <= 16 ? 16 : 24);
+    const bool g8 = K > 24;
+    SortedLaunch sl;
+    sl.mixed = mixed;
+    sl.blocks_per_cu = c.blocks_per_cu;
+    for (int path = lds ? 0 : 1; path < 2; ++path) {
+        sl.path = path;
+        sl.nelem = path == 0 ? nelem : 0;
+        const bool main_path = path == (lds ? 0 : 1);
+        sl.ev = (main_path && !g8) ? ev_t + 2 : nullptr;
+        // where the LDS form cannot run, the global tables serve the same range
+        if (!solve_sorted(top, c.kind, so, sl, st)) {
+            sl.nelem = 0;
+            solve_sorted(top, c.kind, so, sl, st);
+        }
+        if (g8) {
+            sl.nelem = path == 0 ? nelem : 0;
+            sl.ev = main_path ? ev_t + 2 : nullptr;
+            if (!solve_sorted(32, c.kind, so, sl, st)) {
+                sl.nelem = 0;
+                solve_sorted(32, c.kind, so, sl, st);
+            }
+        }
+    }
+    if (mixed) {
+        SolveArgs fb = so;
+        fb.cell_list = fb_cells;
+        fb.n_list = n_fallback;
+        fb.fallback_cells = nullptr;
+        fb.n_fallback = nullptr;
+        solve_fallback(false, c.kind, fb, n, st);
+        if (g8) solve_fallback(true, c.kind, fb, n, st);
+    }
     HPGLX_CHECK(hipGetLastError());
     return 0;
 }

@@ -2,8 +2,20 @@
 // their host launcher.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
+
+// A launch that records start/stop events on its dispatch when given them
+// (profiled calls); without events a plain launch, which stream capture
+// takes into a graph.
+#define HPGLX_LAUNCH_EV(kernel, grid, block, lds, st, e0, e1, ...)                        \
+    do {                                                                                  \
+        if ((e0) != nullptr || (e1) != nullptr)                                           \
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, st, e0, e1, 0, __VA_ARGS__); \
+        else                                                                              \
+            hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);                \
+    } while (0)
 
 namespace hpglx {
 
@@ -85,8 +97,9 @@ struct SolveArgs {
     int nind;
     const double* mean_dev;        // SK automatic mean (device), or null
     double mean;                   // SK explicit mean
-    const double* marg;            // MIK: nind marginal probabilities (device)
-    const int* cats;               // IK categories: indicator k is (ind == cats[k]) (device), else null
+    double marg[16];               // MIK: nind marginal probabilities
+    int cats[16];                  // IK categories: indicator k is (ind == cats[k]) when has_cats
+    int has_cats;
     const float* lvm_mean;         // SK with a locally varying mean (N), else null
     const int* count;
     const int* nboff;
@@ -114,6 +127,31 @@ struct SolveArgs {
     const int* n_cl_cells;         // their number (device)
     int* fallback_cells;           // mixed precision: cells queued for the FP64 form
     int* n_fallback;               // its length (device counter)
+    const int* sorted;             // work list ordered by path and count bucket (k_bucket_scatter), -1 = gap
+    const int* seg;                // its segments: [begin, end) of (path, bucket) at seg[2 * (4 path + b)]
 };
+
+// Count buckets of the solve: the register solver sizes 8, 16, 24 and 32
+// (bucket 0 also takes cells without neighbours).
+__host__ __device__ inline int count_bucket(int n) { return n <= 8 ? 0 : (n <= 16 ? 1 : (n <= 24 ? 2 : 3)); }
+// Neighbour count of a count[] entry (0 for the negative markers).
+__host__ __device__ inline int count_n(int cnt) { return cnt < 0 ? 0 : (cnt & CNT_MASK); }
+
+// Work-list chunk: the cells one wave solves together (4 lanes per cell);
+// segments start at multiples of it.
+constexpr int SOLVE_CHUNK = 16;
+
+// Layout of the per-call device counters (ints; cleared by k_cluster_build).
+enum {
+    CTR_N_CL_CELLS = 0,
+    CTR_OVERFLOW = 2,
+    CTR_N_FALLBACK = 3,
+    CTR_N_OVF = 4,
+    CTR_N_OVF2 = 5,
+    CTR_SEG = 16,        // (path, bucket) segments of the work list: 2 x 4 x [begin, end)
+    CTR_BIN_TOTAL = 32,  // cells per work-list bin (NBINS)
+    CTR_COUNT = 128      // ints cleared per call
+};
+constexpr int NBINS = 66;  // work-list bins: 2 paths x 33 counts
 
 }  // namespace hpglx

@@ -30,6 +30,7 @@ struct DevPlan {
     int* tpack = nullptr;
     int tpad = 0;
     int tkey[4] = {-1, -1, -1, -1};
+    unsigned long long serial = 0;   // distinguishes plans that reuse an address
     std::vector<float> ctab32_host;  // host copy of ctab32 while place()'s copies may be pending
     bool upload(std::string& err);
     // Lays the device tables out in the region [base, base + cap) instead of
@@ -50,6 +51,7 @@ struct Workspace {
     std::map<std::string, std::string> tags;  // state kept in a buffer between calls
     char* arena = nullptr;
     size_t arena_cap = 0, arena_used = 0;
+    unsigned long long gen = 0;  // bumped by every allocation and release (heap mode)
     void* get(const std::string& name, size_t bytes, std::string& err);
     void reset_arena() {
         bufs.clear();
@@ -93,6 +95,7 @@ struct GridCall {
     int search_head = 0;                    // packed offsets staged in LDS by the search (0 = default)
     int cluster_cap = 0;                    // kept candidates per lane of the LDS cluster search (0 = default)
     int cluster_lanes = 0;                  // lanes of the full-table cluster search (multiple of 256; 0 = default)
+    int blocks_per_cu = 0;                  // cap on the persistent solve grids' blocks per CU (0 = all that fit)
     bool profile = false;
 };
 

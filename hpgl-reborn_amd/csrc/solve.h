@@ -305,6 +305,92 @@ struct Chol {
         }
     }
 
+    // Clears the entries on and above the diagonal of each slot's diagonal
+    // tile (the factor leaves them with values it never reads; the diagonal
+    // itself lives on as D).  The solves below then need no per-lane tests:
+    // a cleared entry adds 0 * y to a row that is not to change.
+    static __device__ __forceinline__ void clear_upper(T (&L)[TOT], int t) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                T& e = L[slot_off<G>(s) + G * s + i];
+                e = (t > i) ? e : T(0);
+            }
+    }
+
+    // z <- L^-1 z after clear_upper: every row takes every update (rows not
+    // below j through a cleared entry); row r is scaled by D once, at the
+    // end, with the value its owner broadcast at step r.
+    template <typename V>
+    static __device__ __forceinline__ void forward_cu(const T (&L)[TOT], const T (&D)[S], V (&z)[S]) {
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            const int q = j % G, sj = j / G;
+            const V yj = gbc<G>(z[sj] * (V)D[sj], q);
+#pragma unroll
+            for (int s = sj; s < S; ++s) z[s] = fmadd((V)L[slot_off<G>(s) + j], yj, z[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) z[s] *= (V)D[s];
+    }
+
+    template <typename V>
+    static __device__ __forceinline__ void forward2_cu(const T (&L)[TOT], const T (&D)[S], V (&z)[S], V (&u)[S]) {
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            const int q = j % G, sj = j / G;
+            const V zj = gbc<G>(z[sj] * (V)D[sj], q), uj = gbc<G>(u[sj] * (V)D[sj], q);
+#pragma unroll
+            for (int s = sj; s < S; ++s) {
+                const V lv = (V)L[slot_off<G>(s) + j];
+                z[s] = fmadd(lv, zj, z[s]);
+                u[s] = fmadd(lv, uj, u[s]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            z[s] *= (V)D[s];
+            u[s] *= (V)D[s];
+        }
+    }
+
+    // x <- L^-T x after clear_upper (cleared entries add 0 to the sums).
+    template <typename V>
+    static __device__ __forceinline__ void backward_cu(const T (&L)[TOT], const T (&D)[S], V (&x)[S], int t) {
+#pragma unroll
+        for (int i = NMAX - 1; i >= 0; --i) {
+            const int q = i % G, si = i / G;
+            V p = V(0);
+#pragma unroll
+            for (int s = si; s < S; ++s) p += (V)L[slot_off<G>(s) + i] * x[s];
+            p = gsum<G>(p);
+            if (t == q) x[si] = (x[si] + p) * (V)D[si];
+        }
+    }
+
+    template <typename V>
+    static __device__ __forceinline__ void backward2_cu(const T (&L)[TOT], const T (&D)[S], V (&x)[S], V (&y)[S],
+                                                        int t) {
+#pragma unroll
+        for (int i = NMAX - 1; i >= 0; --i) {
+            const int q = i % G, si = i / G;
+            V px = V(0), py = V(0);
+#pragma unroll
+            for (int s = si; s < S; ++s) {
+                const V lv = (V)L[slot_off<G>(s) + i];
+                px += lv * x[s];
+                py += lv * y[s];
+            }
+            px = gsum<G>(px);
+            py = gsum<G>(py);
+            if (t == q) {
+                x[si] = (x[si] + px) * (V)D[si];
+                y[si] = (y[si] + py) * (V)D[si];
+            }
+        }
+    }
+
     // x <- L^-T x (partial dot products, group sum, owner divides)
     template <typename V>
     static __device__ __forceinline__ void backward(const T (&L)[TOT], const T (&D)[S], V (&x)[S], int t) {
@@ -355,7 +441,9 @@ constexpr int OFF_BIAS = 1024;
 
 // Weights of one cell: w[s] for the lane's rows, mu (OK), status.  Returns
 // false when the mixed form hands the cell to the FP64 form.
-template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK>
+// FULL: the cell has exactly NMAX neighbours, so no row is dead and no
+// liveness test is compiled.
+template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK, bool FULL>
 __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, const double* ltab,
                                              unsigned ox, unsigned oxy, double (&w)[NMAX / G], double (&bq)[NMAX / G],
                                              int (&dnode)[NMAX / G], double& mu, int& status) {
@@ -375,7 +463,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int r = G * s + t;
-        pk[s] = nb[r < n ? r : 0];  // unconditional, so the loads issue together
+        pk[s] = nb[(FULL || r < n) ? r : 0];  // unconditional, so the loads issue together
     }
     const unsigned OX = ox, OXY = oxy;
 #pragma unroll
@@ -383,7 +471,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         const int r = G * s + t;
         int dx, dy, dz;
         unpack_off(pk[s], dx, dy, dz);
-        if (r >= n) dx = dy = dz = 0;
+        if (!FULL && r >= n) dx = dy = dz = 0;
         tr[s] = (dz * SY + dy) * SX + dx;
         dnode[s] = (dz * g.ny + dy) * g.nx + dx;
         if (PACK) {
@@ -425,7 +513,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 #pragma unroll
         for (int s = c / G; s < S; ++s) {
             const int r = G * s + t;
-            const bool live = r < n;
+            const bool live = FULL || r < n;
             T v;
             if (LTAB)
                 v = (T)lcov(s, cx, cy, cz);
@@ -442,7 +530,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const double v = LTAB ? lcov(s, c0x, c0y, c0z) : a.ctab[center + tr[s]];
-        bq[s] = (G * s + t < n) ? v : 0.0;
+        bq[s] = (FULL || G * s + t < n) ? v : 0.0;
     }
     T minpiv;
     bool ok;
@@ -461,6 +549,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         status = ST_SINGULAR;
         return true;
     }
+    CH::clear_upper(L, t);
     double sv[S], ov[S];
     if (OKK) {
         // A s = b and A o = 1 side by side
@@ -468,10 +557,10 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             z[s] = (T)bq[s];
-            u[s] = (G * s + t < n) ? T(1) : T(0);
+            u[s] = (FULL || G * s + t < n) ? T(1) : T(0);
         }
-        CH::forward2(L, D, z, u, t);
-        CH::backward2(L, D, z, u, t);
+        CH::forward2_cu(L, D, z, u);
+        CH::backward2_cu(L, D, z, u, t);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             sv[s] = (double)z[s];
@@ -481,8 +570,8 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         T z[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) z[s] = (T)bq[s];
-        CH::forward(L, D, z, t);
-        CH::backward(L, D, z, t);
+        CH::forward_cu(L, D, z);
+        CH::backward_cu(L, D, z, t);
 #pragma unroll
         for (int s = 0; s < S; ++s) sv[s] = (double)z[s];
     }
@@ -536,13 +625,15 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
                     // bound the loads the scheduler keeps in flight
                     if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
                 }
+                if (!FULL) {
 #pragma unroll
-                for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+                    for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+                }
                 T z[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) z[s] = (T)r[s];
-                CH::forward(L, D, z, t);
-                CH::backward(L, D, z, t);
+                CH::forward_cu(L, D, z);
+                CH::backward_cu(L, D, z, t);
                 double pz = 0.0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) pz += (double)z[s];
@@ -582,13 +673,15 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
                     // bound the loads the scheduler keeps in flight
                     if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
                 }
+                if (!FULL) {
 #pragma unroll
-                for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+                    for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+                }
                 T z[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) z[s] = (T)r[s];
-                CH::forward(L, D, z, t);
-                CH::backward(L, D, z, t);
+                CH::forward_cu(L, D, z);
+                CH::backward_cu(L, D, z, t);
 #pragma unroll
                 for (int s = 0; s < S; ++s) w[s] += (double)z[s];
             }
@@ -600,7 +693,7 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
 // Kriges one cell with the group of G lanes; t = lane index in the group,
 // cnt = the cell's count[] entry, ltab/ox/oxy = the octant table and its
 // strides (LTAB form).
-template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK = false>
+template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK = false, bool FULL = false>
 __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, int cnt, int t, const double* ltab,
                                            unsigned ox, unsigned oxy) {
     constexpr int S = NMAX / G;
@@ -622,7 +715,8 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
     if (n == 0) {
         status = ST_NO_NEIGHBOURS;
-    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB, PACK>(a, cell, t, n, ltab, ox, oxy, w, bq, dnode, mu, status)) {
+    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB, PACK, FULL>(a, cell, t, n, ltab, ox, oxy, w, bq, dnode, mu,
+                                                                     status)) {
         return;  // queued for the FP64 form
     }
     if (status != ST_KRIGED) {
@@ -645,7 +739,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
         double pe = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const bool live = G * s + t < n;
+            const bool live = FULL || G * s + t < n;
             const long long node = cell + (live ? dnode[s] : 0);
             const float v0 = a.values[node];
             const float v = live ? v0 : 0.0f;
@@ -657,7 +751,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
 #pragma unroll
         for (int i = 0; i < NMAX; ++i) {
             const float ti = gbc<G>(term[i / G], i % G);
-            if (i < n) acc += ti;
+            if (FULL || i < n) acc += ti;
         }
         const double e64 = (double)mf + gsum<G>(pe);
         const double var = a.c0 - wb - mu;
@@ -692,12 +786,12 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     for (int k = 0; k < a.nind; ++k) {
         const float pf = (float)a.marg[k];
         const float thr = a.zfield ? a.thresholds[k] : 0.f;
-        const int cat = a.cats ? a.cats[k] : -1;
+        const int cat = a.has_cats ? a.cats[k] : -1;
         double term[S];
         double pe = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const bool live = G * s + t < n;
+            const bool live = FULL || G * s + t < n;
             const long long node = cell + (live ? dnode[s] : 0);
             float iv;
             if (a.zfield)
@@ -716,13 +810,13 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
 #pragma unroll
             for (int i = 0; i < NMAX; ++i) {
                 const double ti = gbc<G>(term[i / G], i % G);
-                if (i < n) accf += (float)ti;
+                if (FULL || i < n) accf += (float)ti;
             }
         } else {
 #pragma unroll
             for (int i = 0; i < NMAX; ++i) {
                 const double ti = gbc<G>(term[i / G], i % G);
-                if (i < n) acc += ti;
+                if (FULL || i < n) acc += ti;
             }
         }
         const double e64 = (double)pf + gsum<G>(pe);
@@ -766,38 +860,133 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
 }
 
-// One group per cell, covariances from global memory: the +-W octant table
-// when OCT (angles all zero), else the full table.  With a cell list, groups
-// stride over the list.
-template <int NMAX, int G, bool MIXED, int KIND, bool OCT>
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-    const int t = threadIdx.x & (G - 1);
-    const long long gid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const long long ngroups = ((long long)gridDim.x * blockDim.x) / G;
-    const long long total = a.cell_list ? (long long)*a.n_list : a.g.n;
-    const unsigned ox = (unsigned)(a.wx + 1), oxy = ox * (unsigned)(a.wy + 1);
-    for (long long item = gid; item < total; item += ngroups) {
-        const long long cell = a.cell_list ? (long long)a.cell_list[item] : item;
-        solve_cell<NMAX, G, MIXED, KIND, OCT>(a, cell, a.count[cell], t, a.otab_w, ox, oxy);
+// ------------------------------------------------ count-bucketed work list
+// The list (k_bucket_scatter) holds the cells to krige ordered by search path
+// (template walk 0, cluster search 1), count bucket (largest first) and
+// neighbour count (descending); each (path, bucket) segment starts at a
+// multiple of SOLVE_CHUNK and gaps hold -1.  A wave takes one chunk of
+// SOLVE_CHUNK consecutive entries, so all its cells share a bucket and it
+// runs one register-solver size; when every cell of the chunk has exactly
+// the bucket's size (and FULL_OK), the solver without liveness tests runs.
+// Chunks are dealt to the waves round-robin, so every wave gets a share of
+// each bucket.
+template <int NB, int G, bool MIXED, int KIND, bool LTAB, bool PACK, bool FULL_OK>
+__device__ __forceinline__ void solve_chunk(const SolveArgs& a, int i0, int iend, int lane, bool full,
+                                            const double* ltab, unsigned ox, unsigned oxy) {
+    constexpr int PER = 64 / G;  // cells per pass of the wave
+    const int grp = lane / G, t = lane & (G - 1);
+#pragma unroll
+    for (int p = 0; p < SOLVE_CHUNK / PER; ++p) {
+        const int idx = i0 + p * PER + grp;
+        const int cell = idx < iend ? a.sorted[idx] : -1;
+        if (cell < 0) continue;
+        const int cnt = a.count[cell];
+        if (FULL_OK && full)
+            solve_cell<NB, G, MIXED, KIND, LTAB, PACK, true>(a, cell, cnt, t, ltab, ox, oxy);
+        else
+            solve_cell<NB, G, MIXED, KIND, LTAB, PACK, false>(a, cell, cnt, t, ltab, ox, oxy);
     }
 }
 
-// Persistent form with the octant covariance table staged in LDS: every
-// template-path cell; cluster-path cells (pair offsets beyond the octant
-// table) are left to k_solve over the cluster cell list.
-template <int NMAX, int G, bool MIXED, int KIND, int BLOCK, bool PACK>
-__global__ __launch_bounds__(BLOCK) void k_solve_lds(SolveArgs a, int nelem) {
+// Chunks of [lo, hi) for one wave, strided over the grid's waves.  TOP = the
+// largest bucket size the range holds (8, 16, 24 with 4 lanes per cell; 32
+// with 8); the chunk's first cell has the chunk's largest count.
+template <int TOP, bool MIXED, int KIND, bool LTAB, bool PACK>
+__device__ __forceinline__ void solve_range(const SolveArgs& a, int lo, int hi, long long wave, long long nwaves,
+                                            int lane, const double* ltab, unsigned ox, unsigned oxy) {
+    for (long long c = lo / SOLVE_CHUNK + wave; c * SOLVE_CHUNK < hi; c += nwaves) {
+        const int i0 = (int)(c * SOLVE_CHUNK);
+        const int iend = min(i0 + SOLVE_CHUNK, hi);
+        // a chunk starts on a cell: segments start at chunk multiples and a
+        // gap never reaches the next one
+        const int c0 = a.sorted[i0];
+        const int n0 = __builtin_amdgcn_readfirstlane(c0 >= 0 ? count_n(a.count[c0]) : 0);
+        const int nb = n0 <= 8 ? 8 : (n0 <= 16 ? 16 : (n0 <= 24 ? 24 : 32));
+        const int idx = i0 + (lane & (SOLVE_CHUNK - 1));
+        const int cell = idx < iend ? a.sorted[idx] : -1;
+        const bool full = __all(cell < 0 || count_n(a.count[cell]) == nb);
+        if constexpr (TOP == 32) {
+            solve_chunk<32, 8, MIXED, KIND, LTAB, PACK, true>(a, i0, iend, lane, full, ltab, ox, oxy);
+        } else {
+            if (nb == 8)
+                solve_chunk<8, 4, MIXED, KIND, LTAB, PACK, TOP == 8>(a, i0, iend, lane, full, ltab, ox, oxy);
+            if constexpr (TOP >= 16)
+                if (nb == 16)
+                    solve_chunk<16, 4, MIXED, KIND, LTAB, PACK, TOP == 16>(a, i0, iend, lane, full, ltab, ox, oxy);
+            if constexpr (TOP >= 24)
+                if (nb == 24)
+                    solve_chunk<24, 4, MIXED, KIND, LTAB, PACK, TOP == 24>(a, i0, iend, lane, full, ltab, ox, oxy);
+        }
+    }
+}
+
+// Range of the work list a launch covers: buckets 0-2 (TOP <= 24) or bucket
+// 3 (TOP == 32) of one path.
+template <int TOP>
+__device__ __forceinline__ void sorted_range(const SolveArgs& a, int path, int& lo, int& hi) {
+    const int* sg = a.seg + 8 * path;
+    if (TOP == 32) {
+        lo = sg[2 * 3];
+        hi = sg[2 * 3 + 1];
+    } else {
+        lo = sg[2 * 2];
+        hi = sg[2 * 0 + 1];
+    }
+}
+
+// Persistent solve over one path's range with the octant covariance table
+// staged in LDS (packed addresses).  Exits before staging when the range is
+// empty.
+template <int TOP, bool MIXED, int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_solve_sorted_lds(SolveArgs a, int nelem, int path) {
     extern __shared__ double otab_lds[];
+    int lo, hi;
+    sorted_range<TOP>(a, path, lo, hi);
+    if (lo >= hi) return;
     for (int i = threadIdx.x; i < nelem; i += BLOCK) otab_lds[i] = a.otab[i];
     __syncthreads();
-    const int t = threadIdx.x & (G - 1);
-    const long long gid = ((long long)blockIdx.x * BLOCK + threadIdx.x) / G;
-    const long long ngroups = ((long long)gridDim.x * BLOCK) / G;
+    const long long wave = ((long long)blockIdx.x * BLOCK + threadIdx.x) / 64;
+    const long long nwaves = ((long long)gridDim.x * BLOCK) / 64;
     const unsigned ox = (unsigned)(a.ex + 1), oxy = ox * (unsigned)(a.ey + 1);
-    for (long long cell = gid; cell < a.g.n; cell += ngroups) {
+    solve_range<TOP, MIXED, KIND, true, true>(a, lo, hi, wave, nwaves, threadIdx.x & 63, otab_lds, ox, oxy);
+}
+
+// The same over global tables: the +-W octant table when OCT, else the full
+// table.  The mixed form is held to 2 waves per SIMD, which it reaches
+// without spilling (the FP64 form would spill).
+template <int TOP, bool MIXED, int KIND, bool OCT>
+__global__ __launch_bounds__(256, MIXED ? 2 : 1) void k_solve_sorted(SolveArgs a, int path) {
+    int lo, hi;
+    sorted_range<TOP>(a, path, lo, hi);
+    if (lo >= hi) return;
+    const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) / 64;
+    const long long nwaves = ((long long)gridDim.x * 256) / 64;
+    const unsigned ox = (unsigned)(a.wx + 1), oxy = ox * (unsigned)(a.wy + 1);
+    solve_range<TOP, MIXED, KIND, OCT, false>(a, lo, hi, wave, nwaves, threadIdx.x & 63, a.otab_w, ox, oxy);
+}
+
+// FP64 re-solve of the mixed form's fallbacks (an unordered list): each
+// group takes its cell's own bucket size, so a cell gets the FP64 form's
+// solver and bits.  G8: the 32 bucket (8 lanes per cell), else the others.
+template <bool G8, int KIND, bool OCT>
+__global__ __launch_bounds__(256) void k_solve_fallback(SolveArgs a) {
+    constexpr int G = G8 ? 8 : 4;
+    const int t = threadIdx.x & (G - 1);
+    const long long gid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const long long ngroups = ((long long)gridDim.x * blockDim.x) / G;
+    const long long total = *a.n_list;
+    const unsigned ox = (unsigned)(a.wx + 1), oxy = ox * (unsigned)(a.wy + 1);
+    for (long long item = gid; item < total; item += ngroups) {
+        const int cell = a.cell_list[item];
         const int cnt = a.count[cell];
-        if (cnt >= 0 && (cnt & CNT_CLUSTER)) continue;
-        solve_cell<NMAX, G, MIXED, KIND, true, PACK>(a, cell, cnt, t, otab_lds, ox, oxy);
+        const int b = count_bucket(count_n(cnt));
+        if constexpr (G8) {
+            if (b == 3) solve_cell<32, 8, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
+        } else {
+            if (b == 0) solve_cell<8, 4, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
+            if (b == 1) solve_cell<16, 4, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
+            if (b == 2) solve_cell<24, 4, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
+        }
     }
 }
 

@@ -11,19 +11,22 @@
 
 namespace hpglx {
 
-// Launches the solve of one call: the LDS-table kernel over template-path
-// cells when nelem > 0 (octant table entries), the global-table kernel over
-// the cluster-path list (or over every cell), and for the mixed form the
-// FP64 re-solve of its fallbacks.  ev = start/stop events for the main
-// kernel (entries may be null).
-void dispatch_solve_8(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                      hipStream_t st);
-void dispatch_solve_16(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                       hipStream_t st);
-void dispatch_solve_24(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                       hipStream_t st);
-void dispatch_solve_32(const SolveArgs& sa, int kind, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                       hipStream_t st);
+// One launch of the count-bucketed solve (solve.h, k_solve_sorted*).
+struct SortedLaunch {
+    bool mixed = false;
+    int path = 0;                  // 0 template walk, 1 cluster search (and every cell without an LDS table)
+    int nelem = 0;                 // octant-table entries staged in LDS (path 0), 0 = global tables
+    int blocks_per_cu = 0;         // cap on the persistent grid's blocks per CU (0 = as many as fit)
+    const hipEvent_t* ev = nullptr;  // start / stop events of this launch (entries may be null)
+};
+
+// The sizes up to top = 8, 16 or 24 (4 lanes per cell), or top = 32 for the
+// 25-32 bucket (8 lanes per cell), of `kind` over one path of the work list.
+// Returns false when nothing was launched (the LDS table does not fit).
+bool solve_sorted(int top, int kind, const SolveArgs& sa, const SortedLaunch& l, hipStream_t st);
+// FP64 re-solve of the mixed form's fallbacks (sa.cell_list / sa.n_list):
+// buckets 0-2 (g8 false) or bucket 3 (g8 true).
+void solve_fallback(bool g8, int kind, const SolveArgs& sa, long long max_cells, hipStream_t st);
 
 // Systems above 32 neighbours (sa.kmax > 32): one workgroup per cell, FP64,
 // every cell of the grid; scratch from ws.  Returns 0, or -1 with err.
@@ -49,10 +52,9 @@ struct SgsLevelArgs {
     unsigned char* mask;      // informed masks (realizations x N), updated
 };
 
-// One per (bucket, kind): solve_n<NMAX>_k<KIND>.hip.
+// One unit per (bucket size, kind): solve_n<NMAX>_k<KIND>.hip.
 #define HPGLX_DECL_SOLVE(NMAX, KIND)                                                                      \
-    void dispatch_solve_##NMAX##_##KIND(const SolveArgs& sa, bool mixed, long long n, int nelem,          \
-                                        const hipEvent_t* ev, hipStream_t st);                            \
+    bool solve_sorted_##NMAX##_##KIND(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st);      \
     void sgs_level_##NMAX##_##KIND(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
 HPGLX_DECL_SOLVE(8, 0)
 HPGLX_DECL_SOLVE(8, 1)
@@ -67,5 +69,11 @@ HPGLX_DECL_SOLVE(32, 0)
 HPGLX_DECL_SOLVE(32, 1)
 HPGLX_DECL_SOLVE(32, 2)
 #undef HPGLX_DECL_SOLVE
+void solve_fallback_g4_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_g4_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_g4_2(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_g8_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_g8_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_g8_2(const SolveArgs& sa, long long max_cells, hipStream_t st);
 
 }  // namespace hpglx

@@ -6,8 +6,10 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 
 #include "solve.h"
@@ -16,116 +18,82 @@
 namespace hpglx {
 namespace {
 
-template <int NMAX, int G, bool MIXED, int KIND>
-void launch_solve(const SolveArgs& sa, long long ncells, const hipEvent_t* ev, hipStream_t st) {
-    const int block = 256;
-    const unsigned grid = (unsigned)((ncells * G + block - 1) / block);
-    if (sa.otab_w)
-        hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, true>), dim3(grid), dim3(block), 0, st, ev[0], ev[1], 0,
-                              sa);
-    else
-        hipExtLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, false>), dim3(grid), dim3(block), 0, st, ev[0], ev[1],
-                              0, sa);
-}
-
-// Global-table form over a device-sized cell list (cluster-path cells, and
-// the FP64 re-solve of the mixed form's fallbacks).
-template <int NMAX, int G, bool MIXED, int KIND>
-void launch_solve_list(const SolveArgs& sa, long long max_cells, hipStream_t st) {
-    const int block = 256;
-    long long blocks = (max_cells * G + block - 1) / block;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks < 1) blocks = 1;
-    if (sa.otab_w)
-        hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, true>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
-    else
-        hipLaunchKernelGGL((k_solve<NMAX, G, MIXED, KIND, false>), dim3((unsigned)blocks), dim3(block), 0, st, sa);
-}
-
-// Persistent LDS-table form: as many blocks as are resident at once.  Returns
-// false (nothing launched) when the table does not fit a block's LDS.
-template <int NMAX, int G, bool MIXED, int KIND, int BLOCK, bool PACK>
-bool launch_solve_lds(const SolveArgs& sa, int nelem, const hipEvent_t* ev, hipStream_t st) {
+// Persistent grid of a kernel: the blocks resident at once on every CU
+// (cached per kernel, device and LDS bytes), raising the kernel's dynamic-LDS
+// limit when needed.  0 when the kernel cannot run with that LDS.
+inline int persistent_grid(const void* fn, int block, int lds) {
     static std::mutex mu;
-    static std::map<std::pair<int, int>, int> grids;  // (device, LDS bytes) -> grid
-    const void* fn = (const void*)&k_solve_lds<NMAX, G, MIXED, KIND, BLOCK, PACK>;
-    const int lds = nelem * (int)sizeof(double);
+    static std::map<std::tuple<const void*, int, int>, int> grids;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return false;
-    int grid = 0;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = grids.find({dev, lds});
-        if (it != grids.end()) {
-            grid = it->second;
-        } else {
-            int cus = 0, nb = 0;
-            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, BLOCK, lds) != hipSuccess)
-                nb = 0;
-            (void)hipGetLastError();
-            grid = nb * cus;
-            grids[{dev, lds}] = grid;
-        }
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(fn, dev, lds);
+    auto it = grids.find(key);
+    if (it != grids.end()) return it->second;
+    int cus = 0, nb = 0;
+    if ((lds > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, lds) != hipSuccess)
+        nb = 0;
+    (void)hipGetLastError();
+    grids[key] = nb * cus;
+    return nb * cus;
+}
+
+// One launch of the bucketed solve (solve.h): the sizes up to TOP (4 lanes
+// per cell, TOP <= 24) or the 25-32 bucket (8 lanes), over one path's range
+// of the work list.  Path 0 with nelem > 0 stages the octant table in LDS;
+// otherwise the global tables serve.  Returns false when nothing could be
+// launched (the LDS table does not fit).
+template <int TOP, int KIND, bool MIXED>
+bool launch_sorted_p(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
+    const hipEvent_t e0 = l.ev ? l.ev[0] : nullptr, e1 = l.ev ? l.ev[1] : nullptr;
+    const int blocks_per_cu_cap = l.blocks_per_cu > 0 ? l.blocks_per_cu : 1 << 20;
+    int cus = 0, dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (l.nelem > 0) {
+        constexpr int BLOCK = (TOP <= 24 && MIXED) ? 512 : 256;
+        const void* fn = (const void*)&k_solve_sorted_lds<TOP, MIXED, KIND, BLOCK>;
+        const int lds = l.nelem * (int)sizeof(double);
+        int grid = persistent_grid(fn, BLOCK, lds);
+        if (grid <= 0) return false;
+        grid = std::min(grid, blocks_per_cu_cap * std::max(cus, 1));
+        HPGLX_LAUNCH_EV((k_solve_sorted_lds<TOP, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK),
+                        (uint32_t)lds, st, e0, e1, sa, l.nelem, l.path);
+        return true;
     }
-    if (grid <= 0) return false;
-    hipExtLaunchKernelGGL((k_solve_lds<NMAX, G, MIXED, KIND, BLOCK, PACK>), dim3((unsigned)grid), dim3(BLOCK),
-                          (uint32_t)lds, st, ev[0], ev[1], 0, sa, nelem);
+    if (sa.otab_w) {
+        const void* fn = (const void*)&k_solve_sorted<TOP, MIXED, KIND, true>;
+        const int grid = std::min(persistent_grid(fn, 256, 0), blocks_per_cu_cap * std::max(cus, 1));
+        HPGLX_LAUNCH_EV((k_solve_sorted<TOP, MIXED, KIND, true>), dim3((unsigned)std::max(grid, 1)), dim3(256), 0,
+                        st, e0, e1, sa, l.path);
+    } else {
+        const void* fn = (const void*)&k_solve_sorted<TOP, MIXED, KIND, false>;
+        const int grid = std::min(persistent_grid(fn, 256, 0), blocks_per_cu_cap * std::max(cus, 1));
+        HPGLX_LAUNCH_EV((k_solve_sorted<TOP, MIXED, KIND, false>), dim3((unsigned)std::max(grid, 1)), dim3(256), 0,
+                        st, e0, e1, sa, l.path);
+    }
     return true;
 }
 
-// Where the plan has an octant table: the LDS form over template-path cells,
-// then the global form over the cluster-path list; otherwise the global form
-// over every cell.  The mixed form then re-solves its fallbacks in FP64.
-// G64 / G32: lanes per cell of the FP64 and of the mixed form.
-template <int NMAX, int G64, int G32, int KIND>
-void dispatch_kind(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev, hipStream_t st) {
-    constexpr int BLOCK32 = NMAX <= 24 ? 512 : 256;
-    // A caller's own cell list (n = its largest length): the global-table
-    // form over the list only.
-    if (sa.cell_list) {
-        if (mixed)
-            launch_solve_list<NMAX, G32, true, KIND>(sa, n, st);
-        else
-            launch_solve_list<NMAX, G64, false, KIND>(sa, n, st);
-        return;
-    }
-    // Packed x/y table addresses when both 16-bit fields stay below 32768:
-    // x up to E_x * 8, y up to E_y * (E_x + 1) * 8.
-    const bool pack = (long long)sa.ex * 8 < 32768 && (long long)sa.ey * (sa.ex + 1) * 8 < 32768;
-    bool lds = false;
-    if (nelem > 0) {
-        if (mixed)
-            lds = pack ? launch_solve_lds<NMAX, G32, true, KIND, BLOCK32, true>(sa, nelem, ev, st)
-                       : launch_solve_lds<NMAX, G32, true, KIND, BLOCK32, false>(sa, nelem, ev, st);
-        else
-            lds = pack ? launch_solve_lds<NMAX, G64, false, KIND, 256, true>(sa, nelem, ev, st)
-                       : launch_solve_lds<NMAX, G64, false, KIND, 256, false>(sa, nelem, ev, st);
-    }
-    if (!lds) {
-        if (mixed)
-            launch_solve<NMAX, G32, true, KIND>(sa, n, ev, st);
-        else
-            launch_solve<NMAX, G64, false, KIND>(sa, n, ev, st);
-    }
-    if (lds) {
-        SolveArgs cl = sa;
-        cl.cell_list = sa.cl_cells;
-        cl.n_list = sa.n_cl_cells;
-        if (mixed)
-            launch_solve_list<NMAX, G32, true, KIND>(cl, n, st);
-        else
-            launch_solve_list<NMAX, G64, false, KIND>(cl, n, st);
-    }
-    if (mixed) {
-        SolveArgs fb = sa;
-        fb.cell_list = sa.fallback_cells;
-        fb.n_list = sa.n_fallback;
-        fb.fallback_cells = nullptr;
-        fb.n_fallback = nullptr;
-        launch_solve_list<NMAX, G64, false, KIND>(fb, n, st);
-    }
+template <int TOP, int KIND>
+bool launch_sorted(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
+    return l.mixed ? launch_sorted_p<TOP, KIND, true>(sa, l, st) : launch_sorted_p<TOP, KIND, false>(sa, l, st);
+}
+
+// FP64 re-solve of the mixed form's fallback list (sa.cell_list / n_list),
+// sized for up to max_cells cells.
+template <bool G8, int KIND>
+void launch_fallback(const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    constexpr int G = G8 ? 8 : 4;
+    long long blocks = (max_cells * G + 255) / 256;
+    // the list is usually empty or short: 16k groups stride over it
+    blocks = std::max(1LL, std::min(blocks, 256LL));
+    if (sa.otab_w)
+        hipLaunchKernelGGL((k_solve_fallback<G8, KIND, true>), dim3((unsigned)blocks), dim3(256), 0, st, sa);
+    else
+        hipLaunchKernelGGL((k_solve_fallback<G8, KIND, false>), dim3((unsigned)blocks), dim3(256), 0, st, sa);
 }
 
 // One level of sequential Gaussian simulation (see SgsLevelArgs): one wave

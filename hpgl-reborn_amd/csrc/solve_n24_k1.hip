@@ -1,11 +1,12 @@
-// solve_n24_k1.hip -- solve kernels for max_neighbours <= 24, kind 1 (0 OK, 1 SK, 2 median IK).
+// solve_n24_k1.hip -- simple kriging: the bucketed solve of calls with
+// max_neighbours above 16 over the buckets up to 24 neighbours (4 lanes per cell) and the
+// simulation level kernel with 24 slots.
 #include "solve_dispatch_impl.h"
 
 namespace hpglx {
 
-void dispatch_solve_24_1(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                           hipStream_t st) {
-    dispatch_kind<24, 4, 4, 1>(sa, mixed, n, nelem, ev, st);
+bool solve_sorted_24_1(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
+    return launch_sorted<24, 1>(sa, l, st);
 }
 
 void sgs_level_24_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {

@@ -1,15 +1,20 @@
-// solve_n32_k1.hip -- solve kernels for max_neighbours <= 32, kind 1 (0 OK, 1 SK, 2 median IK).
+// solve_n32_k1.hip -- simple kriging: the bucketed solve of the 25-32 neighbour
+// bucket (8 lanes per cell), the simulation level kernel with 32 slots and
+// the FP64 re-solve of mixed-precision fallbacks in that bucket.
 #include "solve_dispatch_impl.h"
 
 namespace hpglx {
 
-void dispatch_solve_32_1(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                           hipStream_t st) {
-    dispatch_kind<32, 8, 8, 1>(sa, mixed, n, nelem, ev, st);
+bool solve_sorted_32_1(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
+    return launch_sorted<32, 1>(sa, l, st);
 }
 
 void sgs_level_32_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
     launch_sgs_level<32, 8, 1>(sa, la, st);
+}
+
+void solve_fallback_g8_1(const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    launch_fallback<true, 1>(sa, max_cells, st);
 }
 
 }  // namespace hpglx

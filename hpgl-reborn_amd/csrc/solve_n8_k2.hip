@@ -1,15 +1,21 @@
-// solve_n8_k2.hip -- solve kernels for max_neighbours <= 8, kind 2 (0 OK, 1 SK, 2 median IK).
+// solve_n8_k2.hip -- median indicator kriging: the bucketed solve of calls with
+// max_neighbours <= 8 over the buckets up to 8 neighbours (4 lanes per cell) and the
+// simulation level kernel with 8 slots;
+// the FP64 re-solve of mixed-precision fallbacks up to 24 neighbours.
 #include "solve_dispatch_impl.h"
 
 namespace hpglx {
 
-void dispatch_solve_8_2(const SolveArgs& sa, bool mixed, long long n, int nelem, const hipEvent_t* ev,
-                           hipStream_t st) {
-    dispatch_kind<8, 4, 4, 2>(sa, mixed, n, nelem, ev, st);
+bool solve_sorted_8_2(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
+    return launch_sorted<8, 2>(sa, l, st);
 }
 
 void sgs_level_8_2(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
     launch_sgs_level<8, 4, 2>(sa, la, st);
+}
+
+void solve_fallback_g4_2(const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    launch_fallback<false, 2>(sa, max_cells, st);
 }
 
 }  // namespace hpglx

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(WB) void k_solve_wide(SolveArgs a, double* gscratch
         for (int k = 0; k < a.nind; ++k) {
             const float pf = (float)a.marg[k];
             const float thr = a.zfield ? a.thresholds[k] : 0.f;
-            const int cat = a.cats ? a.cats[k] : -1;
+            const int cat = a.has_cats ? a.cats[k] : -1;
             double acc = (double)pf, pe = 0.0;
             float accf = pf;
             for (int i = 0; i < n; ++i) {
@@ -246,7 +246,7 @@ int launch_wide(const SolveArgs& sa, Workspace& ws, const hipEvent_t* ev, hipStr
             return -1;
         }
         const unsigned blocks = (unsigned)std::min<long long>(n, 8192);
-        hipExtLaunchKernelGGL(k_solve_wide<KIND>, dim3(blocks), dim3(WB), (uint32_t)slot, st, ev[0], ev[1], 0, sa,
+        HPGLX_LAUNCH_EV(k_solve_wide<KIND>, dim3(blocks), dim3(WB), (uint32_t)slot, st, ev[0], ev[1], sa,
                               (double*)nullptr, kcap);
         return 0;
     }
@@ -257,7 +257,7 @@ int launch_wide(const SolveArgs& sa, Workspace& ws, const hipEvent_t* ev, hipStr
     }
     auto* scratch = (double*)ws.get("wide_scratch", (size_t)(blocks * slot), err);
     if (!err.empty()) return -1;
-    hipExtLaunchKernelGGL(k_solve_wide<KIND>, dim3((unsigned)blocks), dim3(WB), 0, st, ev[0], ev[1], 0, sa, scratch,
+    HPGLX_LAUNCH_EV(k_solve_wide<KIND>, dim3((unsigned)blocks), dim3(WB), 0, st, ev[0], ev[1], sa, scratch,
                           kcap);
     return 0;
 }

@@ -344,6 +344,11 @@ def last_call_stats():
     return {"fp64_fallbacks": fb.value, "cluster_path_cells": cl.value}
 
 
+def set_graph_mode(on):
+    """Capture repeated identical grid calls into a hipGraph and replay them."""
+    _hpgl_so.hpglx_set_graph_mode(1 if on else 0)
+
+
 def set_default_precision(precision):
     """Precision of the host-array entry points (ordinary_kriging etc.)."""
     _hpgl_so.hpglx_set_default_precision(_PREC[precision])

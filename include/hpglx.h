@@ -195,6 +195,13 @@ int hpglx_profile_summary(float* ms2, int* ncalls);
  * solver re-solved in FP64 and how many cells took the cluster path. */
 int hpglx_last_call_stats(unsigned int* fp64_fallbacks, int* cluster_path_cells);
 
+/* Graph replay of repeated grid calls: when on, a device call equal to the
+ * previous one (same parameters, geometry and buffers) is captured into a
+ * hipGraph once and replayed while the library's workspace is unchanged.
+ * Off by default (a replay measured no faster on MI355X); the environment
+ * variable HPGLX_GRAPH=1 also turns it on.  No counterpart in the reference. */
+void hpglx_set_graph_mode(int on);
+
 /* Precision used by the host-buffer entry points of hpgl.h (default FP64). */
 void hpglx_set_default_precision(int precision);
 
