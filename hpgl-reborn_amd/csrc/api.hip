@@ -188,6 +188,14 @@ struct Library {
     unsigned long long graph_clock = 0;
     hipStream_t cap_stream = nullptr;
     int graph_mode = -1;  // -1: not read yet
+    // Launch settings chosen by hpglx_autotune per (parameter set, grid,
+    // precision); HPGLX_TUNE_FILE keeps them across processes.
+    struct Tuning {
+        int search_head = 0, blocks_per_cu = 0, dyn_sched = 0;
+    };
+    std::map<std::string, Tuning> tuned;
+    bool tune_file_read = false;
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
 };
 
 Library& lib() {
@@ -469,6 +477,39 @@ int launch_call(Library& L, DevPlan& plan, const GridCall& call, hipStream_t st,
     return launch_grid(plan, L.ws, call, st, nullptr, err);
 }
 
+// ---------------------------------------------------------------- tuning
+std::string tune_key(const hpglx_grid_request_t* q) {
+    const CovParams c = to_cov(q->cov);
+    return plan_key(q->kind, c, q->radiuses, q->max_neighbours) + "|" + std::to_string(q->dims[0]) + "x" +
+           std::to_string(q->dims[1]) + "x" + std::to_string(q->dims[2]) + "|p" + std::to_string(q->precision);
+}
+
+void read_tune_file(Library& L) {
+    L.tune_file_read = true;
+    const char* path = std::getenv("HPGLX_TUNE_FILE");
+    if (!path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char line[1024];
+    while (std::fgets(line, sizeof line, f)) {
+        char* tab = std::strchr(line, '\t');
+        Library::Tuning t;
+        if (!tab || std::sscanf(tab + 1, "%d %d %d", &t.search_head, &t.blocks_per_cu, &t.dyn_sched) != 3) continue;
+        L.tuned[std::string(line, tab)] = t;
+    }
+    std::fclose(f);
+}
+
+void apply_tuning(Library& L, const hpglx_grid_request_t* q, GridCall& call) {
+    if (!L.tune_file_read) read_tune_file(L);
+    if (L.tuned.empty()) return;
+    auto it = L.tuned.find(tune_key(q));
+    if (it == L.tuned.end()) return;
+    if (!std::getenv("HPGLX_SEARCH_HEAD")) call.search_head = it->second.search_head;
+    call.blocks_per_cu = it->second.blocks_per_cu;
+    call.dyn_sched = it->second.dyn_sched;
+}
+
 int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::string& err,
                 const int* cats_host = nullptr) {
     if (!q) {
@@ -478,6 +519,7 @@ int run_request(Library& L, const hpglx_grid_request_t* q, hipStream_t st, std::
     if (!ensure_events(L, err)) return -1;
     GridCall call;
     if (!request_call(q, call, cats_host, err)) return -1;
+    apply_tuning(L, q, call);
     const CovParams c = to_cov(q->cov);
     DevPlan* plan = get_plan(L, q->kind, c, q->radiuses, q->max_neighbours, err);
     if (!plan) return -1;
@@ -1365,6 +1407,71 @@ int hpglx_krige(const hpglx_grid_request_t* req, void* stream) {
     L.last_error.clear();
     std::string err;
     if (run_request(L, req, (hipStream_t)stream, err) != 0) return fail(L, err);
+    return 0;
+}
+
+int hpglx_autotune(const hpglx_grid_request_t* req, void* stream, int reps, int* chosen, float* best_ms) {
+    Library& L = lib();
+    std::lock_guard<std::mutex> g(L.mu);
+    L.last_error.clear();
+    std::string err;
+    if (!req) return fail(L, "null request");
+    if (reps < 1) reps = 1;
+    const hipStream_t st = (hipStream_t)stream;
+    if (!ensure_events(L, err)) return fail(L, err);
+    for (auto& e : L.tune_ev)
+        if (!e && hipEventCreate(&e) != hipSuccess) return fail(L, "hipEventCreate failed");
+    GridCall base;
+    if (!request_call(req, base, nullptr, err)) return fail(L, err);
+    const CovParams c = to_cov(req->cov);
+    DevPlan* plan = get_plan(L, req->kind, c, req->radiuses, req->max_neighbours, err);
+    if (!plan) return fail(L, err);
+    if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess) return fail(L, "hipStreamWaitEvent failed");
+    // The candidates: template entries staged by the search, persistent
+    // blocks per CU of the solve, fixed or dynamic dealing of its chunks.
+    const int heads[] = {0, 512, 256};
+    const int bpcs[] = {0, 1};
+    Library::Tuning best;
+    float best_t = 3.0e38f;
+    for (int head : heads)
+        for (int bpc : bpcs)
+            for (int dyn = 0; dyn < 2; ++dyn) {
+                GridCall call = base;
+                call.search_head = head;
+                call.blocks_per_cu = bpc;
+                call.dyn_sched = dyn;
+                if (launch_grid(*plan, L.ws, call, st, nullptr, err) != 0) return fail(L, err);  // warm
+                if (hipEventRecord(L.tune_ev[0], st) != hipSuccess) return fail(L, "hipEventRecord failed");
+                for (int r = 0; r < reps; ++r)
+                    if (launch_grid(*plan, L.ws, call, st, nullptr, err) != 0) return fail(L, err);
+                float ms = 0.f;
+                if (hipEventRecord(L.tune_ev[1], st) != hipSuccess || hipEventSynchronize(L.tune_ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, L.tune_ev[0], L.tune_ev[1]) != hipSuccess)
+                    return fail(L, "timing a candidate failed");
+                ms /= (float)reps;
+                if (ms < best_t) {
+                    best_t = ms;
+                    best.search_head = head;
+                    best.blocks_per_cu = bpc;
+                    best.dyn_sched = dyn;
+                }
+            }
+    if (hipEventRecord(L.done, st) != hipSuccess) return fail(L, "hipEventRecord failed");
+    if (!L.tune_file_read) read_tune_file(L);
+    const std::string key = tune_key(req);
+    L.tuned[key] = best;
+    if (const char* path = std::getenv("HPGLX_TUNE_FILE")) {
+        if (FILE* f = std::fopen(path, "a")) {
+            std::fprintf(f, "%s\t%d %d %d\n", key.c_str(), best.search_head, best.blocks_per_cu, best.dyn_sched);
+            std::fclose(f);
+        }
+    }
+    if (chosen) {
+        chosen[0] = best.search_head;
+        chosen[1] = best.blocks_per_cu;
+        chosen[2] = best.dyn_sched;
+    }
+    if (best_ms) *best_ms = best_t;
     return 0;
 }
 

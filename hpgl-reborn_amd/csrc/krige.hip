@@ -1440,17 +1440,20 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         sl.nelem = path == 0 ? nelem : 0;
         const bool main_path = path == (lds ? 0 : 1);
         sl.ev = (main_path && !g8) ? ev_t + 2 : nullptr;
+        SolveArgs sp = so;
+        sp.sched = c.dyn_sched ? counters + CTR_SCHED + 2 * path : nullptr;
         // where the LDS form cannot run, the global tables serve the same range
-        if (!solve_sorted(top, c.kind, so, sl, st)) {
+        if (!solve_sorted(top, c.kind, sp, sl, st)) {
             sl.nelem = 0;
-            solve_sorted(top, c.kind, so, sl, st);
+            solve_sorted(top, c.kind, sp, sl, st);
         }
         if (g8) {
             sl.nelem = path == 0 ? nelem : 0;
             sl.ev = main_path ? ev_t + 2 : nullptr;
-            if (!solve_sorted(32, c.kind, so, sl, st)) {
+            sp.sched = c.dyn_sched ? counters + CTR_SCHED + 2 * path + 1 : nullptr;
+            if (!solve_sorted(32, c.kind, sp, sl, st)) {
                 sl.nelem = 0;
-                solve_sorted(32, c.kind, so, sl, st);
+                solve_sorted(32, c.kind, sp, sl, st);
             }
         }
     }

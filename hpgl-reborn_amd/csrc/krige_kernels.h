@@ -129,6 +129,7 @@ struct SolveArgs {
     int* n_fallback;               // its length (device counter)
     const int* sorted;             // work list ordered by path and count bucket (k_bucket_scatter), -1 = gap
     const int* seg;                // its segments: [begin, end) of (path, bucket) at seg[2 * (4 path + b)]
+    int* sched;                    // dynamic chunk dealing: the launch's chunk counter (null: fixed stride)
 };
 
 // Count buckets of the solve: the register solver sizes 8, 16, 24 and 32
@@ -150,6 +151,7 @@ enum {
     CTR_N_OVF2 = 5,
     CTR_SEG = 16,        // (path, bucket) segments of the work list: 2 x 4 x [begin, end)
     CTR_BIN_TOTAL = 32,  // cells per work-list bin (NBINS)
+    CTR_SCHED = 100,     // chunk counters of the solve launches (dynamic dealing): 2 paths x 2 sizes
     CTR_COUNT = 128      // ints cleared per call
 };
 constexpr int NBINS = 66;  // work-list bins: 2 paths x 33 counts

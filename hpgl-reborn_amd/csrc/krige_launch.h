@@ -96,6 +96,7 @@ struct GridCall {
     int cluster_cap = 0;                    // kept candidates per lane of the LDS cluster search (0 = default)
     int cluster_lanes = 0;                  // lanes of the full-table cluster search (multiple of 256; 0 = default)
     int blocks_per_cu = 0;                  // cap on the persistent solve grids' blocks per CU (0 = all that fit)
+    int dyn_sched = 0;                      // solve chunks dealt from a counter instead of a fixed stride
     bool profile = false;
 };
 

@@ -891,10 +891,12 @@ __device__ __forceinline__ void solve_chunk(const SolveArgs& a, int i0, int iend
 // Chunks of [lo, hi) for one wave, strided over the grid's waves.  TOP = the
 // largest bucket size the range holds (8, 16, 24 with 4 lanes per cell; 32
 // with 8); the chunk's first cell has the chunk's largest count.
+// With a.sched set, a wave's first chunk is its own and later ones come from
+// that counter (dynamic dealing) instead of the fixed stride.
 template <int TOP, bool MIXED, int KIND, bool LTAB, bool PACK>
 __device__ __forceinline__ void solve_range(const SolveArgs& a, int lo, int hi, long long wave, long long nwaves,
                                             int lane, const double* ltab, unsigned ox, unsigned oxy) {
-    for (long long c = lo / SOLVE_CHUNK + wave; c * SOLVE_CHUNK < hi; c += nwaves) {
+    for (long long c = lo / SOLVE_CHUNK + wave; c * SOLVE_CHUNK < hi;) {
         const int i0 = (int)(c * SOLVE_CHUNK);
         const int iend = min(i0 + SOLVE_CHUNK, hi);
         // a chunk starts on a cell: segments start at chunk multiples and a
@@ -916,6 +918,13 @@ __device__ __forceinline__ void solve_range(const SolveArgs& a, int lo, int hi, 
             if constexpr (TOP >= 24)
                 if (nb == 24)
                     solve_chunk<24, 4, MIXED, KIND, LTAB, PACK, TOP == 24>(a, i0, iend, lane, full, ltab, ox, oxy);
+        }
+        if (a.sched) {
+            int next = 0;
+            if (lane == 0) next = atomicAdd(a.sched, 1);
+            c = lo / SOLVE_CHUNK + nwaves + __shfl(next, 0);
+        } else {
+            c += nwaves;
         }
     }
 }

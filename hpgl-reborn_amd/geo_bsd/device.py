@@ -52,6 +52,30 @@ def _call(q, stream):
         raise RuntimeError("hpglx_krige: " + _hpgl_so.hpglx_last_error().decode())
 
 
+def autotune(*args, reps=5, **kw):
+    """Times the launch variants of one krige() request (same arguments) and
+    keeps the fastest for later calls with the same parameters, grid and
+    precision.  Returns (outputs, choice) with choice = dict(search_head,
+    blocks_per_cu, dynamic, ms)."""
+    global _call
+    chosen = (C.c_int * 3)()
+    best = C.c_float(0)
+
+    def tune(q, stream):
+        rc = _hpgl_so.hpglx_autotune(C.byref(q), C.c_void_p(stream), int(reps), chosen, C.byref(best))
+        if rc != 0:
+            raise RuntimeError("hpglx_autotune: " + _hpgl_so.hpglx_last_error().decode())
+
+    saved = _call
+    _call = tune
+    try:
+        out = krige(*args, **kw)
+    finally:
+        _call = saved
+    return out, dict(search_head=chosen[0] or 1024, blocks_per_cu=chosen[1], dynamic=bool(chosen[2]),
+                     ms=float(best.value))
+
+
 def _dims(grid):
     if hasattr(grid, "x"):
         return (int(grid.x), int(grid.y), int(grid.z))

@@ -70,6 +70,18 @@ typedef struct hpglx_grid_request_t {
  * `stream` (a hipStream_t; null = the default stream). */
 int hpglx_krige(const hpglx_grid_request_t* req, void* stream);
 
+/* Launch settings for a request, found by timing (no counterpart in the
+ * reference): runs the request `reps` times (plus one warm-up) for each
+ * candidate -- template entries the neighbour search stages in LDS (1024,
+ * 512, 256), persistent blocks per CU of the solve (all that fit, 1), fixed
+ * or dynamic dealing of its 16-cell chunks -- and keeps the fastest for
+ * later calls with the same parameter set, grid and precision (and in the
+ * file named by HPGLX_TUNE_FILE, read at the first call, when set).  The
+ * request's outputs hold the last run's results.  Synchronises the stream.
+ * chosen (may be null) receives {staged entries (0 = 1024), blocks per CU
+ * (0 = all), dynamic}; best_ms the fastest candidate's mean time. */
+int hpglx_autotune(const hpglx_grid_request_t* req, void* stream, int reps, int* chosen, float* best_ms);
+
 /* Size of the allocation-free entry point's scratch, per GPU. */
 #define HPGLX_NOALLOC_SCRATCH_MIB 128
 
