@@ -50,13 +50,13 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (packed)
 # known size (--calibrate-copy; profiles/hbm_traffic_ok_100x100x50.md):
 # FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, KiB x 1024.
 TRAFFIC_BYTES = {
-    "k_solve_lds<24,4,mixed,OK>": 71.81e6,
-    "k_solve_lds<24,4,fp64,OK>": 70.85e6,
+    "k_solve_sorted_lds<24,4,mixed,OK>": 72.68e6,
+    "k_solve_sorted_lds<24,4,fp64,OK>": 72.84e6,
 }
-# Algorithmic bytes of one solve launch: per kriged cell 4 B count + 96 B
-# offsets in, 13 B estimate/mask/variance out; counts of the informed cells,
+# Algorithmic bytes of one solve launch: per kriged cell its work-list entry
+# and count (8 B) and 96 B of offsets in, 13 B estimate/mask/variance out;
 # the value array and the octant table once.
-ALGO_BYTES = 475000 * 113 + 25000 * 4 + 500000 * 4 + 121 * 1024
+ALGO_BYTES = 475000 * 117 + 500000 * 4 + 121 * 1024
 
 
 def flops_ok(n):
@@ -66,6 +66,21 @@ def flops_ok(n):
     n = n.astype(np.float64)
     e = n * (n + 1) / 2 + n + 1
     return 25 * e + n ** 3 / 3 + n ** 2 / 2 + n / 6 + 4 * n ** 2 + 10 * n
+
+
+def flops_mik(n, nind):
+    """Algorithmic flops of one median-IK cell with n neighbours and nind
+    thresholds: covariance evaluations + Cholesky + one solve + tail."""
+    import numpy as np
+    n = n.astype(np.float64)
+    e = n * (n + 1) / 2 + n + 1
+    return 25 * e + n ** 3 / 3 + n ** 2 / 2 + n / 6 + 2 * n ** 2 + 5 * n + 3 * nind * n
+
+
+def main_kernel_roofline(flops, ms, peak, kernel):
+    achieved = flops / (ms / 1e3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "flops_per_launch": flops, "launch_ms": ms}
 
 
 def cpu_baseline(wl):
@@ -279,14 +294,47 @@ def side_workload(name, args):
             ref = lambda: O.ref_lvm_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, wl.lvm_mean,
                                             threads=th)
             sample = "reference lvm_kriging, full grid"
+        # neighbour counts and search paths (untimed) for the main solve
+        # kernel's algorithmic flops: with the LDS table (median IK, LVM) the
+        # template-path cells of every bucket; without it (wells) every cell
+        # of the 25-32 bucket, which the 8-lane kernel solves
+        nbk = dict(zfield=z, thresholds=wl.thresholds, marginal=wl.levels) if name == "mik" else dict(values=v)
+        nb = D.krige(2 if name == "mik" else (1 if name == "lvm" else 0), wl.shape, wl.radiuses, wl.k, model, m,
+                     neighbours=True, variance=False, **nbk)
+        sync()
+        counts, cl = nb["count"].cpu().numpy(), nb["cluster_path"].cpu().numpy()
+        cnt = np.maximum(counts, 0)
+        if name == "wells":
+            main = cnt >= 25
+            kernel = "k_solve_sorted<32,8,{},OK>"
+        else:
+            main = (counts > 0) & ~cl
+            kernel = "k_solve_sorted_lds<{},4,{{}},{}>".format(16 if name == "mik" else 24,
+                                                                "MIK" if name == "mik" else "SK")
+        flops = float((flops_mik(cnt[main], len(wl.thresholds)) if name == "mik" else flops_ok(cnt[main])).sum())
+        kern_ms, tuning = {}, {}
         for prec in ("fp64", "mixed"):
+            if name == "wells" and not args.no_autotune:
+                _, tuning[prec] = D.autotune(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
+                                             variance=False, reps=2)
             per[prec] = _timed(lambda: call(prec), args.steps, args.warmup, sync)
+            D.set_profiling(True)
+            _timed(lambda: call(prec), args.steps, 0, sync)
+            D.set_profiling(False)
+            kern_ms[prec] = D.profile_summary()
         best = min(per, key=per.get)
         line.update(metric=f"kriged cells/s on 1 MI355X ({name})", value=units / per[best], unit="kriged cells/s",
                     ms_per_step=per[best] * 1e3, dtype="f64" if best == "fp64" else "f32+f64",
                     config={"workload": wl.name, "cells": int(wl.mask.size), "kriged_cells": units,
                             "best_precision": best},
-                    fp64_ms=per["fp64"] * 1e3, mixed_ms=per["mixed"] * 1e3)
+                    fp64_ms=per["fp64"] * 1e3, mixed_ms=per["mixed"] * 1e3,
+                    kernels_ms={p: {"search": k["search"], "solve_main": k["solve_main"]} for p, k in kern_ms.items()},
+                    roofline=main_kernel_roofline(flops, kern_ms[best]["solve_main"],
+                                                  PEAK_FP64_TFLOPS if best == "fp64" else PEAK_FP32_TFLOPS,
+                                                  kernel.format("fp64" if best == "fp64" else "mixed")),
+                    main_kernel_cells=int(main.sum()))
+        if name == "wells":
+            line["autotune"] = tuning
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = _cpu("reference", ref, units, "kriged cells/s", sample)
     elif name == "ik":
@@ -463,6 +511,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-loo", action="store_true", help="skip the leave-one-out kriging figures")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch settings (profiled runs: no tuning trials among the kernels)")
     ap.add_argument("--workload", default="ok", choices=["ok", "mik", "wells", "ik", "lvm", "io", "variogram", "sgs"],
                     help="ok (default): the headline OK 100x100x50 workload; others: see the module docstring")
     ap.add_argument("--realizations", type=int, default=32,
@@ -506,8 +556,9 @@ def main():
     tuning = {}
     for prec in ("fp64", "mixed"):
         # launch settings for this request, found by timing (kept for the steps)
-        _, tuning[prec] = D.autotune(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
-                                     variance=True, reps=3)
+        if not args.no_autotune:
+            _, tuning[prec] = D.autotune(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
+                                         variance=True, reps=3)
         for _ in range(args.warmup):
             D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec, variance=True)
         torch.cuda.synchronize()
@@ -555,8 +606,8 @@ def main():
 
     mixed = results["mixed"]
     fp64 = results["fp64"]
-    k_mixed = "k_solve_lds<24,4,mixed,OK>"
-    k_fp64 = "k_solve_lds<24,4,fp64,OK>"
+    k_mixed = "k_solve_sorted_lds<24,4,mixed,OK>"
+    k_fp64 = "k_solve_sorted_lds<24,4,fp64,OK>"
     line = {
         "metric": "kriged cells/s on 1 MI355X: FP32+refine vs FP64, at <=1e-6 estimate error",
         "value": kriged / (mixed["ms"] / 1e3),

@@ -795,8 +795,17 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
              std::vector<unsigned char>& m, const unsigned char* simmask, const hpgl_non_parametric_cdf_t* cdf,
              const float* lvm_host, const long* seeds, int nreal, float* d_v, unsigned char* d_m, hipStream_t st,
              std::string& err) {
+    // Every check comes before the schedules are built and any copy is queued.
     if (p->m_kriging_kind != KIND_OK && p->m_kriging_kind != KIND_SK) {
         err = "sequential simulation takes simple (1) or ordinary (0) kriging";
+        return -1;
+    }
+    if (p->m_max_neighbours > 32) {
+        err = "max_neighbours above 32 is not supported by the GPU simulation";
+        return -1;
+    }
+    if (lvm_host && p->m_kriging_kind != KIND_SK) {
+        err = "a locally varying mean applies to simple kriging only";
         return -1;
     }
     CovParams c;
@@ -817,10 +826,6 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
     std::vector<float> lvm;
     float* d_lvm = nullptr;
     if (lvm_host) {
-        if (p->m_kriging_kind != KIND_SK) {
-            err = "a locally varying mean applies to simple kriging only";
-            return -1;
-        }
         lvm.assign(lvm_host, lvm_host + n);
         if (transform) sgs_means_to_normal_scores(lvm.data(), n, cdf->m_values, cdf->m_probs, cdf->m_size);
         d_lvm = (float*)L.ws.get("sgs_io_lvm", (size_t)n * 4, err);
@@ -858,6 +863,7 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
     if (hipStreamWaitEvent(st, L.done, 0) != hipSuccess ||
         (d_lvm && hipMemcpyAsync(d_lvm, lvm.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess)) {
         err = "host to device copy failed";
+        (void)hipStreamSynchronize(st);  // a queued copy must not outlive the host arrays
         return -1;
     }
     // the data once from the host, then on the device into every other slice
@@ -868,12 +874,16 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
         if (hipMemcpyAsync(d_v + (long long)r * n, sv, (size_t)n * 4, kind, st) != hipSuccess ||
             hipMemcpyAsync(d_m + (long long)r * n, sm, (size_t)n, kind, st) != hipSuccess) {
             err = "host to device copy failed";
+            (void)hipStreamSynchronize(st);
             return -1;
         }
     }
-    // sgs_run synchronises st before returning, so the host arrays above
-    // outlive their copies.
-    return sgs_run(*plan, L.ws, p->m_kriging_kind, dims, d_v, d_m, d_lvm, sched, st, err);
+    // sgs_run synchronises st before returning on success; on an error the
+    // stream is synchronised here, so the host arrays above outlive their
+    // copies either way.
+    const int rc = sgs_run(*plan, L.ws, p->m_kriging_kind, dims, d_v, d_m, d_lvm, sched, st, err);
+    if (rc != 0) (void)hipStreamSynchronize(st);
+    return rc;
 }
 
 // One realization over host copies of the property: v and m (N) are updated

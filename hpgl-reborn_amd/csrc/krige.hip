@@ -1111,6 +1111,15 @@ void* Workspace::get(const std::string& name, size_t bytes, std::string& err) {
     return p;
 }
 
+void Workspace::drop(const std::string& name) {
+    auto it = bufs.find(name);
+    if (it == bufs.end() || arena) return;
+    (void)hipFree(it->second.first);
+    bufs.erase(it);
+    tags.erase(name);
+    ++gen;
+}
+
 void Workspace::release() {
     ++gen;
     for (auto& kv : bufs) (void)hipFree(kv.second.first);

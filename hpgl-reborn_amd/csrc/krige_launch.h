@@ -57,6 +57,7 @@ struct Workspace {
         bufs.clear();
         arena_used = 0;
     }
+    void drop(const std::string& name);  // frees one heap-mode buffer
     void release();
 };
 

@@ -234,9 +234,39 @@ __global__ __launch_bounds__(WB) void k_solve_wide(SolveArgs a, double* gscratch
     }
 }
 
+// Largest neighbour count of the grid (into *out, zeroed by the first lane).
+__global__ __launch_bounds__(256) void k_count_max(const int* count, long long n, int* out) {
+    __shared__ int wmax[4];
+    int m = 0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        m = max(m, count_n(count[i]));
+    for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+// Beyond this the wide solver's global slots are given back after the call.
+constexpr size_t kWideKeepBytes = 256u << 20;
+
 template <int KIND>
 int launch_wide(const SolveArgs& sa, Workspace& ws, const hipEvent_t* ev, hipStream_t st, std::string& err) {
-    const int kcap = sa.kmax;
+    // The working sets are sized by the largest neighbour count the search
+    // found (read back once), not by max_neighbours: a grid smaller than
+    // max_neighbours gives every system fewer rows.
+    int* dmax = (int*)ws.get("wide_max", 4, err);
+    if (!dmax) return -1;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, dmax);
+    hipLaunchKernelGGL(k_count_max, dim3(256), dim3(256), 0, st, sa.count, sa.g.n, dmax);
+    int found = 0;
+    if (hipMemcpyAsync(&found, dmax, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        err = "reading the largest neighbour count failed";
+        return -1;
+    }
+    const int kcap = std::max(1, std::min(found, sa.kmax));
     const long long slot = wide_slot(kcap) * 8;
     const long long n = sa.g.n;
     if (slot <= kWideLdsBytes) {
@@ -259,6 +289,13 @@ int launch_wide(const SolveArgs& sa, Workspace& ws, const hipEvent_t* ev, hipStr
     if (!err.empty()) return -1;
     HPGLX_LAUNCH_EV(k_solve_wide<KIND>, dim3((unsigned)blocks), dim3(WB), 0, st, ev[0], ev[1], sa, scratch,
                           kcap);
+    if ((size_t)(blocks * slot) > kWideKeepBytes && !ws.arena) {
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            err = "the wide solver failed";
+            return -1;
+        }
+        ws.drop("wide_scratch");
+    }
     return 0;
 }
 

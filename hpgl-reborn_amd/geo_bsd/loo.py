@@ -16,6 +16,7 @@ wrapped in a torch.autograd.Function.
     loss = loo_loss(params, xyz, values, nbr)                    (P,)
 """
 import ctypes as C
+import weakref
 
 import torch
 
@@ -45,12 +46,45 @@ def _points(xyz, values):
     return xyz.to(torch.float32).contiguous(), values.to(torch.float32).contiguous()
 
 
+# Neighbour tensors whose indices were checked, by id: a weak reference to the
+# tensor and its version counter when checked.  Reading min and max waits for
+# the device, so a tensor is checked once per content, not once per forward
+# (the fitting loop passes the same tensor every step).
+_nbr_checked = {}
+
+
 def _check_nbr(nbr, n):
     if nbr.dim() != 2 or nbr.shape[0] != n:
         raise ValueError("nbr must be (n, k)")
+    seen = _nbr_checked.get(id(nbr))
+    if seen is not None and seen[0]() is nbr and seen[1] == nbr._version and seen[2] == n:
+        return
     lo, hi = int(nbr.min()), int(nbr.max())
     if lo < 0 or hi >= n:
         raise ValueError("neighbour indices must lie in [0, n)")
+    if len(_nbr_checked) > 64:
+        _nbr_checked.clear()
+    _nbr_checked[id(nbr)] = (weakref.ref(nbr), nbr._version, n)
+
+
+# Free forward-state buffers by (device, elements).  The state of 64 parameter
+# sets x 20,000 points is about 4 GB; the backward hands its buffer back here
+# and the next forward on the same stream reuses it instead of allocating
+# one of that size again.
+_state_pool = {}
+
+
+def _take_state(numel, dev):
+    free = _state_pool.get((dev, numel))
+    if free:
+        return free.pop()
+    return torch.empty(numel, dtype=torch.float64, device=dev)
+
+
+def _give_state(t):
+    free = _state_pool.setdefault((t.device, t.numel()), [])
+    if len(free) < 2:
+        free.append(t)
 
 
 def knn(xyz, k):
@@ -85,7 +119,7 @@ class LooKriging(torch.autograd.Function):
         nbytes = _hpgl_so.hpglx_loo_state_size(n, k, p)
         if nbytes == 0:
             raise ValueError("invalid sizes for leave-one-out kriging")
-        state = torch.empty(nbytes // 8, dtype=torch.float64, device=dev)
+        state = _take_state(nbytes // 8, dev)
         m = MODELS[model]
         _check(_hpgl_so.hpglx_loo_forward(_ptr(xyz), _ptr(values), n, _ptr(nbr), k, _ptr(params), p, m, _ptr(est),
                                           _ptr(status), _ptr(state), _stream(dev)), "loo forward")
@@ -104,6 +138,7 @@ class LooKriging(torch.autograd.Function):
         _check(_hpgl_so.hpglx_loo_backward(_ptr(xyz), _ptr(values), n, _ptr(nbr), k, _ptr(params), p, ctx.model,
                                            _ptr(state), _ptr(grad_est), _ptr(grad), None, _stream(xyz.device)),
                "loo backward")
+        _give_state(state)  # later work on this stream is ordered after the kernel that reads it
         return grad, None, None, None, None
 
 
@@ -155,6 +190,8 @@ def fit(params0, xyz, values, nbr, model="spherical", steps=200, lr=0.05, min_ra
         ranges = u[:, 2:].exp() + min_range
         return torch.cat([sill[:, None], nug[:, None], ranges], dim=1)
 
+    # one int32 tensor for every step, so its indices are checked once
+    nbr = nbr.to(torch.int32).contiguous()
     opt = torch.optim.Adam([u], lr=lr)
     hist = []
     for _ in range(steps):

@@ -150,3 +150,32 @@ def test_loo_estimate_equals_grid_oracle_ordinary_kriging(code, model):
         est = LO.estimates(np.array([prm]), xyz, values_f[allnodes], nbr, model)[0, -1]
         want = o["est64"][target]
         assert est == pytest.approx(want, rel=1e-10, abs=1e-12), (trial, est, want)
+
+
+def test_neighbour_check_runs_once_per_content():
+    """The index check of a neighbour tensor (two host reads) runs once per
+    tensor content: again after an in-place change, which a bad index then
+    fails."""
+    import torch
+    from geo_bsd import loo as L
+    nbr = torch.tensor([[1, 2], [0, 2], [0, 1]], dtype=torch.int32)
+    calls = []
+    orig_min = torch.Tensor.min
+
+    def counting_min(self, *a, **k):
+        calls.append(1)
+        return orig_min(self, *a, **k)
+
+    torch.Tensor.min = counting_min
+    try:
+        L._check_nbr(nbr, 3)
+        L._check_nbr(nbr, 3)
+        assert len(calls) == 1
+        nbr[0, 0] = 7  # in place: the version changes
+        with pytest.raises(ValueError, match=r"\[0, n\)"):
+            L._check_nbr(nbr, 3)
+        assert len(calls) == 2
+        with pytest.raises(ValueError):
+            L._check_nbr(nbr[:2], 3)
+    finally:
+        torch.Tensor.min = orig_min
