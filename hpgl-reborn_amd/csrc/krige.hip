@@ -233,41 +233,71 @@ __global__ void k_tdelta(GridDims g, int tsize, int tpad, const int* tdx, const 
 }
 
 // --------------------------------------------------------- tiled template walk
-// One block per TILE_X x TILE_Y tile of one z slice, one lane per cell.  The
-// block copies the padded-mask box its cells can reach (tile + 2r halo) and
-// the packed offsets of the first SEARCH_HEAD template entries into LDS.
-// Every lane then walks the template in reference order, 32 entries per step:
-// the step's box deltas are wave-uniform and come through scalar loads, each
-// probe is one LDS byte read, and the 32 results form a bit mask whose set
-// bits (ctz order) are the hits until max_neighbours.  Slices are dispatched
-// from both grid faces inwards, because cells near a z face need the most
-// probes and would otherwise finish last.  Cells whose 27 clusters are all
-// below the limit are handed to the cluster path instead.
+// One block per TILE_X x TILE_Y tile of `nslab` consecutive z slices (256
+// lanes per slice, one lane per cell).  The block copies the padded-mask box
+// its cells can reach (tile + 2r halo, nslab + 2 rz slices deep) and the
+// packed offsets of the first SEARCH_HEAD template entries into LDS.  Every
+// lane then walks the template in reference order, 32 entries per step: the
+// step's box deltas are wave-uniform and come through scalar loads (the next
+// step's are loaded while this step's probes are in flight), each probe is
+// one LDS byte read, two probes share a register (low and high half), and the
+// 32 results form a bit mask whose set bits (ctz order) are the hits until
+// max_neighbours.  Slice groups are dispatched from both grid faces inwards,
+// because cells near a z face need the most probes and would otherwise finish
+// last.  Cells whose 27 clusters are all below the limit are handed to the
+// cluster path instead.  Several slices per block share one box: its halo in
+// z is paid once per block, so more cells fit in LDS at a time.
 constexpr int TILE_X = 16, TILE_Y = 16;
+constexpr int TILE_CELLS = TILE_X * TILE_Y;
 constexpr int TMPL_U = 32;       // probes per step; template arrays are padded to a multiple (plus one more)
 constexpr int LOAD_BATCH = 8;    // loads in flight per thread while filling LDS
 constexpr int SEARCH_HEAD = 1024;  // packed offsets staged in LDS
+constexpr int SEARCH_MAX_SLABS = 4;  // z slices per block (1024 lanes)
 constexpr size_t kSearchLdsMax = 160 * 1024;  // one workgroup's LDS
 
 typedef const int __attribute__((address_space(4)))* const_int_ptr;
+typedef unsigned short probe_pair __attribute__((ext_vector_type(2)));
 
-// head = number of staged packed offsets (<= SEARCH_HEAD).
-__global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int head) {
+// LDS bytes of a search block: the box, the head of packed offsets and, when
+// staged, 16-bit hit indices (max_neighbours per lane) and per-lane counts.
+static inline size_t search_tile_lds(int lx, int ly, int rz, int nslab, int K, bool staged) {
+    size_t b = ((size_t)lx * ly * (nslab + 2 * rz) + 15) / 16 * 16 + (size_t)SEARCH_HEAD * 4;
+    if (staged) b += ((size_t)TILE_CELLS * nslab * K * 2 + 15) / 16 * 16 + (size_t)TILE_CELLS * nslab * 4;
+    return b;
+}
+
+// head = number of staged packed offsets (<= SEARCH_HEAD); blockDim.x =
+// 256 * nslab.
+//
+// STAGE (max_neighbours <= 32, template shorter than 65,536 entries): a lane
+// keeps its hits in LDS as 16-bit template indices, and at the end each wave
+// writes its four 16-cell rows' neighbour lists -- one contiguous run of
+// 16 max_neighbours ints per row -- with consecutive lanes on consecutive
+// ints.  Without it every hit is a 4-byte store of its own (after an LDS read
+// of its packed offset), and a wave's store touches 64 cache lines (the
+// lanes' lists are max_neighbours ints apart).
+template <bool STAGE>
+__global__ __launch_bounds__(TILE_CELLS * SEARCH_MAX_SLABS) void k_search_tile(SearchArgs a, int head) {
     extern __shared__ unsigned int lds_dw[];
     const GridDims& g = a.g;
     const int qx = a.lx >> 2;  // dwords per box row
     const int nrows = a.ly * a.lz;
+    const int nslab = (int)blockDim.x / TILE_CELLS;
+    const int nlanes = (int)blockDim.x;
     const unsigned char* box = (const unsigned char*)lds_dw;
     int* tpk = (int*)(lds_dw + ((qx * nrows + 3) & ~3));
+    unsigned short* hits = (unsigned short*)(tpk + ((SEARCH_HEAD + 3) & ~3));
+    // per-lane counts after the hits (16-byte aligned, as the launcher sizes them)
+    int* hitn = (int*)((unsigned char*)hits + ((nlanes * a.kmax * 2 + 15) & ~15));
     const int tid = threadIdx.x;
     const int nzb = (int)gridDim.z;
     const int bz = (blockIdx.z & 1) ? nzb - 1 - (int)(blockIdx.z >> 1) : (int)(blockIdx.z >> 1);
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = bz;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = bz * nslab;
 
     // Box origin is padded (x0, y0, z0) = cell (x0 - rx, y0 - ry, z0 - rz).
     // Thread -> (dword column xq, row); rows advance by rstep per pass.
     {
-        const int rstep = blockDim.x / qx;  // qx <= 256 by the launcher
+        const int rstep = nlanes / qx;  // qx <= 256 by the launcher
         const int xq = tid % qx;
         int row = tid / qx;
         const bool lane_ok = row < rstep;
@@ -299,14 +329,15 @@ __global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int head) {
         }
     }
     head = min(head, a.tsize);
-    for (int i = tid; i < head; i += blockDim.x) tpk[i] = a.tpack[i];
+    for (int i = tid; i < head; i += nlanes) tpk[i] = a.tpack[i];
     __syncthreads();
 
-    const int tx = tid % TILE_X, ty = tid / TILE_X;
-    const int x = x0 + tx, y = y0 + ty, z = z0;
-    const bool inside = x < g.nx && y < g.ny;
+    const int slab = tid / TILE_CELLS, tc = tid % TILE_CELLS;
+    const int tx = tc % TILE_X, ty = tc / TILE_X;
+    const int x = x0 + tx, y = y0 + ty, z = z0 + slab;
+    const bool inside = x < g.nx && y < g.ny && z < g.nz;
     const long long cell = inside ? ((long long)z * g.ny + y) * g.nx + x : 0;
-    const int lcell = (g.rz * a.ly + ty + g.ry) * a.lx + tx + g.rx;  // the cell's own box byte
+    const int lcell = ((g.rz + slab) * a.ly + ty + g.ry) * a.lx + tx + g.rx;  // the cell's own box byte
     bool active = false;
     if (inside) {
         const bool needs = a.kind == 2 || !box[lcell];
@@ -320,32 +351,53 @@ __global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int head) {
             a.cl_cells[pos] = (int)cell;
         }
     }
-    // Inactive lanes probe around the box position of tile cell (0, 0), which
-    // keeps every template delta inside the box; their bits are discarded.
+    // Inactive lanes probe around the box position of cell (0, 0) of the
+    // first slice, which keeps every template delta inside the box; their
+    // bits are discarded.
     const int lbase = active ? lcell : (g.rz * a.ly + g.ry) * a.lx + g.rx;
     const const_int_ptr tdel = (const_int_ptr)a.tdelta_lds;
     int cnt = 0;
     const int K = a.kmax;
     int* out = a.nboff + (active ? cell * K : 0);
+    // This step's deltas; the template arrays hold one padding step past the
+    // last, so the load of the step after the last stays in bounds.
+    int dl[TMPL_U];
+#pragma unroll
+    for (int u = 0; u < TMPL_U; ++u) dl[u] = tdel[u];
     for (int t0 = 0; t0 < a.tsize; t0 += TMPL_U) {
         const bool want = active && cnt < K;
         if (!__any(want)) break;
-        const int tu = __builtin_amdgcn_readfirstlane(t0);
-        // 32 probes into a bit mask (mask bytes are 0 or 1), no branches.
-        // Every probe of the step is issued before the first is used, so the
-        // step waits for the LDS once.
-        unsigned pv[TMPL_U];
+        // 32 probes, two per register, issued before the first is used, so
+        // the step waits for the LDS once; the next step's deltas load
+        // meanwhile.
+        probe_pair pv[TMPL_U / 2];
 #pragma unroll
-        for (int u = 0; u < TMPL_U; ++u) pv[u] = box[lbase + tdel[tu + u]];
+        for (int u = 0; u < TMPL_U / 2; ++u) {
+            probe_pair p;
+            p.x = box[lbase + dl[u]];
+            p.y = box[lbase + dl[u + TMPL_U / 2]];
+            pv[u] = p;
+        }
         __builtin_amdgcn_sched_barrier(0);
+        const int tn = __builtin_amdgcn_readfirstlane(t0 + TMPL_U);
+#pragma unroll
+        for (int u = 0; u < TMPL_U; ++u) dl[u] = tdel[tn + u];
+        __builtin_amdgcn_sched_barrier(0);
+        // mask bytes are 0 or 1: pair u sets bits u and u + 16
         unsigned bits = 0;
 #pragma unroll
-        for (int u = 0; u < TMPL_U; ++u) bits |= pv[u] << u;
+        for (int u = 0; u < TMPL_U / 2; ++u) bits |= __builtin_bit_cast(unsigned, pv[u]) << u;
         const int left = a.tsize - t0;
         if (left < TMPL_U) bits &= (1u << left) - 1u;
         if (!want) bits = 0;
         // Hits in template order until max_neighbours.
-        if (t0 + TMPL_U <= head) {
+        if (STAGE) {
+            while (bits != 0 && cnt < K) {
+                const int u = __builtin_ctz(bits);
+                hits[tid * K + cnt++] = (unsigned short)(t0 + u);
+                bits &= bits - 1u;
+            }
+        } else if (t0 + TMPL_U <= head) {
             while (bits != 0 && cnt < K) {
                 const int u = __builtin_ctz(bits);
                 out[cnt++] = tpk[t0 + u];
@@ -360,6 +412,39 @@ __global__ __launch_bounds__(256) void k_search_tile(SearchArgs a, int head) {
         }
     }
     if (active) a.count[cell] = cnt;
+    if (STAGE) {
+        hitn[tid] = active ? cnt : 0;
+        // The wave's cells are rows 4 (w mod 4) .. + 3 of slice w / 4 (16
+        // cells each); a row's lists are one run of 16 K ints in nboff.
+        // Slots past a cell's count, and the lists of cells the walk skipped
+        // (informed cells; cluster-path cells, whose lists the cluster search
+        // writes later), are written as 0.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int lane = tid & 63, w = tid >> 6;
+        const int wz = z0 + (w >> 2), wrow = 4 * (w & 3);
+        const int nvalid = min(TILE_X, g.nx - x0);
+        // i / K for i < 16 K <= 512 and K <= 32: (i m) >> 16 with m = 65536 / K + 1
+        const unsigned m = 65536u / (unsigned)K + 1u;
+        for (int r = 0; r < 4 && wz < g.nz; ++r) {
+            const int yy = y0 + wrow + r;
+            if (yy >= g.ny) break;
+            int* dst = a.nboff + (((long long)wz * g.ny + yy) * g.nx + x0) * K;
+            const int src0 = (w >> 2) * TILE_CELLS + (wrow + r) * TILE_X;
+            for (int i = lane; i < nvalid * K; i += 64) {
+                const int cx = (int)(((unsigned)i * m) >> 16);
+                const int slot = i - cx * K;
+                // slots past the count are written as 0; every index read was staged
+                int v = 0;
+                if (slot < hitn[src0 + cx]) {
+                    const int idx = hits[(src0 + cx) * K + slot];
+                    v = idx < head ? tpk[idx] : a.tpack[idx];
+                }
+                dst[i] = v;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------ template walk
@@ -1212,10 +1297,36 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     const int node_cap = (int)std::max<long long>(1, std::min<long long>(h.limit, n));
     auto* cl_nodes = (int*)ws.get("cl_nodes", (size_t)ncl * node_cap * 4, err);
     // Tiled search: LDS = mask box (tile + 2r halo) + the template's first
-    // SEARCH_HEAD packed offsets.
-    const int lx = (TILE_X + 2 * g.rx + 3) / 4 * 4, ly = TILE_Y + 2 * g.ry, lz = 1 + 2 * g.rz;
-    const size_t tile_lds = ((size_t)lx * ly * lz + 15) / 16 * 16 + (size_t)SEARCH_HEAD * 4;
-    const bool tiled = tile_lds <= kSearchLdsMax && lx / 4 <= 256 && c.search_variant != 1;
+    // SEARCH_HEAD packed offsets (+ the staged hits).  The slices per block
+    // are the count that keeps the most waves resident on a CU (160 KiB of
+    // LDS, allocated in 512-byte units; at most 32 waves), the smaller count
+    // on ties.
+    const int lx = (TILE_X + 2 * g.rx + 3) / 4 * 4, ly = TILE_Y + 2 * g.ry;
+    const bool staged_ok = K <= 32 && T < 65536 && c.search_variant != 2;
+    auto pick_slabs = [&](bool stg) {
+        int best_waves = 0, ns = 0;
+        for (int s = 1; s <= SEARCH_MAX_SLABS; ++s) {
+            if (c.search_slabs > 0 && s != c.search_slabs) continue;
+            const size_t b = search_tile_lds(lx, ly, g.rz, s, K, stg);
+            if (b > kSearchLdsMax) continue;
+            const int blocks = std::min<int>((int)(kSearchLdsMax / ((b + 511) / 512 * 512)), 32 / (4 * s));
+            if (blocks * 4 * s > best_waves) {
+                best_waves = blocks * 4 * s;
+                ns = s;
+            }
+        }
+        return ns;
+    };
+    int nslab = 0;
+    bool staged = false;
+    if (lx / 4 <= 256 && c.search_variant != 1) {
+        if (staged_ok) nslab = pick_slabs(true);
+        staged = nslab > 0;
+        if (!staged) nslab = pick_slabs(false);
+    }
+    const bool tiled = nslab > 0;
+    const int lz = (tiled ? nslab : 1) + 2 * g.rz;
+    const size_t tile_lds = tiled ? search_tile_lds(lx, ly, g.rz, nslab, K, staged) : 0;
     auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
     auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
     auto* cl_cells = (int*)ws.get("cl_cells", (size_t)n * 4, err);
@@ -1310,11 +1421,15 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     sa.cl_cells = cl_cells;
     sa.n_cl_cells = n_cl_cells;
     if (tiled) {
-        if (tile_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_tile, (int)tile_lds));
-        const dim3 grid((c.nx + TILE_X - 1) / TILE_X, (c.ny + TILE_Y - 1) / TILE_Y, c.nz);
+        const void* fn = staged ? (const void*)&k_search_tile<true> : (const void*)&k_search_tile<false>;
+        if (tile_lds > 65536) HPGLX_CHECK(allow_lds(fn, (int)tile_lds));
+        const dim3 grid((c.nx + TILE_X - 1) / TILE_X, (c.ny + TILE_Y - 1) / TILE_Y, (c.nz + nslab - 1) / nslab);
+        const dim3 block(TILE_CELLS * nslab);
         const int head = c.search_head > 0 && c.search_head < SEARCH_HEAD ? c.search_head : SEARCH_HEAD;
-        HPGLX_LAUNCH_EV(k_search_tile, grid, dim3(TILE_X * TILE_Y), (uint32_t)tile_lds, st, ev_t[0], ev_t[1], sa,
-                        head);
+        if (staged)
+            HPGLX_LAUNCH_EV(k_search_tile<true>, grid, block, (uint32_t)tile_lds, st, ev_t[0], ev_t[1], sa, head);
+        else
+            HPGLX_LAUNCH_EV(k_search_tile<false>, grid, block, (uint32_t)tile_lds, st, ev_t[0], ev_t[1], sa, head);
     } else {
         HPGLX_LAUNCH_EV(k_search_tmpl, dim3(blocks_for(n, B)), dim3(B), 0, st, ev_t[0], ev_t[1], sa);
     }

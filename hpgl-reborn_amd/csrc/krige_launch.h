@@ -91,9 +91,11 @@ struct GridCall {
     int* out_nboff = nullptr;               // debug: packed offsets (device, N x kmax)
     double pivot_tol = 1e-3;
     int refine_steps = 1;
-    int search_variant = 0;                 // 0 = LDS tiles when the box fits, 1 = global walk
+    int search_variant = 0;                 // 0 = LDS tiles when the box fits (hits staged in LDS when
+                                            // max_neighbours <= 32), 1 = global walk, 2 = LDS tiles, per-hit stores
     int solve_variant = 0;                  // 0 = LDS / octant tables, 1 = full table only, 2 = no LDS table
     int search_head = 0;                    // packed offsets staged in LDS by the search (0 = default)
+    int search_slabs = 0;                   // z slices per LDS search block (0 = the most resident waves)
     int cluster_cap = 0;                    // kept candidates per lane of the LDS cluster search (0 = default)
     int cluster_lanes = 0;                  // lanes of the full-table cluster search (multiple of 256; 0 = default)
     int blocks_per_cu = 0;                  // cap on the persistent solve grids' blocks per CU (0 = all that fit)
