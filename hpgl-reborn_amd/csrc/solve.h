@@ -253,58 +253,6 @@ struct Chol {
         }
     }
 
-    // Two right-hand sides at once: independent chains interleaved.
-    template <typename V>
-    static __device__ __forceinline__ void forward2(const T (&L)[TOT], const T (&D)[S], V (&z)[S], V (&u)[S],
-                                                    int t) {
-#pragma unroll
-        for (int j = 0; j < NMAX; ++j) {
-            const int q = j % G, sj = j / G;
-            const V zl = z[sj] * (V)D[sj], ul = u[sj] * (V)D[sj];
-            const V zj = gbc<G>(zl, q), uj = gbc<G>(ul, q);
-#pragma unroll
-            for (int s = sj; s < S; ++s) {
-                const V lv = (V)L[slot_off<G>(s) + j];
-                const V zu = fmadd(lv, zj, z[s]), uu = fmadd(lv, uj, u[s]);
-                if (s == sj) {
-                    z[s] = (t == q) ? zl : ((t > q) ? zu : z[s]);
-                    u[s] = (t == q) ? ul : ((t > q) ? uu : u[s]);
-                } else {
-                    z[s] = zu;
-                    u[s] = uu;
-                }
-            }
-        }
-    }
-
-    template <typename V>
-    static __device__ __forceinline__ void backward2(const T (&L)[TOT], const T (&D)[S], V (&x)[S], V (&y)[S],
-                                                     int t) {
-#pragma unroll
-        for (int i = NMAX - 1; i >= 0; --i) {
-            const int q = i % G, si = i / G;
-            V px = V(0), py = V(0);
-#pragma unroll
-            for (int s = si; s < S; ++s) {
-                const V lv = (V)L[slot_off<G>(s) + i];
-                const V ax = lv * x[s], ay = lv * y[s];
-                if (s == si) {
-                    px += (t > q) ? ax : V(0);
-                    py += (t > q) ? ay : V(0);
-                } else {
-                    px += ax;
-                    py += ay;
-                }
-            }
-            px = gsum<G>(px);
-            py = gsum<G>(py);
-            if (t == q) {
-                x[si] = (x[si] + px) * (V)D[si];
-                y[si] = (y[si] + py) * (V)D[si];
-            }
-        }
-    }
-
     // Clears the entries on and above the diagonal of each slot's diagonal
     // tile (the factor leaves them with values it never reads; the diagonal
     // itself lives on as D).  The solves below then need no per-lane tests:
@@ -366,28 +314,6 @@ struct Chol {
             for (int s = si; s < S; ++s) p += (V)L[slot_off<G>(s) + i] * x[s];
             p = gsum<G>(p);
             if (t == q) x[si] = (x[si] + p) * (V)D[si];
-        }
-    }
-
-    template <typename V>
-    static __device__ __forceinline__ void backward2_cu(const T (&L)[TOT], const T (&D)[S], V (&x)[S], V (&y)[S],
-                                                        int t) {
-#pragma unroll
-        for (int i = NMAX - 1; i >= 0; --i) {
-            const int q = i % G, si = i / G;
-            V px = V(0), py = V(0);
-#pragma unroll
-            for (int s = si; s < S; ++s) {
-                const V lv = (V)L[slot_off<G>(s) + i];
-                px += lv * x[s];
-                py += lv * y[s];
-            }
-            px = gsum<G>(px);
-            py = gsum<G>(py);
-            if (t == q) {
-                x[si] = (x[si] + px) * (V)D[si];
-                y[si] = (y[si] + py) * (V)D[si];
-            }
         }
     }
 
@@ -550,22 +476,38 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         return true;
     }
     CH::clear_upper(L, t);
-    double sv[S], ov[S];
+    T u[S];  // OK: L^-1 1
+    double sum_ones = 0.0;
     if (OKK) {
-        // A s = b and A o = 1 side by side
-        T z[S], u[S];
+        // With z = L^-1 b and u = L^-1 1 (one forward pass, two right-hand
+        // sides): 1'A^-1 b = u.z and 1'A^-1 1 = u.u, so the Schur complement
+        // needs no second backward pass: mu = (u.z - 1) / u.u and
+        // w = A^-1 (b - mu 1) = L^-T (z - mu u).
+        T z[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             z[s] = (T)bq[s];
             u[s] = (FULL || G * s + t < n) ? T(1) : T(0);
         }
         CH::forward2_cu(L, D, z, u);
-        CH::backward2_cu(L, D, z, u, t);
+        double pzu = 0.0, puu = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            sv[s] = (double)z[s];
-            ov[s] = (double)u[s];
+            pzu = fmadd((double)z[s], (double)u[s], pzu);
+            puu = fmadd((double)u[s], (double)u[s], puu);
         }
+        const double sum_sk = gsum<G>(pzu);
+        sum_ones = gsum<G>(puu);
+        if (fabs(sum_ones) < 1e-15) {
+            status = ST_SINGULAR;
+            return true;
+        }
+        mu = (sum_sk - 1) / sum_ones;
+#pragma unroll
+        for (int s = 0; s < S; ++s) z[s] = (T)((double)z[s] - mu * (double)u[s]);
+        CH::backward_cu(L, D, z, t);
+#pragma unroll
+        for (int s = 0; s < S; ++s) w[s] = (double)z[s];
     } else {
         T z[S];
 #pragma unroll
@@ -573,118 +515,70 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
         CH::forward_cu(L, D, z);
         CH::backward_cu(L, D, z, t);
 #pragma unroll
-        for (int s = 0; s < S; ++s) sv[s] = (double)z[s];
+        for (int s = 0; s < S; ++s) w[s] = (double)z[s];
     }
-    // Weights from the (possibly float) solutions.
-    if (OKK) {
-        double ps = 0.0, po = 0.0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            ps += sv[s];
-            po += ov[s];
-        }
-        const double sum_sk = gsum<G>(ps), sum_ones = gsum<G>(po);
-        if (fabs(sum_ones) < 1e-15) {
-            status = ST_SINGULAR;
-            return true;
-        }
-        mu = (sum_sk - 1) / sum_ones;
-#pragma unroll
-        for (int s = 0; s < S; ++s) w[s] = sv[s] - mu * ov[s];
-        if (MIXED) {
-            // Refinement on the bordered system [A 1; 1' 0] [w; mu] = [b; 1]:
-            // r = b - A w - mu, r_mu = 1 - sum w in FP64; A z = r with the
-            // float factor; dmu = (sum z - r_mu) / sum o, dw = z - dmu o.
-            for (int it = 0; it < a.refine_steps; ++it) {
-                double r[S];
+    if (MIXED) {
+        // Refinement in FP64 residuals with the float factor.  SK: r = b - A w,
+        // w += A^-1 r.  OK, on the bordered system [A 1; 1' 0] [w; mu] =
+        // [b; 1]: r = b - A w - mu, r_mu = 1 - sum w; with f = L^-1 r,
+        // dmu = (u.f - r_mu) / u.u and dw = A^-1 (r - dmu 1) = L^-T (f - dmu u).
+        for (int it = 0; it < a.refine_steps; ++it) {
+            double r[S];
+            double r_mu = 0.0;
+            if (OKK) {
                 double pw = 0.0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     r[s] = bq[s] - mu;
                     pw += w[s];
                 }
-                const double r_mu = 1.0 - gsum<G>(pw);
-#pragma unroll
-                for (int c = 0; c < NMAX; ++c) {
-                    int tcol = 0;
-                    unsigned cx = 0, cy = 0, cz = 0;
-                    if (LTAB) {
-                        cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
-                        cy = PACK ? 0u : (unsigned)gbc<G>((int)uy[c / G], c % G);
-                        cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
-                    } else {
-                        tcol = gbc<G>(tr[c / G], c % G);
-                    }
-                    const double xw = gbc<G>(w[c / G], c % G);
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        // dead columns carry w = 0; dead rows are cleared below
-                        const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[center + tr[s] - tcol];
-                        r[s] = __builtin_fma(-av, xw, r[s]);
-                    }
-                    // bound the loads the scheduler keeps in flight
-                    if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
-                }
-                if (!FULL) {
-#pragma unroll
-                    for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
-                }
-                T z[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) z[s] = (T)r[s];
-                CH::forward_cu(L, D, z);
-                CH::backward_cu(L, D, z, t);
-                double pz = 0.0;
-#pragma unroll
-                for (int s = 0; s < S; ++s) pz += (double)z[s];
-                const double dmu = (gsum<G>(pz) - r_mu) / sum_ones;
-#pragma unroll
-                for (int s = 0; s < S; ++s) w[s] += (double)z[s] - dmu * ov[s];
-                mu += dmu;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int s = 0; s < S; ++s) w[s] = sv[s];
-        if (MIXED) {
-            // r = b - A w in FP64, A z = r with the float factor, w += z.
-            for (int it = 0; it < a.refine_steps; ++it) {
-                double r[S];
+                r_mu = 1.0 - gsum<G>(pw);
+            } else {
 #pragma unroll
                 for (int s = 0; s < S; ++s) r[s] = bq[s];
-#pragma unroll
-                for (int c = 0; c < NMAX; ++c) {
-                    int tcol = 0;
-                    unsigned cx = 0, cy = 0, cz = 0;
-                    if (LTAB) {
-                        cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
-                        cy = PACK ? 0u : (unsigned)gbc<G>((int)uy[c / G], c % G);
-                        cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
-                    } else {
-                        tcol = gbc<G>(tr[c / G], c % G);
-                    }
-                    const double xw = gbc<G>(w[c / G], c % G);
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        // dead columns carry w = 0; dead rows are cleared below
-                        const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[center + tr[s] - tcol];
-                        r[s] = __builtin_fma(-av, xw, r[s]);
-                    }
-                    // bound the loads the scheduler keeps in flight
-                    if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
-                }
-                if (!FULL) {
-#pragma unroll
-                    for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
-                }
-                T z[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) z[s] = (T)r[s];
-                CH::forward_cu(L, D, z);
-                CH::backward_cu(L, D, z, t);
-#pragma unroll
-                for (int s = 0; s < S; ++s) w[s] += (double)z[s];
             }
+            // r -= A w over every column (dead columns carry w = 0; dead rows
+            // are cleared below).
+#pragma unroll
+            for (int c = 0; c < NMAX; ++c) {
+                int tcol = 0;
+                unsigned cx = 0, cy = 0, cz = 0;
+                if (LTAB) {
+                    cx = (unsigned)gbc<G>((int)ux[c / G], c % G);
+                    cy = PACK ? 0u : (unsigned)gbc<G>((int)uy[c / G], c % G);
+                    cz = (unsigned)gbc<G>((int)uz[c / G], c % G);
+                } else {
+                    tcol = gbc<G>(tr[c / G], c % G);
+                }
+                const double xw = gbc<G>(w[c / G], c % G);
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const double av = LTAB ? lcov(s, cx, cy, cz) : a.ctab[center + tr[s] - tcol];
+                    r[s] = __builtin_fma(-av, xw, r[s]);
+                }
+                // bound the loads the scheduler keeps in flight
+                if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+            }
+            if (!FULL) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) r[s] = (G * s + t < n) ? r[s] : 0.0;
+            }
+            T f[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) f[s] = (T)r[s];
+            CH::forward_cu(L, D, f);
+            if (OKK) {
+                double pf = 0.0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) pf = fmadd((double)u[s], (double)f[s], pf);
+                const double dmu = (gsum<G>(pf) - r_mu) / sum_ones;
+#pragma unroll
+                for (int s = 0; s < S; ++s) f[s] = (T)((double)f[s] - dmu * (double)u[s]);
+                mu += dmu;
+            }
+            CH::backward_cu(L, D, f, t);
+#pragma unroll
+            for (int s = 0; s < S; ++s) w[s] += (double)f[s];
         }
     }
     return true;
