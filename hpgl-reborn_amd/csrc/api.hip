@@ -833,7 +833,10 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
         if (!err.empty()) return -1;
     }
     // The schedules are independent host work: one thread per realization,
-    // at most 16 at a time.
+    // at most 16 at a time.  A cell's search reaches only the template's
+    // offsets, so the levels separate cells within the template's reach (not
+    // the whole radius box).
+    const int reach[3] = {plan->h.H[0], plan->h.H[1], plan->h.H[2]};
     std::vector<SgsSchedule> sched((size_t)nreal);
     std::vector<std::string> errs((size_t)nreal);
     std::vector<char> ok((size_t)nreal, 0);
@@ -841,7 +844,7 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
     auto work = [&](int w) {
         for (int r = w; r < nreal; r += workers) {
             try {
-                ok[(size_t)r] = sgs_schedule(dims, p->m_radiuses, m.data(), simmask, seeds[r], sched[(size_t)r],
+                ok[(size_t)r] = sgs_schedule(dims, reach, m.data(), simmask, seeds[r], sched[(size_t)r],
                                              errs[(size_t)r], nreal == 1);
             } catch (const std::exception& e) {  // out of host memory: reported, not fatal
                 errs[(size_t)r] = std::string("sequential simulation schedule: ") + e.what();

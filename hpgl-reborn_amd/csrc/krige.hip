@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_cluster_build(GridDims g, const unsigne
             const int x = xa + lx, y = ya + ly, z = za + lz;
             cell = ((long long)z * g.ny + y) * g.nx + x;
             inf = mask[cell] != 0;
-            pmask[((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx)] = inf ? 1 : 0;
+            pmask[((long long)(z + g.hz) * g.py + (y + g.hy)) * g.px + (x + g.hx)] = inf ? 1 : 0;
         }
         // Box order (z, y, x) is ascending cell order, so ranks by a
         // block-wide prefix count give the ascending list.
@@ -235,7 +235,7 @@ __global__ void k_tdelta(GridDims g, int tsize, int tpad, const int* tdx, const 
 // --------------------------------------------------------- tiled template walk
 // One block per TILE_X x TILE_Y tile of `nslab` consecutive z slices (256
 // lanes per slice, one lane per cell).  The block copies the padded-mask box
-// its cells can reach (tile + 2r halo, nslab + 2 rz slices deep) and the
+// its cells can reach (tile + 2h halo, nslab + 2 hz slices deep) and the
 // packed offsets of the first SEARCH_HEAD template entries into LDS.  Every
 // lane then walks the template in reference order, 32 entries per step: the
 // step's box deltas are wave-uniform and come through scalar loads (the next
@@ -260,8 +260,8 @@ typedef unsigned short probe_pair __attribute__((ext_vector_type(2)));
 
 // LDS bytes of a search block: the box, the head of packed offsets and, when
 // staged, 16-bit hit indices (max_neighbours per lane) and per-lane counts.
-static inline size_t search_tile_lds(int lx, int ly, int rz, int nslab, int K, bool staged) {
-    size_t b = ((size_t)lx * ly * (nslab + 2 * rz) + 15) / 16 * 16 + (size_t)SEARCH_HEAD * 4;
+static inline size_t search_tile_lds(int lx, int ly, int hz, int nslab, int K, bool staged) {
+    size_t b = ((size_t)lx * ly * (nslab + 2 * hz) + 15) / 16 * 16 + (size_t)SEARCH_HEAD * 4;
     if (staged) b += ((size_t)TILE_CELLS * nslab * K * 2 + 15) / 16 * 16 + (size_t)TILE_CELLS * nslab * 4;
     return b;
 }
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(TILE_CELLS * SEARCH_MAX_SLABS) void k_search_tile(S
     const int bz = (blockIdx.z & 1) ? nzb - 1 - (int)(blockIdx.z >> 1) : (int)(blockIdx.z >> 1);
     const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = bz * nslab;
 
-    // Box origin is padded (x0, y0, z0) = cell (x0 - rx, y0 - ry, z0 - rz).
+    // Box origin is padded (x0, y0, z0) = cell (x0 - hx, y0 - hy, z0 - hz).
     // Thread -> (dword column xq, row); rows advance by rstep per pass.
     {
         const int rstep = nlanes / qx;  // qx <= 256 by the launcher
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(TILE_CELLS * SEARCH_MAX_SLABS) void k_search_tile(S
     const int x = x0 + tx, y = y0 + ty, z = z0 + slab;
     const bool inside = x < g.nx && y < g.ny && z < g.nz;
     const long long cell = inside ? ((long long)z * g.ny + y) * g.nx + x : 0;
-    const int lcell = ((g.rz + slab) * a.ly + ty + g.ry) * a.lx + tx + g.rx;  // the cell's own box byte
+    const int lcell = ((g.hz + slab) * a.ly + ty + g.hy) * a.lx + tx + g.hx;  // the cell's own box byte
     bool active = false;
     if (inside) {
         const bool needs = a.kind == 2 || !box[lcell];
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(TILE_CELLS * SEARCH_MAX_SLABS) void k_search_tile(S
     // Inactive lanes probe around the box position of cell (0, 0) of the
     // first slice, which keeps every template delta inside the box; their
     // bits are discarded.
-    const int lbase = active ? lcell : (g.rz * a.ly + g.ry) * a.lx + g.rx;
+    const int lbase = active ? lcell : (g.hz * a.ly + g.hy) * a.lx + g.hx;
     const const_int_ptr tdel = (const_int_ptr)a.tdelta_lds;
     int cnt = 0;
     const int K = a.kmax;
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
     bool active = false;
     // Inactive lanes probe around grid cell (0, 0, 0): every template delta
     // stays inside the padded mask; their bits are discarded.
-    long long pbase = ((long long)g.rz * g.py + g.ry) * g.px + g.rx;
+    long long pbase = ((long long)g.hz * g.py + g.hy) * g.px + g.hx;
     if (cell < g.n) {
         const bool needs = a.kind == 2 || !a.mask[cell];
         if (!needs) {
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
             cell_xyz(g, cell, x, y, z);
             if (a.cl_nbhd_exceeded[cluster_of(g, x, y, z)]) {
                 active = true;
-                pbase = ((long long)(z + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
+                pbase = ((long long)(z + g.hz) * g.py + (y + g.hy)) * g.px + (x + g.hx);
             } else {
                 a.count[cell] = CNT_PENDING_CLUSTER;
                 int pos = atomicAdd(a.n_cl_cells, 1);
@@ -562,8 +562,10 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
                         int nx_, ny_, nz_;
                         cell_xyz(g, nd, nx_, ny_, nz_);
                         const int dx = nx_ - x, dy = ny_ - y, dz = nz_ - z;
-                        const double key = a.ctab[center + ((long long)dz * SY + dy) * SX + dx];
-                        if (key > a.threshold) {
+                        // beyond the table the covariance is below C(0)/100
+                        const bool in = abs(dx) <= a.wx && abs(dy) <= a.wy && abs(dz) <= a.wz;
+                        const double key = in ? a.ctab[center + ((long long)dz * SY + dy) * SX + dx] : 0.0;
+                        if (in && key > a.threshold) {
                             if (m < a.capacity) {
                                 LaneStore::Item it;
                                 it.key = key;
@@ -630,7 +632,7 @@ struct ClusterOctArgs {
     int node_cap;
     const unsigned char* nbhd_exceeded;
     const unsigned* rank_w;        // octant rank table over |d| <= W
-    int wx1, wy1;                  // its x and y extents (W + 1)
+    int wx1, wy1, wz1;             // its extents (W + 1)
     int qbits;                     // bits of the candidate index in a sort word
     int cap1;                      // sort buffer entries per lane
     int* count;
@@ -701,7 +703,9 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
                     const unsigned p = cand[q];
                     const int dx = (int)(p & 2047u) - x, dy = (int)((p >> 11) & 2047u) - y, dz = (int)(p >> 22) - z;
                     const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy, az = dz < 0 ? -dz : dz;
-                    rk[u] = a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax];
+                    // beyond the table the covariance is below C(0)/100 (rank 0)
+                    const bool in = ax < a.wx1 && ay < a.wy1 && az < a.wz1;
+                    rk[u] = in ? a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax] : 0u;
                 }
 #pragma unroll
                 for (int u = 0; u < CL_GATHER; ++u) {
@@ -777,7 +781,8 @@ __global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(Clust
                             int nx_, ny_, nz_;
                             cell_xyz(g, nd[u], nx_, ny_, nz_);
                             const int ax = abs(nx_ - x), ay = abs(ny_ - y), az = abs(nz_ - z);
-                            rk[u] = a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax];
+                            const bool in = ax < a.wx1 && ay < a.wy1 && az < a.wz1;
+                            rk[u] = in ? a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax] : 0u;
                         }
 #pragma unroll
                         for (int u = 0; u < CL_GATHER; ++u) {
@@ -1272,9 +1277,12 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     g.rx = h.r[0];
     g.ry = h.r[1];
     g.rz = h.r[2];
-    g.px = (c.nx + 2 * g.rx + 15) / 16 * 16;  // row pitch: 16-byte aligned rows
-    g.py = c.ny + 2 * g.ry;
-    g.pz = c.nz + 2 * g.rz;
+    g.hx = h.H[0];
+    g.hy = h.H[1];
+    g.hz = h.H[2];
+    g.px = (c.nx + 2 * g.hx + 15) / 16 * 16;  // row pitch: 16-byte aligned rows
+    g.py = c.ny + 2 * g.hy;
+    g.pz = c.nz + 2 * g.hz;
     // clusterizer.cpp:87-89: g / r + 2, in double, truncated.
     g.cx = (int)(c.nx / (double)g.rx + 2);
     g.cy = (int)(c.ny / (double)g.ry + 2);
@@ -1301,13 +1309,13 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     // are the count that keeps the most waves resident on a CU (160 KiB of
     // LDS, allocated in 512-byte units; at most 32 waves), the smaller count
     // on ties.
-    const int lx = (TILE_X + 2 * g.rx + 3) / 4 * 4, ly = TILE_Y + 2 * g.ry;
+    const int lx = (TILE_X + 2 * g.hx + 3) / 4 * 4, ly = TILE_Y + 2 * g.hy;
     const bool staged_ok = K <= 32 && T < 65536 && c.search_variant != 2;
     auto pick_slabs = [&](bool stg) {
         int best_waves = 0, ns = 0;
         for (int s = 1; s <= SEARCH_MAX_SLABS; ++s) {
             if (c.search_slabs > 0 && s != c.search_slabs) continue;
-            const size_t b = search_tile_lds(lx, ly, g.rz, s, K, stg);
+            const size_t b = search_tile_lds(lx, ly, g.hz, s, K, stg);
             if (b > kSearchLdsMax) continue;
             const int blocks = std::min<int>((int)(kSearchLdsMax / ((b + 511) / 512 * 512)), 32 / (4 * s));
             if (blocks * 4 * s > best_waves) {
@@ -1325,8 +1333,8 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         if (!staged) nslab = pick_slabs(false);
     }
     const bool tiled = nslab > 0;
-    const int lz = (tiled ? nslab : 1) + 2 * g.rz;
-    const size_t tile_lds = tiled ? search_tile_lds(lx, ly, g.rz, nslab, K, staged) : 0;
+    const int lz = (tiled ? nslab : 1) + 2 * g.hz;
+    const size_t tile_lds = tiled ? search_tile_lds(lx, ly, g.hz, nslab, K, staged) : 0;
     auto* count = c.out_count ? c.out_count : (int*)ws.get("count", (size_t)n * 4, err);
     auto* nboff = c.out_nboff ? c.out_nboff : (int*)ws.get("nboff", (size_t)n * K * 4, err);
     auto* cl_cells = (int*)ws.get("cl_cells", (size_t)n * 4, err);
@@ -1375,8 +1383,8 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     // The padded mask's halo stays zero between calls that share its buffer
     // and geometry; only then is the whole buffer cleared.
     const std::string pm_sig = std::to_string((uintptr_t)pmask) + ":" + std::to_string(g.px) + ":" +
-                               std::to_string(g.py) + ":" + std::to_string(g.pz) + ":" + std::to_string(g.rx) + ":" +
-                               std::to_string(g.ry) + ":" + std::to_string(g.rz);
+                               std::to_string(g.py) + ":" + std::to_string(g.pz) + ":" + std::to_string(g.hx) + ":" +
+                               std::to_string(g.hy) + ":" + std::to_string(g.hz);
     if (ws.tags["pmask"] != pm_sig) {
         hipLaunchKernelGGL(k_fill_u8, dim3(std::min<long long>(4096, blocks_for(P, B))), dim3(B), 0, st, pmask, P,
                            (unsigned char)0);
@@ -1466,6 +1474,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         oa.rank_w = plan.rank_w;
         oa.wx1 = h.W[0] + 1;
         oa.wy1 = h.W[1] + 1;
+        oa.wz1 = h.W[2] + 1;
         oa.qbits = qbits;
         oa.cap1 = cap1;
         oa.count = count;

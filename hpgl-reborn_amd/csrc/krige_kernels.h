@@ -42,7 +42,8 @@ struct GridDims {
     int nx, ny, nz;
     long long n;
     int rx, ry, rz;          // search radiuses (= cluster sizes)
-    int px, py, pz;          // padded mask dims (n + 2r)
+    int hx, hy, hz;          // padded mask halo: the template's reach (<= r)
+    int px, py, pz;          // padded mask dims (n + 2h)
     int cx, cy, cz;          // cluster box dims (n / r + 2)
 };
 

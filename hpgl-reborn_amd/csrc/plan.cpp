@@ -86,6 +86,64 @@ double model_cov(const CovParams& p, const double t[9], double v0, double v1, do
 
 double PlanHost::c0() const { return model_cov(cov, transform, 0, 0, 0); }
 
+namespace {
+// Reach of the model: a distance h_r (in the units of h in model_cov) such
+// that every offset d with C(d) > C(0)/100 has |T d| < h_r.  The model
+// decreases in h beyond the h < 1e-4 branch, so h_r is found by bisection;
+// infinity when C stays above C(0)/100 (no bound).
+double cov_reach(const CovParams& p) {
+    const double thr = p.sill / 100;
+    auto f = [&](double h) {
+        if (h < 0.0001) return p.sill;
+        if (p.type == 2) return (p.sill - p.nugget) * exp(-3 * pow(h / p.ranges[0], 2));
+        if (p.type == 1) return (p.sill - p.nugget) * exp(-3 * h / p.ranges[0]);
+        if (h > p.ranges[0]) return 0.0;
+        return (p.sill - p.nugget) * (1 - 1.5 * h / p.ranges[0] + 0.5 * pow(h / p.ranges[0], 3));
+    };
+    double lo = 0.0001;
+    if (!(f(lo) > thr)) return lo;
+    double hi = 2 * lo;
+    while (f(hi) > thr) {
+        hi *= 2;
+        if (hi > 1e15) return HUGE_VAL;
+    }
+    for (int i = 0; i < 200; ++i) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break;
+        if (f(mid) > thr)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return hi;
+}
+
+// Largest |d_i| over the ellipsoid |T d| < reach: reach * |row i of T^-1|,
+// rounded up with a margin; -1 when there is no bound.
+void reach_extents(const double t[9], double reach, long long ext[3]) {
+    const double det = t[0] * (t[4] * t[8] - t[5] * t[7]) - t[1] * (t[3] * t[8] - t[5] * t[6]) +
+                       t[2] * (t[3] * t[7] - t[4] * t[6]);
+    double inv[9];
+    inv[0] = t[4] * t[8] - t[5] * t[7];
+    inv[1] = t[2] * t[7] - t[1] * t[8];
+    inv[2] = t[1] * t[5] - t[2] * t[4];
+    inv[3] = t[5] * t[6] - t[3] * t[8];
+    inv[4] = t[0] * t[8] - t[2] * t[6];
+    inv[5] = t[2] * t[3] - t[0] * t[5];
+    inv[6] = t[3] * t[7] - t[4] * t[6];
+    inv[7] = t[1] * t[6] - t[0] * t[7];
+    inv[8] = t[0] * t[4] - t[1] * t[3];
+    for (int i = 0; i < 3; ++i) {
+        ext[i] = -1;
+        if (!(std::isfinite(reach) && std::isfinite(det) && det != 0)) continue;
+        const double n = sqrt(inv[3 * i] * inv[3 * i] + inv[3 * i + 1] * inv[3 * i + 1] +
+                              inv[3 * i + 2] * inv[3 * i + 2]) / fabs(det);
+        const double e = reach * n * (1 + 1e-9);
+        if (std::isfinite(e) && e < 1e15) ext[i] = (long long)floor(e) + 1;
+    }
+}
+}  // namespace
+
 bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses[3], int kmax, std::string& err) {
     if (kind != KIND_OK && kind != KIND_SK && kind != KIND_MIK) {
         err = "unknown kriging kind";
@@ -106,11 +164,11 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
             err = "search radiuses must be >= 0";
             return false;
         }
-        if (radiuses[i] > 255) {
-            err = "search radiuses above 255 are not supported";
-            return false;
-        }
         p.r[i] = radiuses[i];
+    }
+    if ((2LL * p.r[0] + 1) * (2LL * p.r[1] + 1) * (2LL * p.r[2] + 1) > kTemplateBoxMaxEntries) {
+        err = "search radiuses too large: the +-radius box exceeds 2^31 offsets";
+        return false;
     }
     if (kmax < 1) {
         err = "max_neighbours must be >= 1";
@@ -153,10 +211,38 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
         p.tdz[i] = vec[i].z;
     }
     p.limit = (long long)(vec.size() / 250);
+    data.clear();
+    data.shrink_to_fit();
+
+    // Extents: H from the template; D adds the cluster path's reach, at most
+    // 2r - 1 (the 27 clusters of r cells) and bounded by the covariance.
+    for (int i = 0; i < 3; ++i) p.H[i] = 0;
+    for (const V& o : vec) {
+        p.H[0] = std::max(p.H[0], std::abs(o.x));
+        p.H[1] = std::max(p.H[1], std::abs(o.y));
+        p.H[2] = std::max(p.H[2], std::abs(o.z));
+    }
+    long long ext[3];
+    reach_extents(p.transform, cov_reach(cov), ext);
+    for (int i = 0; i < 3; ++i) {
+        long long d = 2LL * p.r[i] - 1;
+        if (ext[i] >= 0 && ext[i] < d) d = ext[i];
+        if (d < p.H[i]) d = p.H[i];
+        p.D[i] = (int)d;
+    }
+    // neighbour offsets travel packed in 11/11/10 bits (krige_kernels.h)
+    if (p.D[0] > 1023 || p.D[1] > 1023 || p.D[2] > 511) {
+        err = "the covariance reaches beyond 1023 cells in x or y or 511 in z; not supported";
+        return false;
+    }
 
     // Covariance table over [-W, W].
-    for (int i = 0; i < 3; ++i) p.W[i] = 4 * p.r[i] - 2 > 2 * p.r[i] ? 4 * p.r[i] - 2 : 2 * p.r[i];
+    for (int i = 0; i < 3; ++i) p.W[i] = 2 * p.D[i];
     const int sx = 2 * p.W[0] + 1, sy = 2 * p.W[1] + 1, sz = 2 * p.W[2] + 1;
+    if ((long long)sx * sy * sz > kCovTableMaxEntries) {
+        err = "the covariance table over the neighbour reach exceeds 2^27 entries; not supported";
+        return false;
+    }
     p.ctab.assign((size_t)sx * sy * sz, 0.0);
     const bool clip = kind != KIND_OK;
     auto fill_planes = [&](int z0, int z1) {
@@ -193,7 +279,7 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
     p.otab.clear();
     p.otab_w.clear();
     p.rank_w.clear();
-    for (int i = 0; i < 3; ++i) p.E[i] = 2 * p.r[i];
+    for (int i = 0; i < 3; ++i) p.E[i] = 2 * p.H[i];
     const int wx1 = p.W[0] + 1, wy1 = p.W[1] + 1, wz1 = p.W[2] + 1;
     const size_t wn = (size_t)wx1 * wy1 * wz1;
     if (cov.angles[0] == 0 && cov.angles[1] == 0 && cov.angles[2] == 0 && wn <= kOctantWideMaxEntries) {

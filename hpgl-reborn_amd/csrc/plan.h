@@ -28,6 +28,10 @@ struct CovParams {
 constexpr size_t kOctantMaxBytes = 160 * 1024;
 // Largest +-W octant table built (entries).
 constexpr size_t kOctantWideMaxEntries = size_t(32) << 20;
+// Largest +-radius box the template is scanned over, and largest full
+// covariance table (entries, 8 bytes each).
+constexpr long long kTemplateBoxMaxEntries = 1LL << 31;
+constexpr long long kCovTableMaxEntries = 1LL << 27;
 
 struct PlanHost {
     Kind kind;
@@ -37,9 +41,19 @@ struct PlanHost {
     int kmax;  // max_neighbours
     // Search template: offsets from the cell, in the reference's order.
     std::vector<int> tdx, tdy, tdz;
+    // Extents per axis (all <= the radius-derived bounds, so large radiuses
+    // cost only what the covariance reaches):
+    //   H  largest |offset| of a template entry: the padded mask's halo and
+    //      the template-path neighbour reach
+    //   D  largest |offset| any neighbour of either search path can have:
+    //      cluster-path candidates come from the 27 clusters around the cell
+    //      (|d| <= 2r - 1) and are kept only where C(d) > C(0)/100, which
+    //      bounds |d| by the covariance's own reach; D >= H
+    int H[3] = {0, 0, 0};
+    int D[3] = {0, 0, 0};
     // Covariance of the kind's provider over the box [-W, W] (x fastest):
-    // OK unclipped, SK / MIK zero outside the +-radius box.  W = 4r - 2 per
-    // axis covers every neighbour pair of either search path.
+    // OK unclipped, SK / MIK zero outside the +-radius box.  W = 2D per axis
+    // covers every neighbour pair of either search path.
     int W[3];
     std::vector<double> ctab;
     // Octant tables: with all angles zero the covariance is even in each
@@ -49,7 +63,7 @@ struct PlanHost {
     //   otab_w  |d| <= W: every pair either search path can produce
     //   rank_w  |d| <= W: 0 where C <= C(0)/100, else 1 + the dense rank of C
     //           among the table's values above C(0)/100
-    //   otab    |d| <= E = 2r: every pair of template-path neighbours; empty
+    //   otab    |d| <= E = 2H: every pair of template-path neighbours; empty
     //           when it exceeds kOctantMaxBytes (it is staged in LDS)
     bool octant = false;
     int E[3] = {0, 0, 0};

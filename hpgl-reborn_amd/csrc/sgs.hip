@@ -21,12 +21,13 @@
 namespace hpglx {
 namespace {
 
-// Padded 0/1 mask (halo of r cells, x pitch px) from the cell mask.
+// Padded 0/1 mask (halo of h cells: the template's reach; x pitch px) from
+// the cell mask.
 __global__ void k_sgs_pmask(GridDims g, const unsigned char* mask, unsigned char* pmask) {
     const long long P = (long long)g.px * g.py * g.pz;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P;
          i += (long long)gridDim.x * blockDim.x) {
-        const int x = (int)(i % g.px) - g.rx, y = (int)((i / g.px) % g.py) - g.ry, z = (int)(i / ((long long)g.px * g.py)) - g.rz;
+        const int x = (int)(i % g.px) - g.hx, y = (int)((i / g.px) % g.py) - g.hy, z = (int)(i / ((long long)g.px * g.py)) - g.hz;
         const bool in = x >= 0 && x < g.nx && y >= 0 && y < g.ny && z >= 0 && z < g.nz;
         pmask[i] = in ? (mask[((long long)z * g.ny + y) * g.nx + x] != 0) : 0;
     }
@@ -71,9 +72,12 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
     g.rx = h.r[0];
     g.ry = h.r[1];
     g.rz = h.r[2];
-    g.px = g.nx + 2 * g.rx;
-    g.py = g.ny + 2 * g.ry;
-    g.pz = g.nz + 2 * g.rz;
+    g.hx = h.H[0];
+    g.hy = h.H[1];
+    g.hz = h.H[2];
+    g.px = g.nx + 2 * g.hx;
+    g.py = g.ny + 2 * g.hy;
+    g.pz = g.nz + 2 * g.hz;
     g.cx = g.cy = g.cz = 0;
     const long long n = g.n, P = (long long)g.px * g.py * g.pz;
     const int K = h.kmax, T = plan.tsize;

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(64) void k_sgs_level(SolveArgs a, SgsLevelArgs la) 
     unsigned char* pmask = la.pmask + r * la.pn;
     const GridDims& g = a.g;
     const int x = cell % g.nx, y = (cell / g.nx) % g.ny, zc = cell / (g.nx * g.ny);
-    const long long pcell = ((long long)(zc + g.rz) * g.py + (y + g.ry)) * g.px + (x + g.rx);
+    const long long pcell = ((long long)(zc + g.hz) * g.py + (y + g.hy)) * g.px + (x + g.hx);
     int* nboff = const_cast<int*>(a.nboff) + (long long)cell * a.kmax;
     int cnt = 0;
     for (int t0 = 0; t0 < la.T && cnt < a.kmax; t0 += 64) {
