@@ -801,8 +801,8 @@ int sgs_core(Library& L, const hpgl_sgs_params_t* p, const int dims[3], std::vec
         err = "sequential simulation takes simple (1) or ordinary (0) kriging";
         return -1;
     }
-    if (p->m_max_neighbours > 32) {
-        err = "max_neighbours above 32 is not supported by the GPU simulation";
+    if (p->m_max_neighbours > 64) {
+        err = "max_neighbours above 64 is not supported by the GPU simulation";
         return -1;
     }
     if (lvm_host && p->m_kriging_kind != KIND_SK) {

@@ -59,8 +59,8 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
         err = "a locally varying mean applies to simple kriging only";
         return -1;
     }
-    if (h.kmax > 32) {
-        err = "max_neighbours above 32 is not supported by the GPU solver";
+    if (h.kmax > 64) {
+        err = "max_neighbours above 64 is not supported by the GPU simulation";
         return -1;
     }
     if (!plan.upload(err)) return -1;
@@ -203,8 +203,12 @@ int sgs_run(DevPlan& plan, Workspace& ws, int kind, const int dims[3], float* d_
             kind == KIND_OK ? sgs_level_16_0(so, la, st) : sgs_level_16_1(so, la, st);
         else if (K <= 24)
             kind == KIND_OK ? sgs_level_24_0(so, la, st) : sgs_level_24_1(so, la, st);
-        else
+        else if (K <= 32)
             kind == KIND_OK ? sgs_level_32_0(so, la, st) : sgs_level_32_1(so, la, st);
+        else if (K <= 48)
+            kind == KIND_OK ? sgs_level_48_0(so, la, st) : sgs_level_48_1(so, la, st);
+        else
+            kind == KIND_OK ? sgs_level_64_0(so, la, st) : sgs_level_64_1(so, la, st);
     }
     if (hipGetLastError() != hipSuccess) {
         err = "sequential simulation kernel launch failed";

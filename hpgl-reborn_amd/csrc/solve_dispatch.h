@@ -41,7 +41,7 @@ struct SgsLevelArgs {
     int T;                    // template entries
     const int* tdelta;        // their deltas in the padded mask
     const int* tpack;         // their packed offsets
-    unsigned char* pmask;     // padded 0/1 mask (halo r, pitch px)
+    unsigned char* pmask;     // padded 0/1 mask (halo h, pitch px)
     const int* cells;         // the level's cells
     int ncells;
     const double* z;          // their standard deviates (NaN: the draw gives the mean)
@@ -69,6 +69,11 @@ HPGLX_DECL_SOLVE(32, 0)
 HPGLX_DECL_SOLVE(32, 1)
 HPGLX_DECL_SOLVE(32, 2)
 #undef HPGLX_DECL_SOLVE
+// simulation levels above 32 neighbours (a wavefront per cell; solve_wide.hip)
+void sgs_level_48_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
+void sgs_level_48_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
+void sgs_level_64_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
+void sgs_level_64_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
 void solve_fallback_g4_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
 void solve_fallback_g4_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
 void solve_fallback_g4_2(const SolveArgs& sa, long long max_cells, hipStream_t st);

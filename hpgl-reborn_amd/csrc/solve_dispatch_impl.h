@@ -14,6 +14,7 @@
 
 #include "solve.h"
 #include "solve_dispatch.h"
+#include "solve_wave.h"
 
 namespace hpglx {
 namespace {
@@ -133,7 +134,13 @@ __global__ __launch_bounds__(64) void k_sgs_level(SolveArgs a, SgsLevelArgs la) 
     __threadfence_block();
     __syncthreads();
     const unsigned ox = (unsigned)(a.wx + 1), oxy = ox * (unsigned)(a.wy + 1);
-    if (lane < G) solve_cell<NMAX, G, false, KIND, OCT>(a, cell, cnt, lane, a.otab_w, ox, oxy);
+    if constexpr (NMAX > 32) {
+        // above the register solvers: the whole wave, a row per lane
+        __shared__ double col[64];
+        wave_solve_cell<NMAX, KIND>(a, cell, cnt, lane, col);
+    } else {
+        if (lane < G) solve_cell<NMAX, G, false, KIND, OCT>(a, cell, cnt, lane, a.otab_w, ox, oxy);
+    }
     __threadfence_block();
     __syncthreads();
     if (lane == 0) {

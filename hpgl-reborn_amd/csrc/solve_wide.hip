@@ -3,8 +3,9 @@
 // The register solvers hold a cell's system in the registers of 4 or 8
 // lanes, which bounds them at 32 neighbours.  The reference takes any
 // max_neighbours (its edge tests krige with 1000 on a 10x10x5 grid), so
-// larger systems run here: one workgroup per cell, the system in LDS (up to
-// 141 neighbours) or in a global slot per workgroup, an FP64 Cholesky
+// larger systems run here: up to 64 neighbours one wavefront per cell with a
+// row per lane (solve_wave.h), beyond that one workgroup per cell, the system
+// in LDS (up to 141 neighbours) or in a global slot per workgroup, an FP64 Cholesky
 // factor column by column (my_kriging_weights.h:60-196 SK, :198-361 OK,
 // dpotrf/dpotrs), and the combine, failure policy and outputs of the
 // register path (combiner.h:10-25, cont_kriging.h:73-105, median_ik.cpp:60-78,
@@ -19,6 +20,8 @@
 #include "krige_launch.h"
 #include "solve.h"
 #include "solve_dispatch.h"
+#include "solve_dispatch_impl.h"
+#include "solve_wave.h"
 
 namespace hpglx {
 namespace {
@@ -234,6 +237,16 @@ __global__ __launch_bounds__(WB) void k_solve_wide(SolveArgs a, double* gscratch
     }
 }
 
+// Systems of up to NMAX (<= 64) neighbours: one wavefront per cell
+// (solve_wave.h), four cells per block.
+template <int NMAX, int KIND>
+__global__ __launch_bounds__(256) void k_solve_wave(SolveArgs a) {
+    __shared__ double col[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long cell = (long long)blockIdx.x * 4 + w;
+    if (cell < a.g.n) wave_solve_cell<NMAX, KIND>(a, cell, a.count[cell], lane, col[w]);
+}
+
 // Largest neighbour count of the grid (into *out, zeroed by the first lane).
 __global__ __launch_bounds__(256) void k_count_max(const int* count, long long n, int* out) {
     __shared__ int wmax[4];
@@ -267,8 +280,16 @@ int launch_wide(const SolveArgs& sa, Workspace& ws, const hipEvent_t* ev, hipStr
         return -1;
     }
     const int kcap = std::max(1, std::min(found, sa.kmax));
-    const long long slot = wide_slot(kcap) * 8;
     const long long n = sa.g.n;
+    if (kcap <= 64) {
+        const unsigned blocks = (unsigned)((n + 3) / 4);
+        if (kcap <= 48)
+            HPGLX_LAUNCH_EV((k_solve_wave<48, KIND>), dim3(blocks), dim3(256), 0, st, ev[0], ev[1], sa);
+        else
+            HPGLX_LAUNCH_EV((k_solve_wave<64, KIND>), dim3(blocks), dim3(256), 0, st, ev[0], ev[1], sa);
+        return 0;
+    }
+    const long long slot = wide_slot(kcap) * 8;
     if (slot <= kWideLdsBytes) {
         const void* fn = (const void*)&k_solve_wide<KIND>;
         if (slot > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slot) != hipSuccess) {
@@ -300,6 +321,19 @@ int launch_wide(const SolveArgs& sa, Workspace& ws, const hipEvent_t* ev, hipStr
 }
 
 }  // namespace
+
+void sgs_level_48_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    launch_sgs_level<48, 8, 0>(sa, la, st);
+}
+void sgs_level_48_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    launch_sgs_level<48, 8, 1>(sa, la, st);
+}
+void sgs_level_64_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    launch_sgs_level<64, 8, 0>(sa, la, st);
+}
+void sgs_level_64_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
+    launch_sgs_level<64, 8, 1>(sa, la, st);
+}
 
 int dispatch_solve_wide(const SolveArgs& sa, int kind, Workspace& ws, const hipEvent_t* ev, hipStream_t st,
                         std::string& err) {

@@ -20,6 +20,10 @@ CASES = [
     dict(name="sk_gaussian_k24", kind=1, shape=(40, 36, 8), frac=0.05, cov=(2, (10, 10, 3), 0.2), r=(6, 6, 2),
          k=24, seed=2024),
     dict(name="ok_k32", kind=0, shape=(30, 28, 6), frac=0.06, cov=(1, (9, 9, 3), 0.1), r=(5, 5, 2), k=32, seed=9),
+    # above 32 neighbours: a wavefront per cell
+    dict(name="sk_k40", kind=1, shape=(28, 26, 6), frac=0.08, cov=(1, (9, 9, 3), 0.1), r=(6, 6, 2), k=40, seed=41),
+    dict(name="ok_k64", kind=0, shape=(30, 28, 6), frac=0.10, cov=(0, (10, 10, 3), 0.05), r=(7, 7, 3), k=64,
+         seed=64),
 ]
 
 
@@ -137,7 +141,7 @@ def test_sgs_invalid_parameters_raise():
     m = torch.zeros(8 * 8 * 2, dtype=torch.uint8, device=dev)
     model = CovarianceModel(0, (4, 4, 2), (0.0, 0.0, 0.0), 1.0, 0.0)
     with pytest.raises(RuntimeError):
-        D.sgs_simulation(v, m, (8, 8, 2), (2, 2, 1), 40, model, 1)  # max_neighbours above 32
+        D.sgs_simulation(v, m, (8, 8, 2), (2, 2, 1), 80, model, 1)  # max_neighbours above 64
     with pytest.raises(RuntimeError):
         D.sgs_simulation(v, m, (8, 8, 2), (2, 2, 1), 8, model, -3)  # negative seed
 
