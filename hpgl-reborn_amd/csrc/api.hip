@@ -350,6 +350,7 @@ bool request_call(const hpglx_grid_request_t* q, GridCall& call, const int* cats
     if (const char* sv = std::getenv("HPGLX_SEARCH_HEAD")) call.search_head = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH_SLABS")) call.search_slabs = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_CLUSTER_CAP")) call.cluster_cap = std::atoi(sv);
+    if (const char* sv = std::getenv("HPGLX_OVERLAP")) call.overlap = std::atoi(sv);
     if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
         err = "values and mask are required";
         return false;

@@ -1568,11 +1568,16 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     SortedLaunch sl;
     sl.mixed = mixed;
     sl.blocks_per_cu = c.blocks_per_cu;
+    // The launches below cover disjoint ranges of the work list: every one
+    // after the first may overlap those before it.
+    bool first_solve = true;
     for (int path = lds ? 0 : 1; path < 2; ++path) {
         sl.path = path;
         sl.nelem = path == 0 ? nelem : 0;
         const bool main_path = path == (lds ? 0 : 1);
         sl.ev = (main_path && !g8) ? ev_t + 2 : nullptr;
+        sl.any_order = c.overlap != 0 && !first_solve;
+        first_solve = false;
         SolveArgs sp = so;
         sp.sched = c.dyn_sched ? counters + CTR_SCHED + 2 * path : nullptr;
         // where the LDS form cannot run, the global tables serve the same range
@@ -1583,6 +1588,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         if (g8) {
             sl.nelem = path == 0 ? nelem : 0;
             sl.ev = main_path ? ev_t + 2 : nullptr;
+            sl.any_order = c.overlap != 0;
             sp.sched = c.dyn_sched ? counters + CTR_SCHED + 2 * path + 1 : nullptr;
             if (!solve_sorted(32, c.kind, sp, sl, st)) {
                 sl.nelem = 0;

@@ -16,6 +16,17 @@
         else                                                                              \
             hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);                \
     } while (0)
+// The same, and with `any` the dispatch carries no barrier bit: the kernel
+// may start while earlier kernels of the stream still run (it must not read
+// what they write); a later ordinary launch still waits for all of them.
+#define HPGLX_LAUNCH_EV_ANY(any, kernel, grid, block, lds, st, e0, e1, ...)                              \
+    do {                                                                                                 \
+        if ((any) || (e0) != nullptr || (e1) != nullptr)                                                 \
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, st, e0, e1, (any) ? hipExtAnyOrderLaunch : 0, \
+                                  __VA_ARGS__);                                                          \
+        else                                                                                             \
+            hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);                               \
+    } while (0)
 
 namespace hpglx {
 

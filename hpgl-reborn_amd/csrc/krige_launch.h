@@ -100,6 +100,7 @@ struct GridCall {
     int cluster_lanes = 0;                  // lanes of the full-table cluster search (multiple of 256; 0 = default)
     int blocks_per_cu = 0;                  // cap on the persistent solve grids' blocks per CU (0 = all that fit)
     int dyn_sched = 0;                      // solve chunks dealt from a counter instead of a fixed stride
+    int overlap = 1;                        // independent launches may run concurrently (hipExtAnyOrderLaunch)
     bool profile = false;
 };
 

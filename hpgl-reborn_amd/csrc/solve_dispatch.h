@@ -18,6 +18,10 @@ struct SortedLaunch {
     int nelem = 0;                 // octant-table entries staged in LDS (path 0), 0 = global tables
     int blocks_per_cu = 0;         // cap on the persistent grid's blocks per CU (0 = as many as fit)
     const hipEvent_t* ev = nullptr;  // start / stop events of this launch (entries may be null)
+    // The launch covers a range of the work list no launch before it since the
+    // list was built touches, so it may start before they finish
+    // (hipExtAnyOrderLaunch: no barrier bit on its dispatch packet).
+    bool any_order = false;
 };
 
 // The sizes up to top = 8, 16 or 24 (4 lanes per cell), or top = 32 for the

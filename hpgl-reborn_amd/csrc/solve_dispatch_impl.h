@@ -60,20 +60,20 @@ bool launch_sorted_p(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st)
         int grid = persistent_grid(fn, BLOCK, lds);
         if (grid <= 0) return false;
         grid = std::min(grid, blocks_per_cu_cap * std::max(cus, 1));
-        HPGLX_LAUNCH_EV((k_solve_sorted_lds<TOP, MIXED, KIND, BLOCK>), dim3((unsigned)grid), dim3(BLOCK),
-                        (uint32_t)lds, st, e0, e1, sa, l.nelem, l.path);
+        HPGLX_LAUNCH_EV_ANY(l.any_order, (k_solve_sorted_lds<TOP, MIXED, KIND, BLOCK>), dim3((unsigned)grid),
+                            dim3(BLOCK), (uint32_t)lds, st, e0, e1, sa, l.nelem, l.path);
         return true;
     }
     if (sa.otab_w) {
         const void* fn = (const void*)&k_solve_sorted<TOP, MIXED, KIND, true>;
         const int grid = std::min(persistent_grid(fn, 256, 0), blocks_per_cu_cap * std::max(cus, 1));
-        HPGLX_LAUNCH_EV((k_solve_sorted<TOP, MIXED, KIND, true>), dim3((unsigned)std::max(grid, 1)), dim3(256), 0,
-                        st, e0, e1, sa, l.path);
+        HPGLX_LAUNCH_EV_ANY(l.any_order, (k_solve_sorted<TOP, MIXED, KIND, true>), dim3((unsigned)std::max(grid, 1)),
+                            dim3(256), 0, st, e0, e1, sa, l.path);
     } else {
         const void* fn = (const void*)&k_solve_sorted<TOP, MIXED, KIND, false>;
         const int grid = std::min(persistent_grid(fn, 256, 0), blocks_per_cu_cap * std::max(cus, 1));
-        HPGLX_LAUNCH_EV((k_solve_sorted<TOP, MIXED, KIND, false>), dim3((unsigned)std::max(grid, 1)), dim3(256), 0,
-                        st, e0, e1, sa, l.path);
+        HPGLX_LAUNCH_EV_ANY(l.any_order, (k_solve_sorted<TOP, MIXED, KIND, false>),
+                            dim3((unsigned)std::max(grid, 1)), dim3(256), 0, st, e0, e1, sa, l.path);
     }
     return true;
 }
