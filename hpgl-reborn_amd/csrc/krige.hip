@@ -1152,15 +1152,15 @@ bool solve_sorted(int top, int kind, const SolveArgs& sa, const SortedLaunch& l,
     }
 }
 
-void solve_fallback(bool g8, int kind, const SolveArgs& sa, long long max_cells, hipStream_t st) {
-    if (g8) {
-        if (kind == 0) solve_fallback_g8_0(sa, max_cells, st);
-        else if (kind == 1) solve_fallback_g8_1(sa, max_cells, st);
-        else solve_fallback_g8_2(sa, max_cells, st);
+void solve_fallback(bool b32, int kind, const SolveArgs& sa, long long max_cells, hipStream_t st) {
+    if (b32) {
+        if (kind == 0) solve_fallback_b32_0(sa, max_cells, st);
+        else if (kind == 1) solve_fallback_b32_1(sa, max_cells, st);
+        else solve_fallback_b32_2(sa, max_cells, st);
     } else {
-        if (kind == 0) solve_fallback_g4_0(sa, max_cells, st);
-        else if (kind == 1) solve_fallback_g4_1(sa, max_cells, st);
-        else solve_fallback_g4_2(sa, max_cells, st);
+        if (kind == 0) solve_fallback_b24_0(sa, max_cells, st);
+        else if (kind == 1) solve_fallback_b24_1(sa, max_cells, st);
+        else solve_fallback_b24_2(sa, max_cells, st);
     }
 }
 
@@ -1560,11 +1560,11 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     hipLaunchKernelGGL(k_bucket_scan, dim3(NBINS), dim3(256), 0, st, bucket_hist, counters + CTR_BIN_TOTAL);
     hipLaunchKernelGGL(k_bucket_scatter, dim3(BUCKET_BLOCKS), dim3(256), 0, st, so, all_global,
                        (const int*)bucket_hist, (const int*)(counters + CTR_BIN_TOTAL), counters + CTR_SEG, sorted);
-    // Buckets up to `top` run with 4 lanes per cell in one launch per path;
-    // the 25-32 bucket with 8 lanes in its own.  The main kernel (events) is
-    // the largest bucket's launch on the main path.
+    // Buckets up to `top` run in one launch per path; the 25-32 bucket in
+    // its own (4 lanes per cell throughout).  The main kernel (events) is the
+    // largest bucket's launch on the main path.
     const int top = K <= 8 ? 8 : (K <= 16 ? 16 : 24);
-    const bool g8 = K > 24;
+    const bool b32 = K > 24;
     SortedLaunch sl;
     sl.mixed = mixed;
     sl.blocks_per_cu = c.blocks_per_cu;
@@ -1575,7 +1575,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         sl.path = path;
         sl.nelem = path == 0 ? nelem : 0;
         const bool main_path = path == (lds ? 0 : 1);
-        sl.ev = (main_path && !g8) ? ev_t + 2 : nullptr;
+        sl.ev = (main_path && !b32) ? ev_t + 2 : nullptr;
         sl.any_order = c.overlap != 0 && !first_solve;
         first_solve = false;
         SolveArgs sp = so;
@@ -1585,7 +1585,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
             sl.nelem = 0;
             solve_sorted(top, c.kind, sp, sl, st);
         }
-        if (g8) {
+        if (b32) {
             sl.nelem = path == 0 ? nelem : 0;
             sl.ev = main_path ? ev_t + 2 : nullptr;
             sl.any_order = c.overlap != 0;
@@ -1603,7 +1603,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         fb.fallback_cells = nullptr;
         fb.n_fallback = nullptr;
         solve_fallback(false, c.kind, fb, n, st);
-        if (g8) solve_fallback(true, c.kind, fb, n, st);
+        if (b32) solve_fallback(true, c.kind, fb, n, st);
     }
     HPGLX_CHECK(hipGetLastError());
     return 0;

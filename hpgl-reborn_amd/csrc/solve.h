@@ -802,7 +802,7 @@ __device__ __forceinline__ void solve_range(const SolveArgs& a, int lo, int hi, 
         const int cell = idx < iend ? a.sorted[idx] : -1;
         const bool full = __all(cell < 0 || count_n(a.count[cell]) == nb);
         if constexpr (TOP == 32) {
-            solve_chunk<32, 8, MIXED, KIND, LTAB, PACK, true>(a, i0, iend, lane, full, ltab, ox, oxy);
+            solve_chunk<32, 4, MIXED, KIND, LTAB, PACK, true>(a, i0, iend, lane, full, ltab, ox, oxy);
         } else {
             if (nb == 8)
                 solve_chunk<8, 4, MIXED, KIND, LTAB, PACK, TOP == 8>(a, i0, iend, lane, full, ltab, ox, oxy);
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve_sorted_lds(SolveArgs a, int nel
 // table.  The mixed form is held to 2 waves per SIMD, which it reaches
 // without spilling (the FP64 form would spill).
 template <int TOP, bool MIXED, int KIND, bool OCT>
-__global__ __launch_bounds__(256, MIXED ? 2 : 1) void k_solve_sorted(SolveArgs a, int path) {
+__global__ __launch_bounds__(256, (MIXED && TOP != 32) ? 2 : 1) void k_solve_sorted(SolveArgs a, int path) {
     int lo, hi;
     sorted_range<TOP>(a, path, lo, hi);
     if (lo >= hi) return;
@@ -869,11 +869,11 @@ __global__ __launch_bounds__(256, MIXED ? 2 : 1) void k_solve_sorted(SolveArgs a
 }
 
 // FP64 re-solve of the mixed form's fallbacks (an unordered list): each
-// group takes its cell's own bucket size, so a cell gets the FP64 form's
-// solver and bits.  G8: the 32 bucket (8 lanes per cell), else the others.
-template <bool G8, int KIND, bool OCT>
+// group of 4 lanes takes its cell's own bucket size, so a cell gets the FP64
+// form's solver and bits.  B32: the 25-32 bucket, else the others.
+template <bool B32, int KIND, bool OCT>
 __global__ __launch_bounds__(256) void k_solve_fallback(SolveArgs a) {
-    constexpr int G = G8 ? 8 : 4;
+    constexpr int G = 4;
     const int t = threadIdx.x & (G - 1);
     const long long gid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const long long ngroups = ((long long)gridDim.x * blockDim.x) / G;
@@ -883,8 +883,8 @@ __global__ __launch_bounds__(256) void k_solve_fallback(SolveArgs a) {
         const int cell = a.cell_list[item];
         const int cnt = a.count[cell];
         const int b = count_bucket(count_n(cnt));
-        if constexpr (G8) {
-            if (b == 3) solve_cell<32, 8, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
+        if constexpr (B32) {
+            if (b == 3) solve_cell<32, 4, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
         } else {
             if (b == 0) solve_cell<8, 4, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);
             if (b == 1) solve_cell<16, 4, false, KIND, OCT>(a, cell, cnt, t, a.otab_w, ox, oxy);

@@ -25,12 +25,12 @@ struct SortedLaunch {
 };
 
 // The sizes up to top = 8, 16 or 24 (4 lanes per cell), or top = 32 for the
-// 25-32 bucket (8 lanes per cell), of `kind` over one path of the work list.
+// 25-32 bucket, 4 lanes per cell, of `kind` over one path of the work list.
 // Returns false when nothing was launched (the LDS table does not fit).
 bool solve_sorted(int top, int kind, const SolveArgs& sa, const SortedLaunch& l, hipStream_t st);
 // FP64 re-solve of the mixed form's fallbacks (sa.cell_list / sa.n_list):
-// buckets 0-2 (g8 false) or bucket 3 (g8 true).
-void solve_fallback(bool g8, int kind, const SolveArgs& sa, long long max_cells, hipStream_t st);
+// buckets 0-2 (b32 false) or bucket 3 (b32 true).
+void solve_fallback(bool b32, int kind, const SolveArgs& sa, long long max_cells, hipStream_t st);
 
 // Systems above 32 neighbours (sa.kmax > 32): one workgroup per cell, FP64,
 // every cell of the grid; scratch from ws.  Returns 0, or -1 with err.
@@ -78,11 +78,11 @@ void sgs_level_48_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st)
 void sgs_level_48_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
 void sgs_level_64_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
 void sgs_level_64_1(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st);
-void solve_fallback_g4_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
-void solve_fallback_g4_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
-void solve_fallback_g4_2(const SolveArgs& sa, long long max_cells, hipStream_t st);
-void solve_fallback_g8_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
-void solve_fallback_g8_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
-void solve_fallback_g8_2(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_b24_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_b24_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_b24_2(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_b32_0(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_b32_1(const SolveArgs& sa, long long max_cells, hipStream_t st);
+void solve_fallback_b32_2(const SolveArgs& sa, long long max_cells, hipStream_t st);
 
 }  // namespace hpglx

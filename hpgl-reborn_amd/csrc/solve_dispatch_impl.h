@@ -41,9 +41,9 @@ inline int persistent_grid(const void* fn, int block, int lds) {
     return nb * cus;
 }
 
-// One launch of the bucketed solve (solve.h): the sizes up to TOP (4 lanes
-// per cell, TOP <= 24) or the 25-32 bucket (8 lanes), over one path's range
-// of the work list.  Path 0 with nelem > 0 stages the octant table in LDS;
+// One launch of the bucketed solve (solve.h), 4 lanes per cell: the sizes up
+// to TOP (TOP <= 24) or the 25-32 bucket (its registers allow one wave per
+// SIMD), over one path's range of the work list.  Path 0 with nelem > 0 stages the octant table in LDS;
 // otherwise the global tables serve.  Returns false when nothing could be
 // launched (the LDS table does not fit).
 template <int TOP, int KIND, bool MIXED>
@@ -85,16 +85,16 @@ bool launch_sorted(const SolveArgs& sa, const SortedLaunch& l, hipStream_t st) {
 
 // FP64 re-solve of the mixed form's fallback list (sa.cell_list / n_list),
 // sized for up to max_cells cells.
-template <bool G8, int KIND>
+template <bool B32, int KIND>
 void launch_fallback(const SolveArgs& sa, long long max_cells, hipStream_t st) {
-    constexpr int G = G8 ? 8 : 4;
+    constexpr int G = 4;
     long long blocks = (max_cells * G + 255) / 256;
     // the list is usually empty or short: 16k groups stride over it
     blocks = std::max(1LL, std::min(blocks, 256LL));
     if (sa.otab_w)
-        hipLaunchKernelGGL((k_solve_fallback<G8, KIND, true>), dim3((unsigned)blocks), dim3(256), 0, st, sa);
+        hipLaunchKernelGGL((k_solve_fallback<B32, KIND, true>), dim3((unsigned)blocks), dim3(256), 0, st, sa);
     else
-        hipLaunchKernelGGL((k_solve_fallback<G8, KIND, false>), dim3((unsigned)blocks), dim3(256), 0, st, sa);
+        hipLaunchKernelGGL((k_solve_fallback<B32, KIND, false>), dim3((unsigned)blocks), dim3(256), 0, st, sa);
 }
 
 // One level of sequential Gaussian simulation (see SgsLevelArgs): one wave

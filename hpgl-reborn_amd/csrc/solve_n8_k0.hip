@@ -14,7 +14,7 @@ void sgs_level_8_0(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) 
     launch_sgs_level<8, 4, 0>(sa, la, st);
 }
 
-void solve_fallback_g4_0(const SolveArgs& sa, long long max_cells, hipStream_t st) {
+void solve_fallback_b24_0(const SolveArgs& sa, long long max_cells, hipStream_t st) {
     launch_fallback<false, 0>(sa, max_cells, st);
 }
 
