@@ -301,7 +301,7 @@ def side_workload(name, args):
         # neighbour counts and search paths (untimed) for the main solve
         # kernel's algorithmic flops: with the LDS table (median IK, LVM) the
         # template-path cells of every bucket; without it (wells) every cell
-        # of the 25-32 bucket, which the 8-lane kernel solves
+        # of the 25-32 bucket, which the 4-lane 32-neighbour kernel solves
         nbk = dict(zfield=z, thresholds=wl.thresholds, marginal=wl.levels) if name == "mik" else dict(values=v)
         nb = D.krige(2 if name == "mik" else (1 if name == "lvm" else 0), wl.shape, wl.radiuses, wl.k, model, m,
                      neighbours=True, variance=False, **nbk)
@@ -310,7 +310,7 @@ def side_workload(name, args):
         cnt = np.maximum(counts, 0)
         if name == "wells":
             main = cnt >= 25
-            kernel = "k_solve_sorted<32,8,{},OK>"
+            kernel = "k_solve_sorted<32,4,{},OK>"
         else:
             main = (counts > 0) & ~cl
             kernel = "k_solve_sorted_lds<{},4,{{}},{}>".format(16 if name == "mik" else 24,

@@ -645,6 +645,9 @@ __device__ __forceinline__ unsigned pack_xyz(int x, int y, int z) {
     return (unsigned)x | ((unsigned)y << 11) | ((unsigned)z << 22);
 }
 
+// BOUNDED: candidates can lie beyond the rank table (a radius whose 27
+// clusters reach farther than the covariance does); those are skipped.
+template <bool BOUNDED>
 __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs a) {
     extern __shared__ unsigned cl_lds[];
     __shared__ int off27[28];
@@ -704,7 +707,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
                     const int dx = (int)(p & 2047u) - x, dy = (int)((p >> 11) & 2047u) - y, dz = (int)(p >> 22) - z;
                     const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy, az = dz < 0 ? -dz : dz;
                     // beyond the table the covariance is below C(0)/100 (rank 0)
-                    const bool in = ax < a.wx1 && ay < a.wy1 && az < a.wz1;
+                    const bool in = !BOUNDED || (ax < a.wx1 && ay < a.wy1 && az < a.wz1);
                     rk[u] = in ? a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax] : 0u;
                 }
 #pragma unroll
@@ -746,6 +749,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
 constexpr int CL_LIST_BLOCK = 64;
 constexpr int kClusterListCap = 448;
 
+template <bool BOUNDED>
 __global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(ClusterOctArgs a, const int* cells,
                                                                            const int* ncells_p, int cap2,
                                                                            int* ovf2, int* n_ovf2) {
@@ -781,7 +785,7 @@ __global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(Clust
                             int nx_, ny_, nz_;
                             cell_xyz(g, nd[u], nx_, ny_, nz_);
                             const int ax = abs(nx_ - x), ay = abs(ny_ - y), az = abs(nz_ - z);
-                            const bool in = ax < a.wx1 && ay < a.wy1 && az < a.wz1;
+                            const bool in = !BOUNDED || (ax < a.wx1 && ay < a.wy1 && az < a.wz1);
                             rk[u] = in ? a.rank_w[((long long)az * a.wy1 + ay) * a.wx1 + ax] : 0u;
                         }
 #pragma unroll
@@ -1481,13 +1485,28 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         oa.nboff = nboff;
         oa.ovf_cells = ovf_cells;
         oa.n_ovf = n_ovf;
-        if (cl_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_cluster_oct, (int)cl_lds));
-        hipLaunchKernelGGL(k_search_cluster_oct, dim3((unsigned)std::min(ncl, 2048)), dim3(CL_BLOCK), cl_lds, st, oa);
+        // Candidates come from the 27 clusters around a cell (|d| <= 2r - 1);
+        // only a radius beyond the covariance's reach puts some past the table.
+        const bool bounded = 2 * g.rx - 1 > h.W[0] || 2 * g.ry - 1 > h.W[1] || 2 * g.rz - 1 > h.W[2];
+        const void* fo = bounded ? (const void*)&k_search_cluster_oct<true> : (const void*)&k_search_cluster_oct<false>;
+        if (cl_lds > 65536) HPGLX_CHECK(allow_lds(fo, (int)cl_lds));
+        if (bounded)
+            hipLaunchKernelGGL(k_search_cluster_oct<true>, dim3((unsigned)std::min(ncl, 2048)), dim3(CL_BLOCK), cl_lds,
+                               st, oa);
+        else
+            hipLaunchKernelGGL(k_search_cluster_oct<false>, dim3((unsigned)std::min(ncl, 2048)), dim3(CL_BLOCK), cl_lds,
+                               st, oa);
         // its overflow: one lane per cell with a deeper LDS buffer
         const size_t cll_lds = (size_t)cap2 * CL_LIST_BLOCK * 4;
-        if (cll_lds > 65536) HPGLX_CHECK(allow_lds((const void*)&k_search_cluster_list_oct, (int)cll_lds));
-        hipLaunchKernelGGL(k_search_cluster_list_oct, dim3(1024), dim3(CL_LIST_BLOCK), cll_lds, st, oa,
-                           (const int*)ovf_cells, (const int*)n_ovf, cap2, ovf2_cells, n_ovf2);
+        const void* fl = bounded ? (const void*)&k_search_cluster_list_oct<true>
+                                 : (const void*)&k_search_cluster_list_oct<false>;
+        if (cll_lds > 65536) HPGLX_CHECK(allow_lds(fl, (int)cll_lds));
+        if (bounded)
+            hipLaunchKernelGGL(k_search_cluster_list_oct<true>, dim3(1024), dim3(CL_LIST_BLOCK), cll_lds, st, oa,
+                               (const int*)ovf_cells, (const int*)n_ovf, cap2, ovf2_cells, n_ovf2);
+        else
+            hipLaunchKernelGGL(k_search_cluster_list_oct<false>, dim3(1024), dim3(CL_LIST_BLOCK), cll_lds, st, oa,
+                               (const int*)ovf_cells, (const int*)n_ovf, cap2, ovf2_cells, n_ovf2);
         ca.cl_cells = ovf2_cells;
         ca.n_cl_cells = n_ovf2;
     }

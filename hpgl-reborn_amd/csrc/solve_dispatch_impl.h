@@ -162,7 +162,7 @@ template <int NMAX, int G64, int KIND>
 void launch_sgs_level(const SolveArgs& sa, const SgsLevelArgs& la, hipStream_t st) {
     if constexpr (KIND == 0 || KIND == 1) {  // ordinary or simple kriging (median IK is not simulated)
         // a level holds few cells, so each cell's latency counts: 8 lanes per
-        // cell from 16 neighbours up
+        // cell from 16 neighbours up, the whole wave above 32
         constexpr int GS = NMAX >= 16 ? 8 : G64;
         if (la.ncells <= 0) return;
         if (sa.otab_w)
