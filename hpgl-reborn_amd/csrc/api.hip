@@ -350,6 +350,7 @@ bool request_call(const hpglx_grid_request_t* q, GridCall& call, const int* cats
     if (const char* sv = std::getenv("HPGLX_SEARCH_HEAD")) call.search_head = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_SEARCH_SLABS")) call.search_slabs = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_CLUSTER_CAP")) call.cluster_cap = std::atoi(sv);
+    if (const char* sv = std::getenv("HPGLX_CLUSTER_CAND")) call.cluster_cand = std::atoi(sv);
     if (const char* sv = std::getenv("HPGLX_OVERLAP")) call.overlap = std::atoi(sv);
     if (!q->mask || (q->kind != KIND_MIK && !q->values)) {
         err = "values and mask are required";
@@ -379,7 +380,8 @@ std::string call_key(const GridCall& c, const DevPlan& plan) {
     key_put(k, plan.serial);
     const int ints[] = {c.kind, c.precision, c.nx, c.ny, c.nz, c.nind, c.auto_mean, c.order_relations,
                         c.refine_steps, c.search_variant, c.solve_variant, c.search_head, c.cluster_cap,
-                        c.cluster_lanes, c.blocks_per_cu, c.cats_host ? 1 : 0, c.search_slabs};
+                        c.cluster_lanes, c.blocks_per_cu, c.cats_host ? 1 : 0, c.search_slabs, c.cluster_cand,
+                        c.dyn_sched, c.overlap};
     key_put(k, ints);
     const void* ptrs[] = {c.values, c.mask, c.ind, c.zfield, c.thresholds, c.lvm_mean, c.out_val, c.out_mask,
                           c.out_class, c.out_prob, c.out_var, c.out_est64, c.out_status, c.out_count, c.out_nboff};

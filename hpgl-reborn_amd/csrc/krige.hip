@@ -596,9 +596,14 @@ __global__ __launch_bounds__(256) void k_search_cluster(ClusterSearchArgs a) {
 // rank table; 0 = not above C(0)/100), and the kept candidates go to a
 // per-lane LDS buffer as one word, rank << qbits | candidate index, which the
 // introsort orders exactly as std::sort orders the keys.  Cells with more than
-// `cap1` kept candidates are appended to an overflow list for k_search_cluster.
+// `cap1` kept candidates are appended to an overflow list for
+// k_search_cluster_list_oct, and so are all pending cells of a cluster whose
+// 27 neighbours hold more than `cand_cap` candidates.  The LDS holds the
+// sort buffers (cap1 words per lane) and cand_cap coordinates: at the
+// defaults 79 KB, two blocks (eight waves) per CU.
 constexpr int CL_BLOCK = 256;
-constexpr int kClusterSortCap = 64;  // kept candidates per lane in LDS
+constexpr int kClusterSortCap = 64;    // kept candidates per lane in LDS
+constexpr int kClusterCandCap = 3840;  // candidate coordinates per block in LDS
 constexpr int CL_GATHER = 8;         // rank gathers in flight per lane
 
 struct WordStore {
@@ -635,6 +640,7 @@ struct ClusterOctArgs {
     int wx1, wy1, wz1;             // its extents (W + 1)
     int qbits;                     // bits of the candidate index in a sort word
     int cap1;                      // sort buffer entries per lane
+    int cand_cap;                  // candidate coordinates the block's LDS holds
     int* count;
     int* nboff;
     int* ovf_cells;                // cells left for k_search_cluster
@@ -681,6 +687,18 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
         }
         __syncthreads();
         const int total = off27[27];
+        if (total > a.cand_cap) {
+            // too many candidates to stage: the cluster's pending cells take the list kernel
+            for (int c = tid; c < ncell; c += CL_BLOCK) {
+                const int lx = c % wx, r = c / wx, ly = r % wy, lz = r / wy;
+                const long long cell = ((long long)(za + lz) * g.ny + (ya + ly)) * g.nx + (xa + lx);
+                if (a.count[cell] != CNT_PENDING_CLUSTER) continue;
+                const int pos = atomicAdd(a.n_ovf, 1);
+                a.ovf_cells[pos] = (int)cell;
+            }
+            __syncthreads();  // off27 is rewritten for the next cluster
+            continue;
+        }
         for (int idx = tid; idx < total; idx += CL_BLOCK) {
             int q = 0;
             while (off27[q + 1] <= idx) ++q;
@@ -744,10 +762,11 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
 // Cluster path for a cell list (the overflow of k_search_cluster_oct): one
 // lane per cell, the candidates gathered from the 27 clusters' node lists in
 // global memory, rank keys as above, and a deeper per-lane LDS sort buffer
-// (cap2 words; CL_LIST_BLOCK lanes per block).  Cells with more kept
-// candidates stay pending for k_search_cluster.
+// (cap2 words; CL_LIST_BLOCK lanes per block: 32 KB at the default, five
+// blocks per CU).  Cells with more kept candidates stay pending for
+// k_search_cluster.
 constexpr int CL_LIST_BLOCK = 64;
-constexpr int kClusterListCap = 448;
+constexpr int kClusterListCap = 128;
 
 template <bool BOUNDED>
 __global__ __launch_bounds__(CL_LIST_BLOCK) void k_search_cluster_list_oct(ClusterOctArgs a, const int* cells,
@@ -1357,7 +1376,9 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     int qbits = 1;
     while ((1LL << qbits) < 27LL * node_cap) ++qbits;
     const int cap1 = c.cluster_cap > 0 && c.cluster_cap < kClusterSortCap ? c.cluster_cap : kClusterSortCap;
-    const size_t cl_lds = ((size_t)cap1 * CL_BLOCK + (size_t)27 * node_cap) * 4;
+    const int cand_cap = (int)std::min<long long>(
+        27LL * node_cap, c.cluster_cand > 0 && c.cluster_cand < kClusterCandCap ? c.cluster_cand : kClusterCandCap);
+    const size_t cl_lds = ((size_t)cap1 * CL_BLOCK + (size_t)cand_cap) * 4;
     const int cap2 = c.cluster_cap > 0 && c.cluster_cap < kClusterListCap ? c.cluster_cap : kClusterListCap;
     const bool oct_cluster = plan.rank_w && c.solve_variant != 1 && c.nx <= 2048 && c.ny <= 2048 && c.nz <= 1024 &&
                              qbits < 32 && (unsigned long long)h.rank_max < (1ULL << (32 - qbits)) &&
@@ -1481,6 +1502,7 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
         oa.wz1 = h.W[2] + 1;
         oa.qbits = qbits;
         oa.cap1 = cap1;
+        oa.cand_cap = cand_cap;
         oa.count = count;
         oa.nboff = nboff;
         oa.ovf_cells = ovf_cells;

@@ -97,6 +97,7 @@ struct GridCall {
     int search_head = 0;                    // packed offsets staged in LDS by the search (0 = default)
     int search_slabs = 0;                   // z slices per LDS search block (0 = the most resident waves)
     int cluster_cap = 0;                    // kept candidates per lane of the LDS cluster search (0 = default)
+    int cluster_cand = 0;                   // candidates a block of the LDS cluster search stages (0 = default)
     int cluster_lanes = 0;                  // lanes of the full-table cluster search (multiple of 256; 0 = default)
     int blocks_per_cu = 0;                  // cap on the persistent solve grids' blocks per CU (0 = all that fit)
     int dyn_sched = 0;                      // solve chunks dealt from a counter instead of a fixed stride

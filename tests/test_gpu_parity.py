@@ -98,6 +98,7 @@ VARIANT_ENVS = [
     {"HPGLX_SOLVE": "1", "HPGLX_SEARCH": "1"},   # full covariance table everywhere, global template walk
     {"HPGLX_SOLVE": "2"},                        # octant table from global memory, no LDS table
     {"HPGLX_CLUSTER_CAP": "4"},                  # cluster path mostly through the overflow list
+    {"HPGLX_CLUSTER_CAND": "16"},                # clusters with more than 16 candidates: all through the list
     {"HPGLX_SEARCH": "2"},                       # LDS-tile search with per-hit stores (no staged write-out)
     {"HPGLX_SEARCH_SLABS": "1"},                 # one z slice per search block
     {"HPGLX_SEARCH": "2", "HPGLX_SEARCH_SLABS": "4"},  # four slices per block, per-hit stores
