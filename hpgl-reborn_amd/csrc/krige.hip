@@ -109,6 +109,20 @@ __device__ __forceinline__ int cluster_of(const GridDims& g, int x, int y, int z
     return ((z / g.rz) * g.cy + (y / g.ry)) * g.cx + (x / g.rx);
 }
 
+// Appends the cells of the lanes with `take` to list[*n], in lane order, with
+// one atomic per wave (per-lane atomics on one counter serialise).  Every
+// lane of the wave calls it; blocks are one-dimensional.
+__device__ __forceinline__ void wave_append(int* list, int* n, bool take, int cell) {
+    const unsigned long long b = __ballot(take);
+    if (b == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(b) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(n, __popcll(b));
+    base = __shfl(base, leader);
+    if (take) list[base + __popcll(b & ((1ull << lane) - 1ull))] = cell;
+}
+
 // ------------------------------------------------------------ cluster build
 // One block (256 lanes) per cluster (clusterizer.cpp:14-189: cluster = cell
 // coordinate / radius, the grid split into boxes of r cells).  The block
@@ -294,6 +308,34 @@ __global__ __launch_bounds__(TILE_CELLS * SEARCH_MAX_SLABS) void k_search_tile(S
     const int bz = (blockIdx.z & 1) ? nzb - 1 - (int)(blockIdx.z >> 1) : (int)(blockIdx.z >> 1);
     const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = bz * nslab;
 
+    // The cells' paths first: informed cells are done, cells whose clusters
+    // are all within the limit take the cluster path (listed here, or left
+    // to the octant cluster search, which classifies them itself), and a
+    // block left with no cell to walk loads no box.
+    const int slab = tid / TILE_CELLS, tc = tid % TILE_CELLS;
+    const int tx = tc % TILE_X, ty = tc / TILE_X;
+    const int x = x0 + tx, y = y0 + ty, z = z0 + slab;
+    const bool inside = x < g.nx && y < g.ny && z < g.nz;
+    const long long cell = inside ? ((long long)z * g.ny + y) * g.nx + x : 0;
+    bool active = false, pending = false;
+    if (inside) {
+        // both loads issue before either is used
+        const bool ex = a.cl_nbhd_exceeded[cluster_of(g, x, y, z)] != 0;
+        const bool needs = a.kind == 2 || !a.mask[cell];
+        if (ex || a.cluster_list) {
+            if (!needs) {
+                a.count[cell] = CNT_INFORMED;
+            } else if (ex) {
+                active = true;
+            } else {
+                a.count[cell] = CNT_PENDING_CLUSTER;
+                pending = true;
+            }
+        }
+    }
+    wave_append(a.cl_cells, a.n_cl_cells, pending, (int)cell);
+    if (!__syncthreads_or(active)) return;
+
     // Box origin is padded (x0, y0, z0) = cell (x0 - hx, y0 - hy, z0 - hz).
     // Thread -> (dword column xq, row); rows advance by rstep per pass.
     {
@@ -332,25 +374,7 @@ __global__ __launch_bounds__(TILE_CELLS * SEARCH_MAX_SLABS) void k_search_tile(S
     for (int i = tid; i < head; i += nlanes) tpk[i] = a.tpack[i];
     __syncthreads();
 
-    const int slab = tid / TILE_CELLS, tc = tid % TILE_CELLS;
-    const int tx = tc % TILE_X, ty = tc / TILE_X;
-    const int x = x0 + tx, y = y0 + ty, z = z0 + slab;
-    const bool inside = x < g.nx && y < g.ny && z < g.nz;
-    const long long cell = inside ? ((long long)z * g.ny + y) * g.nx + x : 0;
     const int lcell = ((g.hz + slab) * a.ly + ty + g.hy) * a.lx + tx + g.hx;  // the cell's own box byte
-    bool active = false;
-    if (inside) {
-        const bool needs = a.kind == 2 || !box[lcell];
-        if (!needs) {
-            a.count[cell] = CNT_INFORMED;
-        } else if (a.cl_nbhd_exceeded[cluster_of(g, x, y, z)]) {
-            active = true;
-        } else {
-            a.count[cell] = CNT_PENDING_CLUSTER;
-            const int pos = atomicAdd(a.n_cl_cells, 1);
-            a.cl_cells[pos] = (int)cell;
-        }
-    }
     // Inactive lanes probe around the box position of cell (0, 0) of the
     // first slice, which keeps every template delta inside the box; their
     // bits are discarded.
@@ -457,23 +481,25 @@ __global__ __launch_bounds__(256) void k_search_tmpl(SearchArgs a) {
     // Inactive lanes probe around grid cell (0, 0, 0): every template delta
     // stays inside the padded mask; their bits are discarded.
     long long pbase = ((long long)g.hz * g.py + g.hy) * g.px + g.hx;
+    bool pending = false;
     if (cell < g.n) {
+        int x, y, z;
+        cell_xyz(g, cell, x, y, z);
+        const bool ex = a.cl_nbhd_exceeded[cluster_of(g, x, y, z)] != 0;
         const bool needs = a.kind == 2 || !a.mask[cell];
-        if (!needs) {
-            a.count[cell] = CNT_INFORMED;
-        } else {
-            int x, y, z;
-            cell_xyz(g, cell, x, y, z);
-            if (a.cl_nbhd_exceeded[cluster_of(g, x, y, z)]) {
+        if (ex || a.cluster_list) {
+            if (!needs) {
+                a.count[cell] = CNT_INFORMED;
+            } else if (ex) {
                 active = true;
                 pbase = ((long long)(z + g.hz) * g.py + (y + g.hy)) * g.px + (x + g.hx);
             } else {
                 a.count[cell] = CNT_PENDING_CLUSTER;
-                int pos = atomicAdd(a.n_cl_cells, 1);
-                a.cl_cells[pos] = (int)cell;
+                pending = true;
             }
         }
     }
+    wave_append(a.cl_cells, a.n_cl_cells, pending, (int)cell);
     int cnt = 0;
     const int K = a.kmax;
     int* out = a.nboff + (active ? cell * K : 0);
@@ -631,6 +657,9 @@ struct WordStore {
 
 struct ClusterOctArgs {
     GridDims g;
+    int kind;                      // 0 OK, 1 SK, 2 MIK (every cell kriged)
+    const unsigned char* mask;     // input mask (N)
+    int* n_cl_cells;               // cluster-path cells of the call (device counter)
     int kmax;
     const int* cl_count;
     const int* cl_nodes;
@@ -657,6 +686,7 @@ template <bool BOUNDED>
 __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs a) {
     extern __shared__ unsigned cl_lds[];
     __shared__ int off27[28];
+    __shared__ int npend;
     const GridDims& g = a.g;
     const int tid = threadIdx.x;
     unsigned* sbuf = cl_lds;                                  // cap1 x CL_BLOCK words
@@ -675,6 +705,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
         const int ncell = wx * wy * wz;
         if (ncell == 0) continue;
         if (tid == 0) {
+            npend = 0;
             int acc = 0, q = 0;
             for (int k = kz - 1; k <= kz + 1; ++k)
                 for (int j = ky - 1; j <= ky + 1; ++j)
@@ -687,16 +718,24 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
         }
         __syncthreads();
         const int total = off27[27];
+        // The cluster's cells: informed ones are done (but for median IK,
+        // which krige every cell); the rest take the cluster path.
         if (total > a.cand_cap) {
-            // too many candidates to stage: the cluster's pending cells take the list kernel
+            // too many candidates to stage: the cluster's cells take the list kernel
             for (int c = tid; c < ncell; c += CL_BLOCK) {
                 const int lx = c % wx, r = c / wx, ly = r % wy, lz = r / wy;
                 const long long cell = ((long long)(za + lz) * g.ny + (ya + ly)) * g.nx + (xa + lx);
-                if (a.count[cell] != CNT_PENDING_CLUSTER) continue;
+                if (a.kind != 2 && a.mask[cell]) {
+                    a.count[cell] = CNT_INFORMED;
+                    continue;
+                }
+                a.count[cell] = CNT_PENDING_CLUSTER;
+                atomicAdd(&npend, 1);
                 const int pos = atomicAdd(a.n_ovf, 1);
                 a.ovf_cells[pos] = (int)cell;
             }
             __syncthreads();  // off27 is rewritten for the next cluster
+            if (tid == 0) atomicAdd(a.n_cl_cells, npend);
             continue;
         }
         for (int idx = tid; idx < total; idx += CL_BLOCK) {
@@ -713,7 +752,11 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
             const int lx = c % wx, r = c / wx, ly = r % wy, lz = r / wy;
             const int x = xa + lx, y = ya + ly, z = za + lz;
             const long long cell = ((long long)z * g.ny + y) * g.nx + x;
-            if (a.count[cell] != CNT_PENDING_CLUSTER) continue;
+            if (a.kind != 2 && a.mask[cell]) {
+                a.count[cell] = CNT_INFORMED;
+                continue;
+            }
+            atomicAdd(&npend, 1);
             int m = 0;
             bool over = false;
             for (int q0 = 0; q0 < total; q0 += CL_GATHER) {
@@ -740,6 +783,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
                 }
             }
             if (over) {
+                a.count[cell] = CNT_PENDING_CLUSTER;
                 const int pos = atomicAdd(a.n_ovf, 1);
                 a.ovf_cells[pos] = (int)cell;
                 continue;
@@ -756,6 +800,7 @@ __global__ __launch_bounds__(CL_BLOCK) void k_search_cluster_oct(ClusterOctArgs 
             a.count[cell] = n | CNT_CLUSTER;
         }
         __syncthreads();
+        if (tid == 0) atomicAdd(a.n_cl_cells, npend);
     }
 }
 
@@ -1453,6 +1498,9 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     sa.nboff = nboff;
     sa.cl_cells = cl_cells;
     sa.n_cl_cells = n_cl_cells;
+    // With the octant cluster search the cells of within-limit clusters are
+    // its own (it classifies and counts them); the full-table form takes the list.
+    sa.cluster_list = oct_cluster ? 0 : 1;
     if (tiled) {
         const void* fn = staged ? (const void*)&k_search_tile<true> : (const void*)&k_search_tile<false>;
         if (tile_lds > 65536) HPGLX_CHECK(allow_lds(fn, (int)tile_lds));
@@ -1491,6 +1539,9 @@ int launch_grid(DevPlan& plan, Workspace& ws, const GridCall& c, hipStream_t st,
     if (oct_cluster) {
         ClusterOctArgs oa;
         oa.g = g;
+        oa.kind = c.kind;
+        oa.mask = c.mask;
+        oa.n_cl_cells = n_cl_cells;
         oa.kmax = K;
         oa.cl_count = cl_count;
         oa.cl_nodes = cl_nodes;

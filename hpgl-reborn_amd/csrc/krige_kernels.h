@@ -74,6 +74,8 @@ struct SearchArgs {
     int* nboff;                    // per cell kmax packed offsets
     int* cl_cells;                 // list of cluster-path cells
     int* n_cl_cells;               // its length (device counter)
+    int cluster_list;              // 1: classify cluster-path cells and list them; 0: leave the cells of
+                                   // within-limit clusters to the octant cluster search
 };
 
 struct ClusterSearchArgs {
