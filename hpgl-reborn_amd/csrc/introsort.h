@@ -39,27 +39,32 @@ struct Introsort {
     HPGLX_HD bool lt_ii(int a, int b) const { return s.key(a) > s.key(b); }
     HPGLX_HD static bool lt_vv(const Item& a, const Item& b) { return a.key > b.key; }
 
+    // The three keys are read once, before any swap (lt(x, y) is key(x) > key(y)).
     HPGLX_HD void move_median_to_first(int result, int a, int b, int c) {
-        if (lt_ii(a, b)) {
-            if (lt_ii(b, c))
+        const auto ka = s.key(a), kb = s.key(b), kc = s.key(c);
+        if (ka > kb) {
+            if (kb > kc)
                 s.swap(result, b);
-            else if (lt_ii(a, c))
+            else if (ka > kc)
                 s.swap(result, c);
             else
                 s.swap(result, a);
-        } else if (lt_ii(a, c))
+        } else if (ka > kc)
             s.swap(result, a);
-        else if (lt_ii(b, c))
+        else if (kb > kc)
             s.swap(result, c);
         else
             s.swap(result, b);
     }
 
+    // The pivot never moves while [first, last) is partitioned (it sits
+    // before first), so its key is read once.
     HPGLX_HD int unguarded_partition(int first, int last, int pivot) {
+        const auto kp = s.key(pivot);
         while (true) {
-            while (lt_ii(first, pivot)) ++first;
+            while (s.key(first) > kp) ++first;
             --last;
-            while (lt_ii(pivot, last)) --last;
+            while (kp > s.key(last)) --last;
             if (!(first < last)) return first;
             s.swap(first, last);
             ++first;
