@@ -57,17 +57,33 @@ struct Introsort {
             s.swap(result, b);
     }
 
-    // The pivot never moves while [first, last) is partitioned (it sits
-    // before first), so its key is read once.
+    // The library's loop scans from the left to an element not less than the
+    // pivot, then from the right to one not greater, swaps the two while they
+    // have not crossed, and returns the left position once they have.  Both
+    // scans read an array that no scan changes, so they can move in the same
+    // steps: each step reads both ends, moves the scanner(s) that have not
+    // stopped, and once both have stopped swaps (or returns).  The positions
+    // read, the swaps and the result are the library's; a GPU lane takes one
+    // step per LDS latency instead of one per element and scan, and lanes of
+    // a wave run one loop instead of three nested ones.  The pivot never
+    // moves meanwhile (it sits before first), so its key is read once.
     HPGLX_HD int unguarded_partition(int first, int last, int pivot) {
         const auto kp = s.key(pivot);
+        --last;
         while (true) {
-            while (s.key(first) > kp) ++first;
-            --last;
-            while (kp > s.key(last)) --last;
-            if (!(first < last)) return first;
-            s.swap(first, last);
-            ++first;
+            const Item a = s.get(first), b = s.get(last);
+            const bool left_on = a.key > kp;   // a is less than the pivot
+            const bool right_on = kp > b.key;  // the pivot is less than b
+            if (!left_on && !right_on) {
+                if (!(first < last)) return first;
+                s.put(first, b);
+                s.put(last, a);
+                ++first;
+                --last;
+            } else {
+                first += left_on ? 1 : 0;
+                last -= right_on ? 1 : 0;
+            }
         }
     }
 
