@@ -1,15 +1,22 @@
 // krige.hip -- gfx950 kernels of the grid-kriging hot path.
 //
 // Pipeline for one call (all on one stream, no host synchronisation):
-//   k_prepare       padded 0/1 mask, per-cluster informed counts
-//   k_cluster_flags per cluster: limit exceeded / any of its 27 neighbours exceeded
-//   k_tdelta        template offsets -> deltas in the padded mask
-//   k_search_tmpl   template walk, one lane per cell (cells in template-path clusters)
-//   k_cluster_*     node lists of non-exceeded clusters, ascending node order
-//   k_search_cluster cluster path, one lane per remaining cell: gather, filter,
-//                   libstdc++-identical introsort, first max_neighbours
-//   k_solve         covariance assembly + Cholesky + SK/OK solve + combine,
-//                   one group of G lanes per cell (lane r owns row r)
+//   k_cluster_build    per cluster: padded 0/1 mask, informed count, ascending
+//                      node list; clears the call's counters
+//   k_cluster_flags    per cluster: does any of its 27 neighbours (itself
+//                      included) exceed the limit (then its cells take the
+//                      template walk)
+//   k_search_tile      cell paths, then the template walk in LDS tiles, one
+//                      lane per cell (k_search_tmpl: the same from global memory)
+//   k_search_cluster_oct, _list_oct, k_search_cluster
+//                      cluster path: gather, rank, libstdc++-identical
+//                      introsort, first max_neighbours
+//   k_bucket_count, _scan, _scatter
+//                      the work list: cells by path and neighbour count
+//   k_solve_sorted_lds, k_solve_sorted (solve.h)
+//                      covariance assembly + Cholesky + SK/OK/MIK solve +
+//                      combine, G lanes per cell, persistent over the list;
+//                      k_solve_fallback re-solves mixed-precision fallbacks
 //
 // Reference semantics reproduced (paths relative to the reference repository):
 //   template walk   src/geo_bsd/hpgl/sugarbox_neighbour_lookup.h:37-62
