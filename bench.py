@@ -277,6 +277,9 @@ def side_workload(name, args):
             z = torch.from_numpy(wl.values).to(dev)
             call = lambda prec: D.indicator_cdf(z, m, wl.shape, wl.thresholds, wl.levels, wl.radiuses, wl.k, model,
                                                 precision=prec, variance=False)
+            tune = lambda prec: D.autotune(2, wl.shape, wl.radiuses, wl.k, model, m, zfield=z, thresholds=wl.thresholds,
+                                           marginal=wl.levels, precision=prec, order_relations=True, variance=False,
+                                           reps=2)
             units = wl.mask.size
             ind = [(wl.values <= t).astype(np.uint8) for t in wl.thresholds]
             ref = lambda: [O.ref_median_ik(cov, wl.radiuses, wl.k, wl.shape, i, wl.mask, (1 - p, p), threads=th)
@@ -286,6 +289,8 @@ def side_workload(name, args):
             v = torch.from_numpy(wl.values).to(dev)
             call = lambda prec: D.krige(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
                                         variance=False)
+            tune = lambda prec: D.autotune(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
+                                           variance=False, reps=2)
             units = int((wl.mask == 0).sum())
             ref = lambda: O.ref_ordinary_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, threads=th)
             sample = "reference ordinary_kriging, full grid"
@@ -294,6 +299,8 @@ def side_workload(name, args):
             lm = torch.from_numpy(wl.lvm_mean).to(dev)
             call = lambda prec: D.lvm_kriging(v, m, lm, wl.shape, wl.radiuses, wl.k, model, precision=prec,
                                               variance=False)
+            tune = lambda prec: D.autotune(1, wl.shape, wl.radiuses, wl.k, model, m, values=v, lvm_mean=lm,
+                                           precision=prec, variance=False, reps=2)
             units = int((wl.mask == 0).sum())
             ref = lambda: O.ref_lvm_kriging(cov, wl.radiuses, wl.k, wl.shape, wl.values, wl.mask, wl.lvm_mean,
                                             threads=th)
@@ -318,9 +325,8 @@ def side_workload(name, args):
         flops = float((flops_mik(cnt[main], len(wl.thresholds)) if name == "mik" else flops_ok(cnt[main])).sum())
         kern_ms, tuning = {}, {}
         for prec in ("fp64", "mixed"):
-            if name == "wells" and not args.no_autotune:
-                _, tuning[prec] = D.autotune(0, wl.shape, wl.radiuses, wl.k, model, m, values=v, precision=prec,
-                                             variance=False, reps=2)
+            if not args.no_autotune:
+                _, tuning[prec] = tune(prec)
             per[prec] = _timed(lambda: call(prec), args.steps, args.warmup, sync)
             D.set_profiling(True)
             _timed(lambda: call(prec), args.steps, 0, sync)
@@ -337,7 +343,7 @@ def side_workload(name, args):
                                                   PEAK_FP64_TFLOPS if best == "fp64" else PEAK_FP32_TFLOPS,
                                                   kernel.format("fp64" if best == "fp64" else "mixed")),
                     main_kernel_cells=int(main.sum()))
-        if name == "wells":
+        if tuning:
             line["autotune"] = tuning
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = _cpu("reference", ref, units, "kriged cells/s", sample)
