@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -167,6 +168,8 @@ struct Library {
     std::map<std::string, std::unique_ptr<DevPlan>> na_plans;
     std::map<std::string, unsigned long long> na_use;
     hipStream_t io_stream = nullptr;
+    char* pin = nullptr;         // page-locked staging of the host-array entry points (HostIO)
+    size_t pin_cap = 0;
     IncText text;                // pinned host copy of the last property file read or written
     unsigned long long plan_serial = 0;
     // Graphs of repeated grid calls: a call equal to the previous one (plan,
@@ -742,6 +745,15 @@ int shape_volume(const hpgl_shape_t* s, std::string& err) {
 }
 
 // Host-buffer driver shared by the reference entry points.
+//
+// Host arrays travel through a page-locked buffer of the library's: the
+// runtime copies pageable memory through staging of its own at about half
+// the DMA rate (C2's 2.5 MB of inputs: 0.14 ms, against 0.06 ms from
+// page-locked memory and 0.04 ms for the CPU to copy them there).  Inputs go
+// in chunks, each chunk's DMA running while the CPU copies the next, every
+// array into a region of its own (a queued DMA still reads it); outputs come
+// back into the buffer's start (queued after every input DMA and kernel of
+// the call) and are copied out once the stream is done.
 struct HostIO {
     int n = 0;
     float* d_val = nullptr;
@@ -750,7 +762,82 @@ struct HostIO {
     float* d_out = nullptr;
     unsigned char* d_out_mask = nullptr;
     unsigned char* d_out_class = nullptr;
+    char* stage = nullptr;  // the page-locked buffer and its bytes in use by this call's inputs
+    size_t staged = 0, stage_cap = 0;
 };
+
+// The page-locked buffer, at least `bytes` long.  Every host-array entry
+// point waits for its outputs, so no copy through the buffer is in flight
+// here unless a call failed midway; a buffer is freed only once its stream
+// has drained.
+char* pinned_stage(Library& L, size_t bytes, std::string& err) {
+    if (bytes <= L.pin_cap) return L.pin;
+    if (L.pin) {
+        if (L.io_stream) (void)hipStreamSynchronize(L.io_stream);
+        (void)hipHostFree(L.pin);
+    }
+    L.pin = nullptr;
+    L.pin_cap = 0;
+    const size_t cap = std::max(bytes, (size_t)1 << 20);
+    if (hipHostMalloc((void**)&L.pin, cap, hipHostMallocDefault) != hipSuccess) {
+        L.pin = nullptr;
+        err = "hipHostMalloc failed";
+        return nullptr;
+    }
+    L.pin_cap = cap;
+    return L.pin;
+}
+
+// Queues the copy of `bytes` from host memory to the device through the next
+// region of the call's staging buffer.
+bool upload(HostIO& io, void* dev, const void* host, size_t bytes, hipStream_t st, std::string& err) {
+    constexpr size_t kChunk = (size_t)512 << 10;
+    const size_t at = (io.staged + 255) / 256 * 256;
+    if (at + bytes > io.stage_cap) {
+        err = "host staging buffer too small";
+        return false;
+    }
+    for (size_t off = 0; off < bytes; off += kChunk) {
+        const size_t len = std::min(kChunk, bytes - off);
+        std::memcpy(io.stage + at + off, (const char*)host + off, len);
+        if (hipMemcpyAsync((char*)dev + off, io.stage + at + off, len, hipMemcpyHostToDevice, st) != hipSuccess) {
+            err = "host to device copy failed";
+            return false;
+        }
+    }
+    io.staged = at + bytes;
+    return true;
+}
+
+// Copies outputs back: device to the staging buffer, wait for the stream,
+// then to the host arrays.  Each entry is (host destination, device source,
+// bytes).
+struct Download {
+    void* host;
+    const void* dev;
+    size_t bytes;
+};
+bool download(HostIO& io, std::initializer_list<Download> outs, hipStream_t st, std::string& err) {
+    size_t at = 0;
+    for (const Download& d : outs) {
+        if (at + d.bytes > io.stage_cap ||
+            hipMemcpyAsync(io.stage + at, d.dev, d.bytes, hipMemcpyDeviceToHost, st) != hipSuccess) {
+            err = "device to host copy failed";
+            return false;
+        }
+        at = (at + d.bytes + 255) / 256 * 256;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        err = "device to host copy failed";
+        return false;
+    }
+    at = 0;
+    for (const Download& d : outs) {
+        std::memcpy(d.host, io.stage + at, d.bytes);
+        at = (at + d.bytes + 255) / 256 * 256;
+    }
+    return true;
+}
 
 bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned char* mask, const unsigned char* ind,
               std::string& err) {
@@ -775,13 +862,15 @@ bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned ch
         err = "hipStreamWaitEvent failed";
         return false;
     }
-    if (hipMemcpyAsync(io.d_mask, mask, n, hipMemcpyHostToDevice, L.io_stream) != hipSuccess ||
-        (val && hipMemcpyAsync(io.d_val, val, (size_t)n * 4, hipMemcpyHostToDevice, L.io_stream) != hipSuccess) ||
-        (ind && hipMemcpyAsync(io.d_ind, ind, n, hipMemcpyHostToDevice, L.io_stream) != hipSuccess)) {
-        err = "host to device copy failed";
-        return false;
-    }
-    return true;
+    // room for every input (mask, values or indicators, and a second value
+    // array for the LVM mean) and, from the start, the outputs
+    io.stage_cap = (size_t)n * 10 + 4 * 256;
+    io.stage = pinned_stage(L, io.stage_cap, err);
+    io.staged = 0;
+    if (!io.stage) return false;
+    return upload(io, io.d_mask, mask, n, L.io_stream, err) &&
+           (!val || upload(io, io.d_val, val, (size_t)n * 4, L.io_stream, err)) &&
+           (!ind || upload(io, io.d_ind, ind, n, L.io_stream, err));
 }
 
 bool sgs_transforms(const hpgl_non_parametric_cdf_t* cdf) {
@@ -1000,12 +1089,9 @@ void hpgl_ordinary_kriging(hpgl_cont_masked_array_t* in, hpgl_ok_params_t* p, hp
         q.out_mask = io.d_out_mask;
         q.refine_steps = -1;
         if (run_request(L, &q, L.io_stream, e) != 0) return -1;
-        if (hipMemcpyAsync(out->m_data, io.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipMemcpyAsync(out->m_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipStreamSynchronize(L.io_stream) != hipSuccess) {
-            e = "device to host copy failed";
+        if (!download(io, {{out->m_data, io.d_out, (size_t)n * 4}, {out->m_mask, io.d_out_mask, (size_t)n}},
+                      L.io_stream, e))
             return -1;
-        }
         return 0;
     });
 }
@@ -1048,12 +1134,9 @@ void hpgl_simple_kriging(float* input_data, unsigned char* input_mask, hpgl_shap
         q.out_mask = io.d_out_mask;
         q.refine_steps = -1;
         if (run_request(L, &q, L.io_stream, e) != 0) return -1;
-        if (hipMemcpyAsync(output_data, io.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipMemcpyAsync(output_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipStreamSynchronize(L.io_stream) != hipSuccess) {
-            e = "device to host copy failed";
+        if (!download(io, {{output_data, io.d_out, (size_t)n * 4}, {output_mask, io.d_out_mask, (size_t)n}},
+                      L.io_stream, e))
             return -1;
-        }
         return 0;
     });
 }
@@ -1170,12 +1253,9 @@ void hpgl_median_ik(hpgl_ind_masked_array_t* in, hpgl_median_ik_params_t* p, hpg
         q.out_mask = io.d_out_mask;
         q.refine_steps = -1;
         if (run_request(L, &q, L.io_stream, e) != 0) return -1;
-        if (hipMemcpyAsync(out->m_data, io.d_out_class, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipMemcpyAsync(out->m_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipStreamSynchronize(L.io_stream) != hipSuccess) {
-            e = "device to host copy failed";
+        if (!download(io, {{out->m_data, io.d_out_class, (size_t)n}, {out->m_mask, io.d_out_mask, (size_t)n}},
+                      L.io_stream, e))
             return -1;
-        }
         return 0;
     });
 }
@@ -1196,11 +1276,7 @@ void hpgl_lvm_kriging(float* input_data, unsigned char* input_mask, hpgl_shape_t
         HostIO io;
         if (!stage_in(L, io, n, input_data, input_mask, nullptr, e)) return -1;
         float* d_mean = (float*)L.ws.get("io_lvm_mean", (size_t)n * 4, e);
-        if (!d_mean) return -1;
-        if (hipMemcpyAsync(d_mean, mean_data, (size_t)n * 4, hipMemcpyHostToDevice, L.io_stream) != hipSuccess) {
-            e = "host to device copy failed";
-            return -1;
-        }
+        if (!d_mean || !upload(io, d_mean, mean_data, (size_t)n * 4, L.io_stream, e)) return -1;
         hpglx_grid_request_t q;
         std::memset(&q, 0, sizeof q);
         q.kind = KIND_SK;
@@ -1222,12 +1298,9 @@ void hpgl_lvm_kriging(float* input_data, unsigned char* input_mask, hpgl_shape_t
         q.out_mask = io.d_out_mask;
         q.refine_steps = -1;
         if (run_request(L, &q, L.io_stream, e) != 0) return -1;
-        if (hipMemcpyAsync(output_data, io.d_out, (size_t)n * 4, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipMemcpyAsync(output_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipStreamSynchronize(L.io_stream) != hipSuccess) {
-            e = "device to host copy failed";
+        if (!download(io, {{output_data, io.d_out, (size_t)n * 4}, {output_mask, io.d_out_mask, (size_t)n}},
+                      L.io_stream, e))
             return -1;
-        }
         return 0;
     });
 }
@@ -1249,12 +1322,9 @@ void hpgl_indicator_kriging(hpgl_ind_masked_array_t* in, hpgl_ind_masked_array_t
         if (run_indicator(L, params, indicator_count, in->m_shape.m_data, io.d_ind, io.d_mask, io.d_out_class,
                           io.d_out_mask, nullptr, L.default_precision, L.io_stream, e) != 0)
             return -1;
-        if (hipMemcpyAsync(out->m_data, io.d_out_class, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipMemcpyAsync(out->m_mask, io.d_out_mask, n, hipMemcpyDeviceToHost, L.io_stream) != hipSuccess ||
-            hipStreamSynchronize(L.io_stream) != hipSuccess) {
-            e = "device to host copy failed";
+        if (!download(io, {{out->m_data, io.d_out_class, (size_t)n}, {out->m_mask, io.d_out_mask, (size_t)n}},
+                      L.io_stream, e))
             return -1;
-        }
         return 0;
     });
 }
