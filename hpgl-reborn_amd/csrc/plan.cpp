@@ -180,9 +180,12 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
     // value inside the +-radius box, so the unclipped model is used.
     const int rx = p.r[0], ry = p.r[1], rz = p.r[2];
     const int xd = 2 * rx + 1, yd = 2 * ry + 1;
-    std::vector<double> data;
-    data.reserve((size_t)xd * yd * (2 * rz + 1));
-    struct V { int x, y, z; };
+    (void)xd;
+    (void)yd;
+    // Each entry carries its key (the covariance the reference's comparator
+    // looks up), so the comparisons, and std::sort's permutation, are the
+    // reference's.
+    struct V { int x, y, z; double key; };
     std::vector<V> vec;
     const double zero_cov = model_cov(cov, p.transform, 0, 0, 0);
     const double threshold = zero_cov / 100;
@@ -190,15 +193,15 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
         for (int y = -ry; y <= ry; ++y)
             for (int x = -rx; x <= rx; ++x) {
                 double v = model_cov(cov, p.transform, 0.0 - x, 0.0 - y, 0.0 - z);
-                data.push_back(v);
-                if (v > threshold) vec.push_back(V{-x, -y, -z});
+                if (v > threshold) vec.push_back(V{-x, -y, -z, v});
             }
-    auto key = [&](const V& o) {
-        return data[(size_t)(o.z + rz) * yd * xd + (size_t)(o.y + ry) * xd + (size_t)(o.x + rx)];
-    };
-    std::sort(vec.begin(), vec.end(), [&](const V& a, const V& b) { return key(a) > key(b); });
+    std::sort(vec.begin(), vec.end(), [](const V& a, const V& b) { return a.key > b.key; });
+    // The reference then cuts the sorted list at the first entry below the
+    // threshold, evaluating C(-d) (covariance_field.h:93-103); C(-d) rounds
+    // as C(d) (the transform is linear and only squares enter), which is the
+    // kept key, above the threshold by construction: the cut keeps all.
     for (size_t i = 0; i < vec.size(); ++i)
-        if (model_cov(cov, p.transform, 0.0 - vec[i].x, 0.0 - vec[i].y, 0.0 - vec[i].z) < threshold) {
+        if (vec[i].key < threshold) {
             vec.resize(i);
             break;
         }
@@ -211,8 +214,6 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
         p.tdz[i] = vec[i].z;
     }
     p.limit = (long long)(vec.size() / 250);
-    data.clear();
-    data.shrink_to_fit();
 
     // Extents: H from the template; D adds the cluster path's reach, at most
     // 2r - 1 (the 27 clusters of r cells) and bounded by the covariance.
@@ -245,62 +246,66 @@ bool build_plan(PlanHost& p, Kind kind, const CovParams& cov, const int radiuses
     }
     p.ctab.assign((size_t)sx * sy * sz, 0.0);
     const bool clip = kind != KIND_OK;
-    auto fill_planes = [&](int z0, int z1) {
-        for (int z = z0; z < z1; ++z) {
-            const int dz = z - p.W[2];
-            for (int y = 0; y < sy; ++y) {
-                const int dy = y - p.W[1];
-                for (int x = 0; x < sx; ++x) {
-                    const int dx = x - p.W[0];
-                    double v;
-                    if (clip && (dx < -rx || dx > rx || dy < -ry || dy > ry || dz < -rz || dz > rz))
-                        v = 0;
-                    else
-                        v = model_cov(cov, p.transform, (double)dx, (double)dy, (double)dz);
-                    p.ctab[((size_t)z * sy + y) * sx + x] = v;
+    auto entry = [&](int dx, int dy, int dz) {
+        if (clip && (dx < -rx || dx > rx || dy < -ry || dy > ry || dz < -rz || dz > rz)) return 0.0;
+        return model_cov(cov, p.transform, (double)dx, (double)dy, (double)dz);
+    };
+    // Planes [0, nz) split over threads.
+    auto threaded = [](int nz, size_t work, auto&& planes) {
+        unsigned nt = std::thread::hardware_concurrency();
+        nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+        if (work < 65536) nt = 1;
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nt; ++t) {
+            const int z0 = (int)((long long)nz * t / nt), z1 = (int)((long long)nz * (t + 1) / nt);
+            pool.emplace_back(planes, z0, z1);
+        }
+        for (auto& th : pool) th.join();
+    };
+    // With all angles zero the transform is diagonal but for signed zeros:
+    // each component of T d is its one nonzero product (or a signed zero) and
+    // only squares enter, so model_cov rounds identically for every sign of
+    // each offset component (and so does the clipping box).  Then the octant
+    // d >= 0 is evaluated and mirrored; it is also the octant table.
+    const int wx1 = p.W[0] + 1, wy1 = p.W[1] + 1, wz1 = p.W[2] + 1;
+    const size_t wn = (size_t)wx1 * wy1 * wz1;
+    const bool even = cov.angles[0] == 0 && cov.angles[1] == 0 && cov.angles[2] == 0;
+    std::vector<double> oct;
+    if (even) {
+        oct.resize(wn);
+        threaded(wz1, wn, [&](int z0, int z1) {
+            for (int z = z0; z < z1; ++z)
+                for (int y = 0; y < wy1; ++y)
+                    for (int x = 0; x < wx1; ++x) oct[((size_t)z * wy1 + y) * wx1 + x] = entry(x, y, z);
+        });
+        threaded(sz, (size_t)sx * sy * sz, [&](int z0, int z1) {
+            for (int z = z0; z < z1; ++z) {
+                const int az = std::abs(z - p.W[2]);
+                for (int y = 0; y < sy; ++y) {
+                    const double* orow = oct.data() + ((size_t)az * wy1 + std::abs(y - p.W[1])) * wx1;
+                    double* row = p.ctab.data() + ((size_t)z * sy + y) * sx;
+                    for (int x = 0; x < sx; ++x) row[x] = orow[std::abs(x - p.W[0])];
                 }
             }
-        }
-    };
-    unsigned nt = std::thread::hardware_concurrency();
-    nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
-    if ((size_t)sx * sy * sz < 65536) nt = 1;
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < nt; ++t) {
-        const int z0 = (int)((long long)sz * t / nt), z1 = (int)((long long)sz * (t + 1) / nt);
-        pool.emplace_back(fill_planes, z0, z1);
+        });
+    } else {
+        threaded(sz, (size_t)sx * sy * sz, [&](int z0, int z1) {
+            for (int z = z0; z < z1; ++z)
+                for (int y = 0; y < sy; ++y)
+                    for (int x = 0; x < sx; ++x)
+                        p.ctab[((size_t)z * sy + y) * sx + x] = entry(x - p.W[0], y - p.W[1], z - p.W[2]);
+        });
     }
-    for (auto& th : pool) th.join();
 
-    // Octant tables.  With all angles zero the covariance is even in each
-    // offset component; the symmetry is checked bit for bit over the whole
-    // +-W box before any octant table is used.
+    // Octant tables (all angles zero).
     p.octant = false;
     p.otab.clear();
     p.otab_w.clear();
     p.rank_w.clear();
     for (int i = 0; i < 3; ++i) p.E[i] = 2 * p.H[i];
-    const int wx1 = p.W[0] + 1, wy1 = p.W[1] + 1, wz1 = p.W[2] + 1;
-    const size_t wn = (size_t)wx1 * wy1 * wz1;
-    if (cov.angles[0] == 0 && cov.angles[1] == 0 && cov.angles[2] == 0 && wn <= kOctantWideMaxEntries) {
-        auto at = [&](int x, int y, int z) {
-            return p.ctab[((size_t)(z + p.W[2]) * sy + (y + p.W[1])) * sx + (x + p.W[0])];
-        };
-        p.otab_w.resize(wn);
-        bool same = true;
-        for (int z = 0; z < wz1 && same; ++z)
-            for (int y = 0; y < wy1 && same; ++y)
-                for (int x = 0; x < wx1 && same; ++x) {
-                    const double v = at(x, y, z);
-                    p.otab_w[((size_t)z * wy1 + y) * wx1 + x] = v;
-                    for (int m = 1; m < 8; ++m) {
-                        const double u = at(m & 1 ? -x : x, m & 2 ? -y : y, m & 4 ? -z : z);
-                        if (std::memcmp(&u, &v, sizeof v) != 0) same = false;
-                    }
-                }
-        if (!same) {
-            p.otab_w.clear();
-        } else {
+    if (even && wn <= kOctantWideMaxEntries) {
+        p.otab_w = std::move(oct);
+        {
             p.octant = true;
             // Dense ranks of the keys above C(0)/100 (0 = filtered out): the
             // cluster path's std::sort comparator (key a > key b) and its
