@@ -679,7 +679,7 @@ struct ClusterOctArgs {
     int cand_cap;                  // candidate coordinates the block's LDS holds
     int* count;
     int* nboff;
-    int* ovf_cells;                // cells left for k_search_cluster
+    int* ovf_cells;                // cells left for k_search_cluster_list_oct
     int* n_ovf;
 };
 
