@@ -174,6 +174,45 @@ def test_table_address_forms(orc, radius):
     assert np.abs(gm["est64"][kr] - o["est64"][kr]).max() <= 1e-6
 
 
+def _wells_with_dense_corner():
+    """The wells workload at 64x64x16 plus 8 % random data in the x, y < 32
+    corner: the corner's clusters exceed the limit, so their neighbourhoods
+    take the template walk while the rest takes the cluster path (12 % of the
+    searched cells), in tiles that hold both."""
+    from geo_bsd import synthetic as S
+    wl = S.ok_wells_c4(0.25)
+    nx, ny, nz = wl.shape
+    rng = np.random.default_rng(7)
+    m = wl.mask.reshape((nz, ny, nx)).copy()
+    m[:, :32, :32] |= (rng.random((nz, 32, 32)) < 0.08).astype(np.uint8)
+    mask = m.reshape(-1)
+    field = S.moving_average_field(rng, wl.shape, (9, 9, 3), 0.2, 0.05)
+    return S.Workload(name="wells_dense_corner", kind="ok", shape=wl.shape, mask=mask,
+                      values=np.where(mask == 1, field, -99).astype(np.float32), cov=wl.cov, radiuses=wl.radiuses,
+                      k=wl.k)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mixed_search_paths(orc, precision):
+    """Both search paths in one call: each cell's path, neighbour list and
+    estimate as the oracle's (the template walk classifies the cells of
+    over-limit neighbourhoods, the octant cluster search the rest)."""
+    wl = _wells_with_dense_corner()
+    o = _oracle(orc, wl)
+    g = _device(wl, precision)
+    searched = o["status"] != orc.ST_INFORMED
+    frac = o["path"][searched].mean()
+    assert 0.05 < frac < 0.95, frac
+    assert np.array_equal(g["cluster_path"][searched], o["path"][searched].astype(bool))
+    assert np.array_equal(g["count"][searched], o["count"][searched])
+    nodes = _node_lists(wl, g["count"], g["nboff"])
+    assert np.array_equal(nodes[searched], o["nb"][searched].astype(np.int64))
+    assert np.array_equal(g["status"], o["status"].astype(np.uint8))
+    kr = o["status"] == orc.ST_KRIGED
+    tol = 1e-12 if precision == "fp64" else 1e-9
+    assert np.allclose(g["est64"][kr], o["est64"][kr], rtol=0, atol=tol)
+
+
 @pytest.mark.parametrize("name,scale", CASES)
 def test_estimates_fp64(orc, name, scale):
     wl = _wl(name, scale)
