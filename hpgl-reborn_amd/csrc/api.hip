@@ -753,7 +753,10 @@ int shape_volume(const hpgl_shape_t* s, std::string& err) {
 // in chunks, each chunk's DMA running while the CPU copies the next, every
 // array into a region of its own (a queued DMA still reads it); outputs come
 // back into the buffer's start (queued after every input DMA and kernel of
-// the call) and are copied out once the stream is done.
+// the call) and are copied out once the stream is done.  Calls that would
+// need more than kPinnedMax bytes of it (or when page-locking fails) copy
+// straight from and to the caller's arrays.
+constexpr size_t kPinnedMax = (size_t)256 << 20;
 struct HostIO {
     int n = 0;
     float* d_val = nullptr;
@@ -780,6 +783,7 @@ char* pinned_stage(Library& L, size_t bytes, std::string& err) {
     L.pin_cap = 0;
     const size_t cap = std::max(bytes, (size_t)1 << 20);
     if (hipHostMalloc((void**)&L.pin, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();  // not left for a later check to report
         L.pin = nullptr;
         err = "hipHostMalloc failed";
         return nullptr;
@@ -789,8 +793,15 @@ char* pinned_stage(Library& L, size_t bytes, std::string& err) {
 }
 
 // Queues the copy of `bytes` from host memory to the device through the next
-// region of the call's staging buffer.
+// region of the call's staging buffer (without one, straight).
 bool upload(HostIO& io, void* dev, const void* host, size_t bytes, hipStream_t st, std::string& err) {
+    if (!io.stage) {
+        if (hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+            err = "host to device copy failed";
+            return false;
+        }
+        return true;
+    }
     constexpr size_t kChunk = (size_t)512 << 10;
     const size_t at = (io.staged + 255) / 256 * 256;
     if (at + bytes > io.stage_cap) {
@@ -818,6 +829,18 @@ struct Download {
     size_t bytes;
 };
 bool download(HostIO& io, std::initializer_list<Download> outs, hipStream_t st, std::string& err) {
+    if (!io.stage) {
+        for (const Download& d : outs)
+            if (hipMemcpyAsync(d.host, d.dev, d.bytes, hipMemcpyDeviceToHost, st) != hipSuccess) {
+                err = "device to host copy failed";
+                return false;
+            }
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            err = "device to host copy failed";
+            return false;
+        }
+        return true;
+    }
     size_t at = 0;
     for (const Download& d : outs) {
         if (at + d.bytes > io.stage_cap ||
@@ -865,9 +888,13 @@ bool stage_in(Library& L, HostIO& io, int n, const float* val, const unsigned ch
     // room for every input (mask, values or indicators, and a second value
     // array for the LVM mean) and, from the start, the outputs
     io.stage_cap = (size_t)n * 10 + 4 * 256;
-    io.stage = pinned_stage(L, io.stage_cap, err);
+    io.stage = nullptr;
     io.staged = 0;
-    if (!io.stage) return false;
+    const char* sv = std::getenv("HPGLX_HOST_STAGING");  // "0": straight copies
+    if (io.stage_cap <= kPinnedMax && !(sv && std::atoi(sv) == 0)) {
+        std::string perr;
+        io.stage = pinned_stage(L, io.stage_cap, perr);  // without it the copies go straight
+    }
     return upload(io, io.d_mask, mask, n, L.io_stream, err) &&
            (!val || upload(io, io.d_val, val, (size_t)n * 4, L.io_stream, err)) &&
            (!ind || upload(io, io.d_ind, ind, n, L.io_stream, err));

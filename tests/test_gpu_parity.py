@@ -282,6 +282,33 @@ def test_geo_ordinary_kriging_matches_oracle(orc):
     assert np.array_equal(prop.data.reshape(-1, order="F"), wl.values)
 
 
+@pytest.mark.parametrize("name", ["ok", "mik"])
+def test_host_arrays_staged_or_straight(monkeypatch, name):
+    """Host arrays copied through the library's page-locked buffer, or
+    straight from and to the caller's memory: the same bits."""
+    import geo_bsd
+    wl = _wl(name, 0.3)
+    nx, ny, nz = wl.shape
+    grid = geo_bsd.SugarboxGrid(nx, ny, nz)
+
+    def call():
+        if name == "ok":
+            prop = geo_bsd.ContProperty(wl.values.reshape((nx, ny, nz), order="F"),
+                                        wl.mask.reshape((nx, ny, nz), order="F"))
+            out = geo_bsd.ordinary_kriging(prop, grid, wl.radiuses, wl.k, _model(wl))
+        else:
+            ind = (wl.values <= wl.thresholds[0]).astype(np.uint8).reshape((nx, ny, nz), order="F")
+            prop = geo_bsd.IndProperty(ind, wl.mask.reshape((nx, ny, nz), order="F"), 2)
+            out = geo_bsd.median_ik(prop, grid, (0.5, 0.5), wl.radiuses, wl.k, _model(wl))
+        return out.data.copy(), out.mask.copy()
+
+    staged = call()
+    monkeypatch.setenv("HPGLX_HOST_STAGING", "0")
+    straight = call()
+    for a, b in zip(staged, straight):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
 def test_geo_simple_kriging_auto_mean_matches_oracle(orc):
     import geo_bsd
     wl = _wl("sk", 0.6)
