@@ -371,8 +371,9 @@ constexpr int OFF_BIAS = 1024;
 // liveness test is compiled.
 template <int NMAX, int G, bool MIXED, int KIND, bool LTAB, bool PACK, bool FULL>
 __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell, int t, int n, const double* ltab,
-                                             unsigned ox, unsigned oxy, double (&w)[NMAX / G], double (&bq)[NMAX / G],
-                                             int (&dnode)[NMAX / G], double& mu, int& status) {
+                                             unsigned ox, unsigned oxy, double (&w)[NMAX / G],
+                                             double (&bq_out)[NMAX / G], double& wb, int (&dnode)[NMAX / G],
+                                             double& mu, int& status) {
     using T = typename std::conditional<MIXED, float, double>::type;
     typedef Chol<NMAX, G, T> CH;
     constexpr int S = CH::S;
@@ -423,6 +424,11 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
     };
     T L[CH::TOT];
     T D[S];
+    // b, the covariances of my rows with the target: the mixed form keeps it
+    // here and returns w.b; the FP64 form hands it to the caller, which
+    // forms w.b (each allocates its registers better that way, measured).
+    double bq_here[S];
+    double(&bq)[S] = MIXED ? bq_here : bq_out;
 #pragma unroll
     for (int c = 0; c < NMAX; ++c) {
         int tcol = 0;
@@ -581,6 +587,12 @@ __device__ __forceinline__ bool cell_weights(const SolveArgs& a, long long cell,
             for (int s = 0; s < S; ++s) w[s] += (double)f[s];
         }
     }
+    if (MIXED) {
+        double pwb = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) pwb += w[s] * bq[s];
+        wb = gsum<G>(pwb);
+    }
     return true;
 }
 
@@ -600,7 +612,7 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     int status = ST_KRIGED;
     double w[S], bq[S];
     int dnode[S];
-    double mu = 0.0;
+    double mu = 0.0, wb = 0.0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         w[s] = 0.0;
@@ -609,19 +621,22 @@ __device__ __forceinline__ void solve_cell(const SolveArgs& a, long long cell, i
     }
     if (n == 0) {
         status = ST_NO_NEIGHBOURS;
-    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB, PACK, FULL>(a, cell, t, n, ltab, ox, oxy, w, bq, dnode, mu,
-                                                                     status)) {
+    } else if (!cell_weights<NMAX, G, MIXED, KIND, LTAB, PACK, FULL>(a, cell, t, n, ltab, ox, oxy, w, bq, wb, dnode,
+                                                                     mu, status)) {
         return;  // queued for the FP64 form
     }
     if (status != ST_KRIGED) {
 #pragma unroll
         for (int s = 0; s < S; ++s) w[s] = 0.0;
         mu = 0.0;
+        wb = 0.0;
     }
-    double pwb = 0.0;
+    if (!MIXED) {
+        double pwb = 0.0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) pwb += w[s] * bq[s];
-    const double wb = gsum<G>(pwb);
+        for (int s = 0; s < S; ++s) pwb += w[s] * bq[s];
+        wb = gsum<G>(pwb);
+    }
 
     if (KIND != 2) {
         const double mean = (KIND == 1) ? (a.mean_dev ? a.mean_dev[0] : a.mean) : 0.0;
