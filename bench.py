@@ -50,7 +50,7 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 (packed)
 # known size (--calibrate-copy; profiles/hbm_traffic_ok_100x100x50.md):
 # FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, KiB x 1024.
 TRAFFIC_BYTES = {
-    "k_solve_sorted_lds<24,4,mixed,OK>": 72.70e6,
+    "k_solve_sorted_lds<24,4,mixed,OK>": 72.86e6,
     "k_solve_sorted_lds<24,4,fp64,OK>": 72.80e6,
 }
 # Algorithmic bytes of one solve launch: per kriged cell its work-list entry
